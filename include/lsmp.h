@@ -1,0 +1,131 @@
+/*
+ * lsmp.h -- C ABI of the MI355X (gfx950) level-set transport library.
+ *
+ * The library replaces the reference's hot path: TVD-RK3 CRWENO5 advection,
+ * PDE reinitialisation and the global mass-preserving correction of the MPLS
+ * method.  In the reference these are argument-less internal Fortran
+ * subroutines acting on the globals of module `all` (Vortex/src/main.f90:1-8);
+ * the entry points below are what a Fortran driver binds through
+ * ISO_C_BINDING in their place (one entry point per reference call site).
+ *
+ * Conventions
+ *  - Host arrays are FP64 boxes in the reference storage contract: column-major
+ *    (i fastest), extents (1-ghc : n+ghc) per direction with ghc = 3; in 2D the
+ *    z extent is 7 planes (Disk/lib/data/branches.f90:134-136).
+ *  - The library owns every device buffer; phi stays resident in HBM between
+ *    calls.  Host pointers are never retained past a call.
+ *  - Every function returns 0 on success or a negative LSMP_E* code; the text
+ *    is available from lsmp_error_string().  The reference stops the program
+ *    on the corresponding conditions (Disk/lib/data/tree.f90:304-306,
+ *    branches.f90:129-132, 314-315).
+ *  - One host thread per handle; calls on a handle are not re-entrant, which is
+ *    the reference's own call pattern.
+ */
+#ifndef LSMP_H
+#define LSMP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSMP_OK 0
+#define LSMP_E_INVALID (-1)    /* bad parameter / size */
+#define LSMP_E_HIP (-2)        /* HIP runtime error */
+#define LSMP_E_NAN (-3)        /* non-finite value detected */
+#define LSMP_E_EMPTY_BAND (-4) /* mass-fix denominator is zero: no cell in the band */
+#define LSMP_E_STATE (-5)      /* call out of order (e.g. no velocity set) */
+
+/* arithmetic of the bulk arrays */
+#define LSMP_FP64 0            /* plain form, drop-in parity with the reference */
+#define LSMP_FP32 1            /* FP32 bulk; mass sums, lambda and area in FP64 */
+
+/* prescribed velocity fields for lsmp_set_velocity_analytic */
+#define LSMP_VEL_ROTATION 0    /* Disk/src/setup/velocity.f90:63-114 */
+#define LSMP_VEL_VORTEX2D 1    /* Vortex2D/src/setup/velocity.f90:35-36 */
+#define LSMP_VEL_LEVEQUE3D 2   /* Vortex/src/setup/velocity.f90:27-29 */
+
+typedef struct lsmp_params {
+    int nx, ny, nz;          /* interior cells (glb%node_x..z); nz = 1 in 2D */
+    int ghc;                 /* ghost width of host boxes; must be 3 */
+    int dim;                 /* 2 or 3 */
+    double dx, dy, dz;       /* dz = 1 in 2D (Disk/lib/data/tree.f90:252) */
+    double dt;               /* physical step, already scaled by dx (tree.f90:278) */
+    double rdt;              /* pseudo-time step, scaled by dx (tree.f90:279) */
+    double ls_wid;           /* interface half width, scaled by dx (tree.f90:280) */
+    double rho_12;           /* density ratio (deck line 50) */
+    int inverse;             /* inverse marker function (deck line 52) */
+    int precision;           /* LSMP_FP64 | LSMP_FP32 */
+    double xstart, xend, ystart, yend, zstart, zend;   /* domain (deck 18-22) */
+} lsmp_params;
+
+typedef struct lsmp_stats {
+    int last_redis_iters;    /* pseudo-steps of the last lsmp_reinit */
+    int last_mass_iters;     /* iterations of the last lsmp_mass_fix */
+    double last_red_error;   /* max |dphi| of the last pseudo-step */
+    double t_advect_ms;      /* accumulated device time per phase (HIP events) */
+    double t_reinit_ms;
+    double t_massfix_ms;
+    long long n_advect, n_reinit, n_massfix;  /* calls */
+} lsmp_stats;
+
+typedef struct lsmp_ctx *lsmp_handle;
+
+/* Set-up: replaces job_init's allocation and the CCD factorisation done once
+ * per solver (Vortex/lib/data/branches.f90:97-174, ccd_solvers.f90:382-410). */
+int lsmp_init(const lsmp_params *params, lsmp_handle *out);
+void lsmp_finalize(lsmp_handle h);
+const char *lsmp_error_string(int code);
+
+/* phi in / out as a host box (ghosts are filled on download by the
+ * zero-gradient rule, branches.f90:275-320). */
+int lsmp_upload_phi(lsmp_handle h, const double *phi_box);
+int lsmp_download_phi(lsmp_handle h, double *phi_box);
+/* Keep the current phi as phi0 = phi%tmp2, the target of the L1 shape metric
+ * (problem_init.f90:104-113 -> roots.f90:150-166). */
+int lsmp_store_phi0(lsmp_handle h);
+
+/* Node velocity nvel%old used by the advection (level_set_rk3_solver.f90:118-125).
+ * w is ignored in 2D. */
+int lsmp_set_velocity(lsmp_handle h, const double *u_box, const double *v_box, const double *w_box);
+/* The same velocity evaluated on the device from the reference's formulas at
+ * time t (velocity.f90; main.f90:23-28 evaluates it at t^{n+1}). */
+int lsmp_set_velocity_analytic(lsmp_handle h, int field, double t, double t2s);
+
+/* One TVD-RK3 CRWENO5 advection step with step dt
+ * (level_set_rk3_solver.f90:1-221). */
+int lsmp_rk3_advect(lsmp_handle h, double dt);
+
+/* Reinitialisation toward |grad phi| = 1 (level_set_rk3_redis.f90:1-79).
+ * btn = 1: sign from the smoothed Heaviside, constrained lambda; btn = 0:
+ * raw-phi sign, phi /= max|grad phi|, no lambda (Disk initialisation).
+ * timestop < 0 selects the reference default (2 ls_wid, or 1.5 x domain
+ * extent for btn = 0); otherwise it is the optional argument T. */
+int lsmp_reinit(lsmp_handle h, int btn, double timestop, double *red_error, int *n_pseudo);
+
+/* LS mass and volume sums of the current phi (ns_solver/others.f90:172-213). */
+int lsmp_mass_vol(lsmp_handle h, double *mass, double *vol);
+
+/* Global mass correction with target imass (mass_preserving_level_set.f90:1-74).
+ * mass receives the mass measured at the start of the last iteration (the
+ * value the exit test used), n_iter the iterations run. */
+int lsmp_mass_fix(lsmp_handle h, double imass, double *mass, int *n_iter);
+
+/* L1 shape error sum |H(phi0/eps) - H(phi/eps)| / (nx ny nz)
+ * (output.f90:169-204, 226, 233-248). */
+int lsmp_interface_l1(lsmp_handle h, double *l1);
+
+/* Device-time accounting per phase (the reference's ls_adv / ls_red timers,
+ * level_set_rk3_solver.f90:91-92, level_set_rk3_redis.f90:76-77). */
+int lsmp_get_stats(lsmp_handle h, lsmp_stats *out);
+/* Per-kernel timing with HIP events on the library's stream (off by default).
+ * lsmp_kernel_time reports the launches and summed milliseconds of one kernel
+ * by name ("adv_x", "adv_y", "adv_z", "redis_stage", "ccd_x", ...). */
+int lsmp_set_profiling(lsmp_handle h, int on);
+int lsmp_kernel_time(lsmp_handle h, const char *name, long long *launches, double *ms);
+/* Block until all work queued by the handle has finished. */
+int lsmp_synchronize(lsmp_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

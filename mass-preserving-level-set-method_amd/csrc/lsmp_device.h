@@ -1,0 +1,182 @@
+// lsmp_device.h -- per-point numerics of the level-set transport step, shared
+// by the gfx950 kernels.  Each function restates one reference formula; the
+// association order of every sum and product is the reference's (Fortran
+// evaluates left to right; x**2.0d0 is a square).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define LSMP_DEV __device__ __forceinline__
+
+namespace lsmp {
+
+template <typename T> LSMP_DEV T sq(T x) { return x * x; }
+template <typename T> LSMP_DEV T dabs(T x) { return fabs(x); }
+LSMP_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Pointwise flux split g = 1/2 (u + |u|) phi, f = 1/2 (u - |u|) phi
+// (level_set_rk3_solver.f90:118-125).
+template <typename T> LSMP_DEV void flux_split(T phi, T u, T &g, T &f)
+{
+    g = T(0.5) * (u + fabs(u)) * phi;
+    f = T(0.5) * (u - fabs(u)) * phi;
+}
+
+// WENO-JS5 reconstruction of the positive flux at i+1/2 from g(i-2..i+2)
+// (level_set_rk3_solver.f90:296-317).
+template <typename T> LSMP_DEV T wenojs_g(T g0, T g1, T g2, T g3, T g4)
+{
+    const T EPS = T(1.0e-13);
+    T b1 = T(13) * sq(g0 - T(2) * g1 + g2) + T(3) * sq(g0 - T(4) * g1 + T(3) * g2);
+    T b2 = T(13) * sq(g1 - T(2) * g2 + g3) + T(3) * sq(g1 - g3);
+    T b3 = T(13) * sq(g2 - T(2) * g3 + g4) + T(3) * sq(T(3) * g2 - T(4) * g3 + g4);
+    T a1 = T(1) / sq(EPS + b1);
+    T a2 = T(6) / sq(EPS + b2);
+    T a3 = T(3) / sq(EPS + b3);
+    T w1 = a1 / (a1 + a2 + a3);
+    T w2 = a2 / (a1 + a2 + a3);
+    T w3 = a3 / (a1 + a2 + a3);
+    return w1 / T(3) * g0 - (T(7) * w1 + w2) / T(6) * g1 +
+           (T(11) * w1 + T(5) * w2 + T(2) * w3) / T(6) * g2 +
+           (T(2) * w2 + T(5) * w3) / T(6) * g3 - w3 / T(6) * g4;
+}
+
+// WENO-JS5 reconstruction of the negative flux at i+1/2 from f(i-1..i+3)
+// (level_set_rk3_solver.f90:319-342).
+template <typename T> LSMP_DEV T wenojs_f(T f0, T f1, T f2, T f3, T f4)
+{
+    const T EPS = T(1.0e-13);
+    T b3 = T(13) * sq(f0 - T(2) * f1 + f2) + T(3) * sq(f0 - T(4) * f1 + T(3) * f2);
+    T b2 = T(13) * sq(f1 - T(2) * f2 + f3) + T(3) * sq(f1 - f3);
+    T b1 = T(13) * sq(f2 - T(2) * f3 + f4) + T(3) * sq(T(3) * f2 - T(4) * f3 + f4);
+    T a1 = T(1) / sq(EPS + b1);
+    T a2 = T(6) / sq(EPS + b2);
+    T a3 = T(3) / sq(EPS + b3);
+    T w1 = a1 / (a1 + a2 + a3);
+    T w2 = a2 / (a1 + a2 + a3);
+    T w3 = a3 / (a1 + a2 + a3);
+    return w3 * (-f0 + T(5) * f1 + T(2) * f2) / T(6) +
+           w2 * (T(2) * f1 + T(5) * f2 - f3) / T(6) +
+           w1 * (T(11) * f2 - T(7) * f3 + T(2) * f4) / T(6);
+}
+
+// One CRWENO5 row of the positive-flux system from g(i-2..i+2)
+// (level_set_rk3_solver.f90:362-381).
+template <typename T>
+LSMP_DEV void crweno_g_row(T g0, T g1, T g2, T g3, T g4, T &A, T &B, T &C, T &S)
+{
+    const T EPS = T(1.0e-13);
+    const T c1 = T(0.2089141306), c2 = T(0.4999999998), c3 = T(0.2910858692);
+    T b1 = T(13) * sq(g0 - T(2) * g1 + g2) + T(3) * sq(g0 - T(4) * g1 + T(3) * g2);
+    T b2 = T(13) * sq(g1 - T(2) * g2 + g3) + T(3) * sq(g1 - g3);
+    T b3 = T(13) * sq(g2 - T(2) * g3 + g4) + T(3) * sq(T(3) * g2 - T(4) * g3 + g4);
+    T a1 = c1 * (T(1) + fabs(b3 - b1) / (EPS + b1));
+    T a2 = c2 * (T(1) + fabs(b3 - b1) / (EPS + b2));
+    T a3 = c3 * (T(1) + fabs(b3 - b1) / (EPS + b3));
+    T w1 = a1 / (a1 + a2 + a3);
+    T w2 = a2 / (a1 + a2 + a3);
+    T w3 = a3 / (a1 + a2 + a3);
+    A = (T(2) * w1 + w2) / T(3);
+    B = (w1 + T(2) * (w2 + w3)) / T(3);
+    C = w3 / T(3);
+    S = w1 / T(6) * g1 + (T(5) * (w1 + w2) + w3) / T(6) * g2 + (w2 + T(5) * w3) / T(6) * g3;
+}
+
+// One CRWENO5 row of the negative-flux system from f(i-1..i+3)
+// (level_set_rk3_solver.f90:388-407).
+template <typename T>
+LSMP_DEV void crweno_f_row(T f0, T f1, T f2, T f3, T f4, T &A, T &B, T &C, T &S)
+{
+    const T EPS = T(1.0e-13);
+    const T c1 = T(0.2089141306), c2 = T(0.4999999998), c3 = T(0.2910858692);
+    T b3 = T(13) * sq(f0 - T(2) * f1 + f2) + T(3) * sq(f0 - T(4) * f1 + T(3) * f2);
+    T b2 = T(13) * sq(f1 - T(2) * f2 + f3) + T(3) * sq(f1 - f3);
+    T b1 = T(13) * sq(f2 - T(2) * f3 + f4) + T(3) * sq(T(3) * f2 - T(4) * f3 + f4);
+    T a1 = c1 * (T(1) + fabs(b3 - b1) / (EPS + b1));
+    T a2 = c2 * (T(1) + fabs(b3 - b1) / (EPS + b2));
+    T a3 = c3 * (T(1) + fabs(b3 - b1) / (EPS + b3));
+    T w1 = a1 / (a1 + a2 + a3);
+    T w2 = a2 / (a1 + a2 + a3);
+    T w3 = a3 / (a1 + a2 + a3);
+    A = (w3) / T(3);
+    B = (w1 + T(2) * (w2 + w3)) / T(3);
+    C = (w2 + T(2) * w1) / T(3);
+    S = (T(5) * w3 + w2) / T(6) * f1 + (w3 + T(5) * (w2 + w1)) / T(6) * f2 + w1 / T(6) * f3;
+}
+
+// WENO-HJ correction term (level_set_rk3_redis.f90:456-471).
+template <typename T> LSMP_DEV T phyn(T a, T b, T c, T d)
+{
+    const T eps = T(1.0e-6);
+    T is0 = T(13) * sq(a - b) + T(3) * sq(a - T(3) * b);
+    T is1 = T(13) * sq(b - c) + T(3) * sq(b + c);
+    T is2 = T(13) * sq(c - d) + T(3) * sq(T(3) * c - d);
+    T alp0 = T(1) / sq(eps + is0);
+    T alp1 = T(6) / sq(eps + is1);
+    T alp2 = T(3) / sq(eps + is2);
+    T w0 = alp0 / (alp0 + alp1 + alp2);
+    T w2 = alp2 / (alp0 + alp1 + alp2);
+    return w0 / T(3) * (a - T(2) * b + c) + (w2 - T(0.5)) / T(6) * (b - T(2) * c + d);
+}
+
+// Fourth-order central slope times 12 dx (level_set_rk3_redis.f90:290-293).
+template <typename T> LSMP_DEV T central4(T m2, T m1, T c, T p1, T p2)
+{
+    return -(m1 - m2) + T(7) * (c - m1) + T(7) * (p1 - c) - (p2 - p1);
+}
+
+// D+ and D- along one direction from phi(i-3..i+3) (level_set_rk3_redis.f90:305-315).
+template <typename T>
+LSMP_DEV void weno_hj_pm(T m3, T m2, T m1, T c, T p1, T p2, T p3, T inv12d, T invd, T &dp, T &dm)
+{
+    T a = inv12d * central4(m2, m1, c, p1, p2);
+    dp = a + invd * phyn(p3 - T(2) * p2 + p1, p2 - T(2) * p1 + c, p1 - T(2) * c + m1, c - T(2) * m1 + m2);
+    dm = a - invd * phyn(m3 - T(2) * m2 + m1, m2 - T(2) * m1 + c, m1 - T(2) * c + p1, c - T(2) * p1 + p2);
+}
+
+// Godunov |grad phi| selected by the frozen sign S0 (level_set_rk3_redis.f90:341-360).
+template <typename T, int DIM>
+LSMP_DEV T godunov(T xs1, T xs2, T ys1, T ys2, T zs1, T zs2, T s0)
+{
+    T upm = -fmin(xs1, T(0)), upp = fmax(xs1, T(0));
+    T umm = -fmin(xs2, T(0)), ump = fmax(xs2, T(0));
+    T vpm = -fmin(ys1, T(0)), vpp = fmax(ys1, T(0));
+    T vmm = -fmin(ys2, T(0)), vmp = fmax(ys2, T(0));
+    if (DIM == 3) {
+        T wpm = -fmin(zs1, T(0)), wpp = fmax(zs1, T(0));
+        T wmm = -fmin(zs2, T(0)), wmp = fmax(zs2, T(0));
+        if (s0 >= T(0)) return sqrt(sq(fmax(upm, ump)) + sq(fmax(vpm, vmp)) + sq(fmax(wpm, wmp)));
+        return sqrt(sq(fmax(upp, umm)) + sq(fmax(vpp, vmm)) + sq(fmax(wpp, wmm)));
+    } else {
+        if (s0 >= T(0)) return sqrt(sq(fmax(upm, ump)) + sq(fmax(vpm, vmp)));
+        return sqrt(sq(fmax(upp, umm)) + sq(fmax(vpp, vmm)));
+    }
+}
+
+// Smoothed Heaviside and delta (ns_solver/others.f90:18-31).
+template <typename T> LSMP_DEV void ls_funs(T phi, T ls_wid, T &H, T &D)
+{
+    const T eps = T(1.0e-12);
+    const T pi = T(3.141592653589793238462643383279502884);
+    T x = phi / ls_wid;
+    if (x > T(1) - eps) {
+        H = T(1); D = T(0);
+    } else if (x < T(-1) + eps) {
+        H = T(0); D = T(0);
+    } else {
+        H = T(0.5) * ((T(1) + x) + sin(pi * x) / pi);
+        D = T(0.5) * (T(1) + cos(pi * x)) / ls_wid;
+    }
+    H = fmax(fmin(H, T(1)), T(0));
+}
+
+// Heaviside used by the shape metric, without the clip (output.f90:233-248).
+template <typename T> LSMP_DEV T heavyside(T x)
+{
+    const T eps = T(1.0e-12);
+    const T pi = T(3.141592653589793238462643383279502884);
+    if (x > T(1) - eps) return T(1);
+    if (x < T(-1) + eps) return T(0);
+    return T(0.5) * ((T(1) + x) + sin(pi * x) / pi);
+}
+
+}  // namespace lsmp

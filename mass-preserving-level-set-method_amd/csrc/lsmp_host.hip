@@ -1,0 +1,710 @@
+// lsmp_host.hip -- C ABI of the gfx950 level-set transport library (include/lsmp.h).
+//
+// Owns the device state of one solver instance, orders the kernel launches of
+// each reference call on one HIP stream, and converts between the reference's
+// ghosted FP64 host boxes and the interior-only device layout.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/lsmp.h"
+#include "lsmp_internal.h"
+
+using namespace lsmp;
+
+namespace {
+
+constexpr int GHC = 3;
+constexpr int NPHASE = 3;   // advect, reinit, mass fix
+
+struct Pending {
+    int phase;            // -1: kernel timing only
+    std::string kname;    // empty: phase timing only
+    hipEvent_t e0, e1;
+};
+
+}  // namespace
+
+struct lsmp_ctx {
+    lsmp_params p{};
+    int dim = 3, nx = 0, ny = 0, nz = 1;
+    size_t ncell = 0;
+    bool fp32 = false;
+    size_t esz = 8;
+    hipStream_t st = nullptr;
+    // device fields (element type double or float)
+    void *phi = nullptr, *phiB = nullptr, *phiC = nullptr;
+    void *u = nullptr, *v = nullptr, *w = nullptr;
+    void *R = nullptr, *L1 = nullptr, *L2 = nullptr, *S0 = nullptr, *phi0 = nullptr;
+    void *dxv = nullptr, *dyv = nullptr, *gr = nullptr, *scr = nullptr;
+    void *coef[3] = {nullptr, nullptr, nullptr};
+    double *partial = nullptr, *scal = nullptr;
+    unsigned long long *u64 = nullptr;
+    double *hbuf = nullptr;   // pinned readback
+    int npartial = 0;
+    bool vel_set = false, phi0_set = false;
+    lsmp_stats stats{};
+    bool profiling = false;
+    std::map<std::string, std::pair<long long, double>> ktime;
+    std::vector<Pending> pend;
+    std::vector<hipEvent_t> pool;
+};
+
+namespace {
+
+thread_local char g_errbuf[256];
+
+#define LSMP_HIP(call)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            snprintf(g_errbuf, sizeof g_errbuf, "%s: %s", #call, hipGetErrorString(e_));           \
+            return LSMP_E_HIP;                                                                     \
+        }                                                                                          \
+    } while (0)
+
+#define LSMP_LAUNCHED()                                                                            \
+    do {                                                                                           \
+        hipError_t e_ = hipGetLastError();                                                         \
+        if (e_ != hipSuccess) {                                                                    \
+            snprintf(g_errbuf, sizeof g_errbuf, "kernel launch: %s", hipGetErrorString(e_));       \
+            return LSMP_E_HIP;                                                                     \
+        }                                                                                          \
+    } while (0)
+
+hipEvent_t get_event(lsmp_ctx *c)
+{
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Resolve finished timing records (all of them when wait is true).
+void resolve(lsmp_ctx *c, bool wait)
+{
+    size_t keep = 0;
+    for (size_t q = 0; q < c->pend.size(); ++q) {
+        Pending &r = c->pend[q];
+        if (!wait && hipEventQuery(r.e1) != hipSuccess) {
+            c->pend[keep++] = r;
+            continue;
+        }
+        (void)hipEventSynchronize(r.e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.e0, r.e1);
+        if (r.phase == 0) c->stats.t_advect_ms += ms;
+        else if (r.phase == 1) c->stats.t_reinit_ms += ms;
+        else if (r.phase == 2) c->stats.t_massfix_ms += ms;
+        if (!r.kname.empty()) {
+            auto &kt = c->ktime[r.kname];
+            kt.first += 1;
+            kt.second += ms;
+        }
+        c->pool.push_back(r.e0);
+        c->pool.push_back(r.e1);
+    }
+    c->pend.resize(keep);
+}
+
+// Record a start event; returns it (or null when not timing).
+struct Timer {
+    lsmp_ctx *c;
+    int phase;
+    std::string name;
+    hipEvent_t e0 = nullptr;
+    Timer(lsmp_ctx *c_, int phase_, const char *name_, bool on) : c(c_), phase(phase_)
+    {
+        if (!on) return;
+        if (name_) name = name_;
+        e0 = get_event(c);
+        if (e0) (void)hipEventRecord(e0, c->st);
+    }
+    void stop()
+    {
+        if (!e0) return;
+        hipEvent_t e1 = get_event(c);
+        if (!e1) return;
+        (void)hipEventRecord(e1, c->st);
+        c->pend.push_back(Pending{phase, name, e0, e1});
+        e0 = nullptr;
+        if (c->pend.size() > 4096) resolve(c, true);
+    }
+};
+
+// Kernel-level timing: only when profiling is on.
+#define KT(ctx, kname, launch)                                                                     \
+    do {                                                                                           \
+        Timer kt_((ctx), -1, (kname), (ctx)->profiling);                                           \
+        launch;                                                                                    \
+        LSMP_LAUNCHED();                                                                           \
+        kt_.stop();                                                                                \
+    } while (0)
+
+int check_params(const lsmp_params *p)
+{
+    if (!p) return LSMP_E_INVALID;
+    if (p->ghc != GHC) return LSMP_E_INVALID;
+    if (p->dim != 2 && p->dim != 3) return LSMP_E_INVALID;
+    if (p->nx < 4 || p->ny < 4) return LSMP_E_INVALID;
+    if (p->dim == 3 && p->nz < 4) return LSMP_E_INVALID;
+    if (p->dim == 2 && p->nz != 1) return LSMP_E_INVALID;
+    if (!(p->dx > 0) || !(p->dy > 0) || !(p->dz > 0) || !(p->ls_wid > 0) || !(p->rdt > 0))
+        return LSMP_E_INVALID;
+    if (p->precision != LSMP_FP64 && p->precision != LSMP_FP32) return LSMP_E_INVALID;
+    // line indices and per-line scratch offsets are kept in 32-bit ints
+    const double lines = (double)p->nx * p->ny * (p->dim == 3 ? p->nz : 1);
+    if (lines > 2.0e9) return LSMP_E_INVALID;
+    return LSMP_OK;
+}
+
+// CCD coefficient tables (ccd_solvers.f90:80-163), factorised once by the
+// 2x2 block elimination of TWIN_DEC (matrix_solver.f90:6-45), for a box line
+// of nb points with spacing dx.  Output: A, B, AA, BB, each (3, nb).
+void ccd_tables(int nb, double dx, std::vector<double> &out)
+{
+    out.assign((size_t)12 * nb, 0.0);
+    double *A = out.data(), *B = A + 3 * nb, *AA = A + 6 * nb, *BB = A + 9 * nb;
+    auto M = [&](double *X, int m, int i) -> double & { return X[m + 3 * i]; };
+    const int is = 0, ie = nb - 1;
+    M(A, 1, is) = 1.0;  M(B, 1, is) = 0.0;  M(AA, 1, is) = 0.0;  M(BB, 1, is) = 1.0;
+    M(A, 2, is) = 2.0;  M(B, 2, is) = -dx;  M(AA, 2, is) = -2.5 / dx;  M(BB, 2, is) = 8.5;
+    M(A, 1, ie) = 1.0;  M(B, 1, ie) = 0.0;  M(AA, 1, ie) = 0.0;  M(BB, 1, ie) = 1.0;
+    M(A, 0, ie) = 2.0;  M(B, 0, ie) = dx;   M(AA, 0, ie) = 2.5 / dx;  M(BB, 0, ie) = 8.5;
+    for (int i = is + 1; i <= ie - 1; ++i) {
+        M(A, 0, i) = 7.0 / 16.0;  M(A, 1, i) = 1.0;  M(A, 2, i) = 7.0 / 16.0;
+        M(B, 0, i) = dx / 16.0;   M(B, 1, i) = 0.0;  M(B, 2, i) = -dx / 16.0;
+        M(AA, 0, i) = -(9.0 / 8.0 / dx);  M(AA, 1, i) = 0.0;  M(AA, 2, i) = 9.0 / 8.0 / dx;
+        M(BB, 0, i) = -1.0 / 8.0;  M(BB, 1, i) = 1.0;  M(BB, 2, i) = -1.0 / 8.0;
+    }
+    M(A, 0, is) = 0.0;  M(AA, 0, is) = 0.0;  M(A, 2, ie) = 0.0;  M(AA, 2, ie) = 0.0;
+    M(B, 0, is) = 0.0;  M(BB, 0, is) = 0.0;  M(B, 2, ie) = 0.0;  M(BB, 2, ie) = 0.0;
+    for (int i = is + 1; i <= ie; ++i) {
+        const double den = M(A, 1, i - 1) * M(BB, 1, i - 1) - M(AA, 1, i - 1) * M(B, 1, i - 1);
+        double sc1 = -(M(AA, 1, i - 1) * M(B, 0, i) - M(A, 0, i) * M(BB, 1, i - 1)) / den;
+        double sc2 = (M(B, 0, i) * M(A, 1, i - 1) - M(B, 1, i - 1) * M(A, 0, i)) / den;
+        M(A, 0, i) = sc1;
+        M(A, 1, i) = M(A, 1, i) - (sc1 * M(A, 2, i - 1) + sc2 * M(AA, 2, i - 1));
+        M(B, 0, i) = sc2;
+        M(B, 1, i) = M(B, 1, i) - (sc1 * M(B, 2, i - 1) + sc2 * M(BB, 2, i - 1));
+        sc1 = -(M(AA, 1, i - 1) * M(BB, 0, i) - M(AA, 0, i) * M(BB, 1, i - 1)) / den;
+        sc2 = (M(BB, 0, i) * M(A, 1, i - 1) - M(B, 1, i - 1) * M(AA, 0, i)) / den;
+        M(AA, 0, i) = sc1;
+        M(AA, 1, i) = M(AA, 1, i) - (sc1 * M(A, 2, i - 1) + sc2 * M(AA, 2, i - 1));
+        M(BB, 0, i) = sc2;
+        M(BB, 1, i) = M(BB, 1, i) - (sc1 * M(B, 2, i - 1) + sc2 * M(BB, 2, i - 1));
+    }
+}
+
+inline long box_index(const lsmp_ctx *c, int i, int j, int k)   // 0-based interior coords
+{
+    const long nxb = c->nx + 2 * GHC, nyb = c->ny + 2 * GHC;
+    return ((long)(k + GHC) * nyb + (j + GHC)) * nxb + (i + GHC);
+}
+
+// host box -> interior (double) vector
+void pack_box(const lsmp_ctx *c, const double *box, std::vector<double> &out)
+{
+    out.resize(c->ncell);
+    size_t q = 0;
+    for (int k = 0; k < c->nz; ++k)
+        for (int j = 0; j < c->ny; ++j) {
+            const double *row = box + box_index(c, 0, j, k);
+            std::memcpy(&out[q], row, sizeof(double) * c->nx);
+            q += c->nx;
+        }
+}
+
+// interior -> host box with zero-gradient ghosts (branches.f90:283-310)
+void unpack_box(const lsmp_ctx *c, const std::vector<double> &in, double *box)
+{
+    const int nxb = c->nx + 2 * GHC, nyb = c->ny + 2 * GHC, nzb = c->nz + 2 * GHC;
+    for (int kb = 0; kb < nzb; ++kb) {
+        const int k = std::min(std::max(kb - GHC, 0), c->nz - 1);
+        for (int jb = 0; jb < nyb; ++jb) {
+            const int j = std::min(std::max(jb - GHC, 0), c->ny - 1);
+            const double *src = &in[((size_t)k * c->ny + j) * c->nx];
+            double *dst = box + ((long)kb * nyb + jb) * nxb;
+            for (int ib = 0; ib < nxb; ++ib) {
+                const int i = std::min(std::max(ib - GHC, 0), c->nx - 1);
+                dst[ib] = src[i];
+            }
+        }
+    }
+}
+
+int upload_field(lsmp_ctx *c, void *dst, const double *box)
+{
+    std::vector<double> tmp;
+    pack_box(c, box, tmp);
+    if (c->fp32) {
+        std::vector<float> f(tmp.begin(), tmp.end());
+        LSMP_HIP(hipMemcpyAsync(dst, f.data(), sizeof(float) * c->ncell, hipMemcpyHostToDevice, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+    } else {
+        LSMP_HIP(hipMemcpyAsync(dst, tmp.data(), sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+    }
+    return LSMP_OK;
+}
+
+int download_field(lsmp_ctx *c, const void *src, double *box)
+{
+    std::vector<double> tmp(c->ncell);
+    if (c->fp32) {
+        std::vector<float> f(c->ncell);
+        LSMP_HIP(hipMemcpyAsync(f.data(), src, sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+        for (size_t q = 0; q < c->ncell; ++q) tmp[q] = f[q];
+    } else {
+        LSMP_HIP(hipMemcpyAsync(tmp.data(), src, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+    }
+    unpack_box(c, tmp, box);
+    return LSMP_OK;
+}
+
+int alloc(void **p, size_t bytes)
+{
+    LSMP_HIP(hipMalloc(p, bytes));
+    return LSMP_OK;
+}
+
+// ------------------------------------------------------------------ phases
+template <typename T> int advect(lsmp_ctx *c, double dt)
+{
+    T *phi = (T *)c->phi, *R = (T *)c->R, *L1 = (T *)c->L1, *L2 = (T *)c->L2, *scr = (T *)c->scr;
+    const T *u = (const T *)c->u, *v = (const T *)c->v, *w = (const T *)c->w;
+    const T dx = (T)c->p.dx, dy = (T)c->p.dy, dz = (T)c->p.dz, tdt = (T)dt;
+    for (int stage = 1; stage <= 3; ++stage) {
+        KT(c, "adv_x", (launch_adv_pass<T>(c->st, 0, 0, stage, phi, u, R, L1, L2, scr, c->nx, c->ny, c->nz, dx, tdt)));
+        if (c->dim == 3) {
+            KT(c, "adv_y", (launch_adv_pass<T>(c->st, 1, 1, stage, phi, v, R, L1, L2, scr, c->nx, c->ny, c->nz, dy, tdt)));
+            KT(c, "adv_z", (launch_adv_pass<T>(c->st, 2, 2, stage, phi, w, R, L1, L2, scr, c->nx, c->ny, c->nz, dz, tdt)));
+        } else {
+            KT(c, "adv_y", (launch_adv_pass<T>(c->st, 1, 2, stage, phi, v, R, L1, L2, scr, c->nx, c->ny, c->nz, dy, tdt)));
+        }
+    }
+    return LSMP_OK;
+}
+
+RedisArgs redis_args(const lsmp_ctx *c)
+{
+    RedisArgs ra;
+    ra.nx = c->nx; ra.ny = c->ny; ra.nz = c->nz;
+    ra.inv12dx = 1.0 / (12.0 * c->p.dx);
+    ra.inv12dy = 1.0 / (12.0 * c->p.dy);
+    ra.inv12dz = 1.0 / (12.0 * c->p.dz);
+    ra.invdx = 1.0 / c->p.dx;
+    ra.invdy = 1.0 / c->p.dy;
+    ra.invdz = 1.0 / c->p.dz;
+    ra.ls_wid = c->p.ls_wid;
+    ra.rdt = c->p.rdt;
+    return ra;
+}
+
+template <typename T> int reinit(lsmp_ctx *c, int btn, double timestop, double *red_error, int *n_pseudo)
+{
+    const RedisArgs ra = redis_args(c);
+    const size_t n = c->ncell;
+    T *S0 = (T *)c->S0;
+    // level_set_redis_init / level_set_redis_stable (redis.f90:81-174)
+    if (btn == 1) {
+        KT(c, "sign_init", (launch_sign_init<T>(c->st, (const T *)c->phi, S0, n, (T)c->p.ls_wid, 0)));
+    } else {
+        KT(c, "sign_init", (launch_sign_init<T>(c->st, (const T *)c->phi, S0, n, (T)c->p.ls_wid, 1)));
+        launch_zero_u64(c->st, c->u64 + 1, 1);
+        KT(c, "grad_max", (launch_grad_max<T>(c->st, c->dim, (const T *)c->phi, S0, ra, c->u64 + 1)));
+        KT(c, "scale_max", (launch_scale_by_max<T>(c->st, (T *)c->phi, n, c->u64 + 1)));
+    }
+    if (timestop < 0.0) {
+        if (btn == 0) {
+            double m = c->p.xend - c->p.xstart;
+            m = std::max(m, c->p.yend - c->p.ystart);
+            m = std::max(m, c->p.zend - c->p.zstart);
+            timestop = 1.5 * m;
+        } else {
+            timestop = 2.0 * c->p.ls_wid;
+        }
+    }
+    int iter = 0;
+    double time = 0.0, error = 0.0;
+    for (;;) {
+        iter = iter + 1;
+        time = time + c->p.rdt;
+        T *cur = (T *)c->phi, *b = (T *)c->phiB, *cc = (T *)c->phiC;
+        launch_zero_u64(c->st, c->u64, 1);
+        LSMP_LAUNCHED();
+        KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 1, btn, cur, b, cur, S0, ra, c->u64)));
+        KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 2, btn, b, cc, cur, S0, ra, c->u64)));
+        KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 3, btn, cc, b, cur, S0, ra, c->u64)));
+        LSMP_HIP(hipMemcpyAsync(c->hbuf, c->u64, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+        unsigned long long bits;
+        std::memcpy(&bits, c->hbuf, sizeof bits);
+        std::memcpy(&error, &bits, sizeof error);
+        // the new phi is in b; the old one becomes a spare
+        c->phiB = c->phi;
+        c->phi = b;
+        if (time > timestop || error <= 1.0e-6) break;
+    }
+    if (std::isnan(error)) return LSMP_E_NAN;
+    c->stats.last_redis_iters = iter;
+    c->stats.last_red_error = error;
+    if (red_error) *red_error = error;
+    if (n_pseudo) *n_pseudo = iter;
+    return LSMP_OK;
+}
+
+CcdArgs ccd_args(const lsmp_ctx *c)
+{
+    CcdArgs ca;
+    ca.dx = c->p.dx; ca.dy = c->p.dy; ca.dz = c->p.dz;
+    ca.dv = c->p.dx * c->p.dy * c->p.dz;
+    ca.ls_wid = c->p.ls_wid;
+    ca.rho_12 = c->p.rho_12;
+    ca.inverse = c->p.inverse;
+    return ca;
+}
+
+template <typename T> int mass_fix(lsmp_ctx *c, double imass, double *mass_out, int *n_iter)
+{
+    const CcdArgs ca = ccd_args(c);
+    T *phi = (T *)c->phi;
+    int iter, nblk = 0, rc = LSMP_OK;
+    double mass = 0.0, prev_rel = 0.0;
+    for (iter = 1; iter <= 15; ++iter) {
+        KT(c, "ccd_x", (launch_ccd_pass<T>(c->st, 0, c->dim, false, phi, (const T *)c->coef[0], (T *)c->scr,
+                                           (T *)c->dxv, nullptr, nullptr, nullptr, nullptr, c->nx, c->ny, c->nz,
+                                           ca, nullptr)));
+        if (c->dim == 3) {
+            KT(c, "ccd_y", (launch_ccd_pass<T>(c->st, 1, 3, false, phi, (const T *)c->coef[1], (T *)c->scr,
+                                               (T *)c->dyv, nullptr, nullptr, nullptr, nullptr, c->nx, c->ny, c->nz,
+                                               ca, nullptr)));
+            KT(c, "ccd_z", (launch_ccd_pass<T>(c->st, 2, 3, true, phi, (const T *)c->coef[2], (T *)c->scr, nullptr,
+                                               (const T *)c->dxv, (const T *)c->dyv, (T *)c->gr, c->partial, c->nx,
+                                               c->ny, c->nz, ca, &nblk)));
+        } else {
+            KT(c, "ccd_y", (launch_ccd_pass<T>(c->st, 1, 2, true, phi, (const T *)c->coef[1], (T *)c->scr, nullptr,
+                                               (const T *)c->dxv, nullptr, (T *)c->gr, c->partial, c->nx, c->ny,
+                                               c->nz, ca, &nblk)));
+        }
+        if (nblk > c->npartial) return LSMP_E_INVALID;
+        KT(c, "finalize", (launch_finalize3(c->st, c->partial, nblk, imass, c->scal)));
+        KT(c, "mass_update", (launch_mass_update<T>(c->st, phi, (const T *)c->gr, c->ncell, (T)c->p.ls_wid, c->scal)));
+        LSMP_HIP(hipMemcpyAsync(c->hbuf, c->scal, 5 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+        mass = c->hbuf[1];
+        if (c->hbuf[4] != 0.0) { rc = LSMP_E_EMPTY_BAND; break; }
+        if (std::isnan(mass) || std::isnan(c->hbuf[3])) { rc = LSMP_E_NAN; break; }
+        const double rel = std::fabs(imass - mass) / imass;
+        if (rel < 1.0e-14) break;   // the reference's exit test (mass_preserving_level_set.f90:70)
+        if (c->fp32) {
+            // FP32 bulk phi resolves the mass only to ~1e-10..1e-8 relative; stop
+            // once the correction no longer halves the defect
+            if (rel < 1.0e-10 || (iter > 1 && rel >= 0.5 * prev_rel)) break;
+        }
+        prev_rel = rel;
+    }
+    if (iter > 15) iter = 15;
+    c->stats.last_mass_iters = iter;
+    if (mass_out) *mass_out = mass;
+    if (n_iter) *n_iter = iter;
+    return rc;
+}
+
+int reduce_partials(lsmp_ctx *c, int nblk, double *out3)
+{
+    launch_finalize3(c->st, c->partial, nblk, 0.0, c->scal);
+    LSMP_LAUNCHED();
+    LSMP_HIP(hipMemcpyAsync(c->hbuf, c->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    LSMP_HIP(hipStreamSynchronize(c->st));
+    out3[0] = c->hbuf[0];
+    out3[1] = c->hbuf[1];
+    out3[2] = c->hbuf[2];
+    return LSMP_OK;
+}
+
+int reduce_blocks(const lsmp_ctx *c)
+{
+    size_t b = (c->ncell + 255) / 256;
+    if (b > 1024) b = 1024;
+    if ((int)b > c->npartial) b = c->npartial;
+    return (int)b;
+}
+
+template <typename T> int mass_vol(lsmp_ctx *c, double *mass, double *vol)
+{
+    const int nb = reduce_blocks(c);
+    KT(c, "mass_vol", (launch_mass_vol<T>(c->st, (const T *)c->phi, c->ncell, ccd_args(c), c->partial, nb)));
+    double r[3];
+    int rc = reduce_partials(c, nb, r);
+    if (rc) return rc;
+    if (vol) *vol = r[0];
+    if (mass) *mass = r[1];
+    return LSMP_OK;
+}
+
+template <typename T> int interface_l1(lsmp_ctx *c, double *l1)
+{
+    const int nb = reduce_blocks(c);
+    KT(c, "l1", (launch_l1<T>(c->st, (const T *)c->phi, (const T *)c->phi0, c->ncell, c->p.ls_wid, c->partial, nb)));
+    double r[3];
+    int rc = reduce_partials(c, nb, r);
+    if (rc) return rc;
+    *l1 = r[0] / (double)c->nx / (double)c->ny / (double)c->nz;
+    return LSMP_OK;
+}
+
+template <typename T> int velocity_analytic(lsmp_ctx *c, int field, double t, double t2s)
+{
+    const double pi = std::acos(-1.0);
+    VelArgs va;
+    va.nx = c->nx; va.ny = c->ny; va.nz = c->nz; va.field = field;
+    va.xstart = c->p.xstart; va.ystart = c->p.ystart; va.zstart = c->p.zstart;
+    va.dx = c->p.dx; va.dy = c->p.dy; va.dz = c->p.dz;
+    va.ct = std::cos(pi * t / t2s);
+    va.omega = 2.0 * pi / t2s;
+    KT(c, "velocity", (launch_velocity<T>(c->st, (T *)c->u, (T *)c->v, (T *)c->w, va)));
+    return LSMP_OK;
+}
+
+void free_all(lsmp_ctx *c)
+{
+    void *bufs[] = {c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
+                    c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->hbuf) (void)hipHostFree(c->hbuf);
+    for (auto &r : c->pend) { (void)hipEventDestroy(r.e0); (void)hipEventDestroy(r.e1); }
+    for (auto e : c->pool) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+const char *lsmp_error_string(int code)
+{
+    switch (code) {
+    case LSMP_OK: return "ok";
+    case LSMP_E_INVALID: return "invalid parameter or size";
+    case LSMP_E_HIP: return g_errbuf[0] ? g_errbuf : "HIP runtime error";
+    case LSMP_E_NAN: return "non-finite value in the level-set field";
+    case LSMP_E_EMPTY_BAND: return "mass correction: no cell inside the interface band";
+    case LSMP_E_STATE: return "call out of order (velocity or phi0 not set)";
+    default: return "unknown error";
+    }
+}
+
+int lsmp_init(const lsmp_params *params, lsmp_handle *out)
+{
+    if (!out) return LSMP_E_INVALID;
+    *out = nullptr;
+    int rc = check_params(params);
+    if (rc) return rc;
+    lsmp_ctx *c = new lsmp_ctx();
+    c->p = *params;
+    c->dim = params->dim;
+    c->nx = params->nx;
+    c->ny = params->ny;
+    c->nz = params->dim == 3 ? params->nz : 1;
+    c->ncell = (size_t)c->nx * c->ny * c->nz;
+    c->fp32 = params->precision == LSMP_FP32;
+    c->esz = c->fp32 ? sizeof(float) : sizeof(double);
+    auto fail = [&](int code) {
+        free_all(c);
+        delete c;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return fail(LSMP_E_HIP);
+    const size_t fb = c->ncell * c->esz;
+    void **fields[] = {&c->phi, &c->phiB, &c->phiC, &c->u, &c->v, &c->w, &c->R, &c->L1, &c->L2,
+                       &c->S0, &c->phi0, &c->dxv, &c->dyv, &c->gr};
+    for (void **f : fields)
+        if (alloc(f, fb)) return fail(LSMP_E_HIP);
+    // scratch: CRWENO (c', d') of 2 systems for n-2 rows per line, or CCD
+    // (S, SS) for n+5 rows per line, whichever direction needs most
+    size_t sc = 0;
+    const int ns[3] = {c->nx, c->ny, c->nz};
+    for (int d = 0; d < c->dim; ++d) {
+        const size_t nlines = c->ncell / (size_t)ns[d];
+        sc = std::max(sc, (size_t)4 * (size_t)(ns[d] - 2) * nlines);
+        sc = std::max(sc, (size_t)2 * (size_t)(ns[d] + 5) * nlines);
+    }
+    if (alloc(&c->scr, sc * c->esz)) return fail(LSMP_E_HIP);
+    for (int d = 0; d < c->dim; ++d) {
+        std::vector<double> tab;
+        ccd_tables(ns[d] + 2 * GHC, params->dx, tab);   // all three solvers use dx (ccd_solvers.f90:406-408)
+        if (alloc(&c->coef[d], tab.size() * c->esz)) return fail(LSMP_E_HIP);
+        if (c->fp32) {
+            std::vector<float> tf(tab.begin(), tab.end());
+            if (hipMemcpy(c->coef[d], tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+                return fail(LSMP_E_HIP);
+        } else if (hipMemcpy(c->coef[d], tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) !=
+                   hipSuccess) {
+            return fail(LSMP_E_HIP);
+        }
+    }
+    // partial sums: one slot per block of the last CCD pass or reduction kernel
+    size_t maxlines = 0;
+    for (int d = 1; d < c->dim; ++d) maxlines = std::max(maxlines, c->ncell / (size_t)ns[d]);
+    c->npartial = (int)std::max<size_t>(1024, (maxlines + 255) / 256);
+    if (alloc((void **)&c->partial, (size_t)3 * c->npartial * sizeof(double))) return fail(LSMP_E_HIP);
+    if (alloc((void **)&c->scal, 8 * sizeof(double))) return fail(LSMP_E_HIP);
+    if (alloc((void **)&c->u64, 4 * sizeof(unsigned long long))) return fail(LSMP_E_HIP);
+    if (hipHostMalloc((void **)&c->hbuf, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return fail(LSMP_E_HIP);
+    // velocity starts at zero so that an advect before set_velocity is defined
+    if (hipMemsetAsync(c->u, 0, fb, c->st) != hipSuccess || hipMemsetAsync(c->v, 0, fb, c->st) != hipSuccess ||
+        hipMemsetAsync(c->w, 0, fb, c->st) != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess)
+        return fail(LSMP_E_HIP);
+    *out = c;
+    return LSMP_OK;
+}
+
+void lsmp_finalize(lsmp_handle h)
+{
+    if (!h) return;
+    (void)hipStreamSynchronize(h->st);
+    free_all(h);
+    delete h;
+}
+
+int lsmp_upload_phi(lsmp_handle h, const double *phi_box)
+{
+    if (!h || !phi_box) return LSMP_E_INVALID;
+    return upload_field(h, h->phi, phi_box);
+}
+
+int lsmp_download_phi(lsmp_handle h, double *phi_box)
+{
+    if (!h || !phi_box) return LSMP_E_INVALID;
+    return download_field(h, h->phi, phi_box);
+}
+
+int lsmp_store_phi0(lsmp_handle h)
+{
+    if (!h) return LSMP_E_INVALID;
+    LSMP_HIP(hipMemcpyAsync(h->phi0, h->phi, h->ncell * h->esz, hipMemcpyDeviceToDevice, h->st));
+    LSMP_HIP(hipStreamSynchronize(h->st));
+    h->phi0_set = true;
+    return LSMP_OK;
+}
+
+int lsmp_set_velocity(lsmp_handle h, const double *u_box, const double *v_box, const double *w_box)
+{
+    if (!h || !u_box || !v_box || (h->dim == 3 && !w_box)) return LSMP_E_INVALID;
+    int rc = upload_field(h, h->u, u_box);
+    if (!rc) rc = upload_field(h, h->v, v_box);
+    if (!rc && h->dim == 3) rc = upload_field(h, h->w, w_box);
+    if (!rc) h->vel_set = true;
+    return rc;
+}
+
+int lsmp_set_velocity_analytic(lsmp_handle h, int field, double t, double t2s)
+{
+    if (!h || field < 0 || field > 2 || !(t2s > 0)) return LSMP_E_INVALID;
+    if (field == LSMP_VEL_LEVEQUE3D && h->dim != 3) return LSMP_E_INVALID;
+    int rc = h->fp32 ? velocity_analytic<float>(h, field, t, t2s) : velocity_analytic<double>(h, field, t, t2s);
+    if (!rc) h->vel_set = true;
+    return rc;
+}
+
+int lsmp_rk3_advect(lsmp_handle h, double dt)
+{
+    if (!h || !(dt > 0)) return LSMP_E_INVALID;
+    if (!h->vel_set) return LSMP_E_STATE;
+    Timer tm(h, 0, nullptr, true);
+    int rc = h->fp32 ? advect<float>(h, dt) : advect<double>(h, dt);
+    tm.stop();
+    h->stats.n_advect++;
+    resolve(h, false);
+    return rc;
+}
+
+int lsmp_reinit(lsmp_handle h, int btn, double timestop, double *red_error, int *n_pseudo)
+{
+    if (!h || (btn != 0 && btn != 1)) return LSMP_E_INVALID;
+    Timer tm(h, 1, nullptr, true);
+    int rc = h->fp32 ? reinit<float>(h, btn, timestop, red_error, n_pseudo)
+                     : reinit<double>(h, btn, timestop, red_error, n_pseudo);
+    tm.stop();
+    h->stats.n_reinit++;
+    resolve(h, false);
+    return rc;
+}
+
+int lsmp_mass_vol(lsmp_handle h, double *mass, double *vol)
+{
+    if (!h) return LSMP_E_INVALID;
+    return h->fp32 ? mass_vol<float>(h, mass, vol) : mass_vol<double>(h, mass, vol);
+}
+
+int lsmp_mass_fix(lsmp_handle h, double imass, double *mass, int *n_iter)
+{
+    if (!h || !(imass > 0)) return LSMP_E_INVALID;
+    Timer tm(h, 2, nullptr, true);
+    int rc = h->fp32 ? mass_fix<float>(h, imass, mass, n_iter) : mass_fix<double>(h, imass, mass, n_iter);
+    tm.stop();
+    h->stats.n_massfix++;
+    resolve(h, false);
+    return rc;
+}
+
+int lsmp_interface_l1(lsmp_handle h, double *l1)
+{
+    if (!h || !l1) return LSMP_E_INVALID;
+    if (!h->phi0_set) return LSMP_E_STATE;
+    return h->fp32 ? interface_l1<float>(h, l1) : interface_l1<double>(h, l1);
+}
+
+int lsmp_get_stats(lsmp_handle h, lsmp_stats *out)
+{
+    if (!h || !out) return LSMP_E_INVALID;
+    LSMP_HIP(hipStreamSynchronize(h->st));
+    resolve(h, true);
+    *out = h->stats;
+    return LSMP_OK;
+}
+
+int lsmp_set_profiling(lsmp_handle h, int on)
+{
+    if (!h) return LSMP_E_INVALID;
+    LSMP_HIP(hipStreamSynchronize(h->st));
+    resolve(h, true);
+    h->profiling = on != 0;
+    h->ktime.clear();
+    return LSMP_OK;
+}
+
+int lsmp_kernel_time(lsmp_handle h, const char *name, long long *launches, double *ms)
+{
+    if (!h || !name) return LSMP_E_INVALID;
+    LSMP_HIP(hipStreamSynchronize(h->st));
+    resolve(h, true);
+    auto it = h->ktime.find(name);
+    if (launches) *launches = it == h->ktime.end() ? 0 : it->second.first;
+    if (ms) *ms = it == h->ktime.end() ? 0.0 : it->second.second;
+    return LSMP_OK;
+}
+
+int lsmp_synchronize(lsmp_handle h)
+{
+    if (!h) return LSMP_E_INVALID;
+    LSMP_HIP(hipStreamSynchronize(h->st));
+    return LSMP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// lsmp_internal.h -- tile constants and kernel launchers shared by the kernel
+// and host translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace lsmp {
+
+// x-line kernels stage XCH rows of 64 lines through LDS
+constexpr int XCH = 32;
+
+// reinitialisation tile: RTX x RTY interior cells, one ring for the lambda
+// box, four for the WENO-HJ stencil; RKC planes per workgroup in z
+constexpr int RTX = 32, RTY = 16;
+constexpr int REX = RTX + 2, REY = RTY + 2;
+constexpr int RPX = RTX + 8, RPY = RTY + 8;
+constexpr int RNT = 640;
+constexpr int RKC = 32;
+static_assert(REX * REY <= RNT, "reinit tile needs one thread per ring cell");
+
+struct RedisArgs {
+    int nx, ny, nz;
+    double inv12dx, inv12dy, inv12dz, invdx, invdy, invdz, ls_wid, rdt;
+};
+
+struct CcdArgs {
+    double dx, dy, dz, dv, ls_wid, rho_12;
+    int inverse;
+};
+
+struct VelArgs {
+    int nx, ny, nz, field;
+    double xstart, ystart, zstart, dx, dy, dz, ct, omega;
+};
+
+template <typename T>
+void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const T *vel, T *R, T *L1, T *L2,
+                     T *scr, int nx, int ny, int nz, T dd, T dt);
+template <typename T>
+void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *phi, T *phi_out, const T *phi0,
+                        const T *S0, const RedisArgs &ra, unsigned long long *errmax);
+template <typename T> void launch_sign_init(hipStream_t st, const T *phi, T *S0, size_t n, T ls_wid, int raw);
+template <typename T>
+void launch_grad_max(hipStream_t st, int dim, const T *phi, const T *S0, const RedisArgs &ra,
+                     unsigned long long *gmax);
+template <typename T> void launch_scale_by_max(hipStream_t st, T *phi, size_t n, const unsigned long long *gmax);
+template <typename T>
+void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T *phi, const T *coef, T *scr, T *out,
+                     const T *dxv, const T *dyv, T *gr, double *partial, int nx, int ny, int nz, const CcdArgs &ca,
+                     int *nblocks_out);
+void launch_finalize3(hipStream_t st, const double *partial, int nblk, double imass, double *scal);
+template <typename T>
+void launch_mass_update(hipStream_t st, T *phi, const T *gr, size_t n, T ls_wid, const double *scal);
+template <typename T>
+void launch_mass_vol(hipStream_t st, const T *phi, size_t n, const CcdArgs &ca, double *partial, int nblocks);
+template <typename T>
+void launch_l1(hipStream_t st, const T *phi, const T *phi0, size_t n, double ls_wid, double *partial, int nblocks);
+template <typename T> void launch_velocity(hipStream_t st, T *u, T *v, T *w, const VelArgs &va);
+template <typename TO, typename TI> void launch_convert(hipStream_t st, TO *out, const TI *in, size_t n);
+void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
+
+}  // namespace lsmp

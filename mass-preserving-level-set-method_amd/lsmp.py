@@ -1,0 +1,234 @@
+"""Python host binding of the gfx950 level-set transport library (liblsmp.so).
+
+Mirrors the C ABI of include/lsmp.h one to one; each method names the
+reference routine it stands in for.  Arrays cross the boundary as FP64 host
+boxes in the reference's storage contract: column-major (i fastest) over
+(1-3 : n+3) per direction, 7 z planes in 2D.  In numpy terms a box is a
+C-contiguous float64 array of shape (nz+6, ny+6, nx+6) (or (7, ny+6, nx+6)).
+
+There is no CPU fallback: constructing a Solver without the built library or
+without a GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "liblsmp.so")
+
+FP64, FP32 = 0, 1
+VEL_ROTATION, VEL_VORTEX2D, VEL_LEVEQUE3D = 0, 1, 2
+
+ERRORS = {0: "ok", -1: "invalid parameter or size", -2: "HIP runtime error",
+          -3: "non-finite value", -4: "empty interface band", -5: "call out of order"}
+
+# Every entry point declared in include/lsmp.h.
+EXPORTS = [
+    "lsmp_init", "lsmp_finalize", "lsmp_error_string", "lsmp_upload_phi", "lsmp_download_phi",
+    "lsmp_store_phi0", "lsmp_set_velocity", "lsmp_set_velocity_analytic", "lsmp_rk3_advect",
+    "lsmp_reinit", "lsmp_mass_vol", "lsmp_mass_fix", "lsmp_interface_l1", "lsmp_get_stats",
+    "lsmp_set_profiling", "lsmp_kernel_time", "lsmp_synchronize",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ghc", C.c_int), ("dim", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("dt", C.c_double), ("rdt", C.c_double), ("ls_wid", C.c_double),
+        ("rho_12", C.c_double), ("inverse", C.c_int), ("precision", C.c_int),
+        ("xstart", C.c_double), ("xend", C.c_double), ("ystart", C.c_double),
+        ("yend", C.c_double), ("zstart", C.c_double), ("zend", C.c_double),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("last_redis_iters", C.c_int), ("last_mass_iters", C.c_int),
+        ("last_red_error", C.c_double), ("t_advect_ms", C.c_double),
+        ("t_reinit_ms", C.c_double), ("t_massfix_ms", C.c_double),
+        ("n_advect", C.c_longlong), ("n_reinit", C.c_longlong), ("n_massfix", C.c_longlong),
+    ]
+
+
+_DP = C.POINTER(C.c_double)
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load liblsmp.so and declare its signatures.  Raises when it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    H = C.c_void_p
+    L.lsmp_init.argtypes = [C.POINTER(Params), C.POINTER(H)]
+    L.lsmp_finalize.argtypes = [H]
+    L.lsmp_finalize.restype = None
+    L.lsmp_error_string.argtypes = [C.c_int]
+    L.lsmp_error_string.restype = C.c_char_p
+    L.lsmp_upload_phi.argtypes = [H, _DP]
+    L.lsmp_download_phi.argtypes = [H, _DP]
+    L.lsmp_store_phi0.argtypes = [H]
+    L.lsmp_set_velocity.argtypes = [H, _DP, _DP, _DP]
+    L.lsmp_set_velocity_analytic.argtypes = [H, C.c_int, C.c_double, C.c_double]
+    L.lsmp_rk3_advect.argtypes = [H, C.c_double]
+    L.lsmp_reinit.argtypes = [H, C.c_int, C.c_double, _DP, C.POINTER(C.c_int)]
+    L.lsmp_mass_vol.argtypes = [H, _DP, _DP]
+    L.lsmp_mass_fix.argtypes = [H, C.c_double, _DP, C.POINTER(C.c_int)]
+    L.lsmp_interface_l1.argtypes = [H, _DP]
+    L.lsmp_get_stats.argtypes = [H, C.POINTER(Stats)]
+    L.lsmp_set_profiling.argtypes = [H, C.c_int]
+    L.lsmp_kernel_time.argtypes = [H, C.c_char_p, C.POINTER(C.c_longlong), _DP]
+    L.lsmp_synchronize.argtypes = [H]
+    for name in EXPORTS:
+        if name not in ("lsmp_finalize", "lsmp_error_string"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+class LsmpError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load_library().lsmp_error_string(rc).decode()
+        raise LsmpError(f"lsmp error {rc}: {msg}")
+
+
+def _box_ptr(a):
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("boxes must be C-contiguous float64 arrays")
+    return a.ctypes.data_as(_DP)
+
+
+def make_params(n, dim=3, dt_cfl=0.1, rdt_cfl=0.1, ls_wid_cells=1.5, rho_12=1.0, inverse=0,
+                precision=FP64, ny=None, nz=None):
+    """Parameters derived the way the reference derives them from its deck
+    (Vortex/lib/data/tree.f90:230-280) on the unit box."""
+    ny = n if ny is None else ny
+    nz = (n if nz is None else nz) if dim == 3 else 1
+    dx = 1.0 / n
+    return Params(nx=n, ny=ny, nz=nz, ghc=3, dim=dim, dx=dx, dy=1.0 / ny,
+                  dz=(1.0 / nz) if dim == 3 else 1.0, dt=dt_cfl * dx, rdt=rdt_cfl * dx,
+                  ls_wid=ls_wid_cells * dx, rho_12=rho_12, inverse=inverse, precision=precision,
+                  xstart=0.0, xend=1.0, ystart=0.0, yend=1.0, zstart=0.0, zend=1.0)
+
+
+class Solver:
+    """One device-resident level-set state (the job's phi, nvel and scratch)."""
+
+    def __init__(self, params):
+        self.lib = load_library()
+        self.params = params
+        self.h = C.c_void_p()
+        _check(self.lib.lsmp_init(C.byref(params), C.byref(self.h)))
+        self.nzb = params.nz + 6 if params.dim == 3 else 7
+        self.shape = (self.nzb, params.ny + 6, params.nx + 6)
+
+    def close(self):
+        if self.h:
+            self.lib.lsmp_finalize(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- data in / out
+    def upload_phi(self, box):
+        _check(self.lib.lsmp_upload_phi(self.h, _box_ptr(np.ascontiguousarray(box, np.float64))))
+
+    def download_phi(self):
+        out = np.empty(self.shape, np.float64)
+        _check(self.lib.lsmp_download_phi(self.h, _box_ptr(out)))
+        return out
+
+    def store_phi0(self):
+        _check(self.lib.lsmp_store_phi0(self.h))
+
+    def set_velocity(self, u, v, w=None):
+        u = np.ascontiguousarray(u, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        w = np.zeros_like(u) if w is None else np.ascontiguousarray(w, np.float64)
+        _check(self.lib.lsmp_set_velocity(self.h, _box_ptr(u), _box_ptr(v), _box_ptr(w)))
+
+    def set_velocity_analytic(self, field, t, t2s):
+        _check(self.lib.lsmp_set_velocity_analytic(self.h, field, t, t2s))
+
+    # -- the hot path
+    def rk3_advect(self, dt=None):
+        """level_set_rk3_solver (level_set_rk3_solver.f90:1-94)."""
+        _check(self.lib.lsmp_rk3_advect(self.h, self.params.dt if dt is None else dt))
+
+    def reinit(self, btn=1, timestop=-1.0):
+        """level_set_rk3_redis(btn[,T]) (level_set_rk3_redis.f90:1-79).
+        Returns (red_error, pseudo-steps)."""
+        err = C.c_double()
+        n = C.c_int()
+        _check(self.lib.lsmp_reinit(self.h, btn, timestop, C.byref(err), C.byref(n)))
+        return err.value, n.value
+
+    def mass_vol(self):
+        """ls_mv, LS part (ns_solver/others.f90:172-213).  Returns (mass, vol)."""
+        m = C.c_double()
+        v = C.c_double()
+        _check(self.lib.lsmp_mass_vol(self.h, C.byref(m), C.byref(v)))
+        return m.value, v.value
+
+    def mass_fix(self, imass):
+        """mass_preserving_level_set (mass_preserving_level_set.f90:1-74).
+        Returns (mass before the last correction, iterations)."""
+        m = C.c_double()
+        n = C.c_int()
+        _check(self.lib.lsmp_mass_fix(self.h, imass, C.byref(m), C.byref(n)))
+        return m.value, n.value
+
+    def interface_l1(self):
+        """calculate_interface_loss, LS part (output.f90:169-204, 226)."""
+        v = C.c_double()
+        _check(self.lib.lsmp_interface_l1(self.h, C.byref(v)))
+        return v.value
+
+    # -- accounting
+    def stats(self):
+        s = Stats()
+        _check(self.lib.lsmp_get_stats(self.h, C.byref(s)))
+        return s
+
+    def set_profiling(self, on=True):
+        _check(self.lib.lsmp_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        n = C.c_longlong()
+        ms = C.c_double()
+        _check(self.lib.lsmp_kernel_time(self.h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def synchronize(self):
+        _check(self.lib.lsmp_synchronize(self.h))
+
+    def mpls_step(self, t, t2s, field, imass, t2red=1, iteration=1):
+        """One MPLS step as the merged branch of mpls runs it
+        (interface_solver.f90:123-129) after the driver set the velocity at
+        t^{n+1} (main.f90:23-29).  Returns (n_pseudo, n_mass)."""
+        self.set_velocity_analytic(field, t, t2s)
+        self.rk3_advect()
+        npseudo = 0
+        if iteration % t2red == 0:
+            _, npseudo = self.reinit(1)
+        _, nm = self.mass_fix(imass)
+        return npseudo, nm

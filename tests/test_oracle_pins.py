@@ -1,0 +1,408 @@
+"""Pins for the CPU oracle (oracle/lsmp_oracle.c).
+
+The oracle is only trusted as a checker once each of its functions is tied to
+something outside itself: the reference's own Fortran units (golden vectors
+in tests/golden/ref_vectors.npz and, when built, oracle/_ref), closed forms,
+polynomial exactness, invariants, or brute force on tiny inputs.  Each test
+names what it pins.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD, allow_pickle=False)
+
+
+def dp(a):
+    return O._dp(a)
+
+
+# ---------------------------------------------------------------- Thomas (TDMA.f90:1-36)
+@pytest.mark.parametrize("tag", ["tdma_dd", "tdma_crw"])
+def test_thomas_bitexact_vs_reference_vectors(gold, tag):
+    A, B, C, S = (gold[f"{tag}_{k}"].copy() for k in "ABCS")
+    n = A.size
+    X = np.zeros(n)
+    O.lib().orc_solve_tridiagonal(dp(A), dp(B), dp(C), dp(S), dp(X), n)
+    assert np.array_equal(X, gold[f"{tag}_X"])
+
+
+def test_thomas_vs_dense_solve():
+    rng = np.random.default_rng(1)
+    n = 50
+    A = rng.uniform(-0.4, 0.4, n); C = rng.uniform(-0.4, 0.4, n)
+    B = np.abs(A) + np.abs(C) + 0.5
+    S = rng.normal(size=n)
+    M = np.diag(B) + np.diag(A[1:], -1) + np.diag(C[:-1], 1)
+    want = np.linalg.solve(M, S)
+    X = np.zeros(n)
+    O.lib().orc_solve_tridiagonal(dp(A.copy()), dp(B.copy()), dp(C.copy()), dp(S.copy()), dp(X), n)
+    assert np.max(np.abs(X - want)) < 1e-13 * np.max(np.abs(want))
+
+
+def test_thomas_live_reference_unit():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+    rng = np.random.default_rng(7)
+    for n in (2, 5, 255):
+        w = rng.dirichlet([0.3, 1.0, 1.0], n)
+        A = (2 * w[:, 0] + w[:, 1]) / 3; B = (w[:, 0] + 2 * (w[:, 1] + w[:, 2])) / 3; C = w[:, 2] / 3
+        S = rng.normal(size=n)
+        X1, X2 = np.zeros(n), np.zeros(n)
+        O.lib().orc_solve_tridiagonal(dp(A.copy()), dp(B.copy()), dp(C.copy()), dp(S.copy()), dp(X1), n)
+        R.ref_solve_tridiagonal(dp(A.copy()), dp(B.copy()), dp(C.copy()), dp(S.copy()), dp(X2), n)
+        assert np.array_equal(X1, X2)
+
+
+# ---------------------------------------------------------------- CCD (ccd_solvers.f90, matrix_solver.f90)
+@pytest.mark.parametrize("n", [7, 38, 262])
+def test_ccd_bitexact_vs_reference_vectors(gold, n):
+    f = gold[f"ccd{n}_f"]
+    dx = float(gold[f"ccd{n}_dx"])
+    fx, fxx = np.zeros(n), np.zeros(n)
+    O.lib().orc_ccd_line(n, dx, dp(f.copy()), dp(fx), dp(fxx))
+    assert np.array_equal(fx, gold[f"ccd{n}_fx"])
+    assert np.array_equal(fxx, gold[f"ccd{n}_fxx"])
+
+
+def test_ccd_live_reference_unit():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+    rng = np.random.default_rng(3)
+    for n in (7, 10, 134, 518):
+        dx = 1.0 / (n - 6)
+        f = rng.normal(size=n)
+        a, b, c, d = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        O.lib().orc_ccd_line(n, dx, dp(f.copy()), dp(a), dp(b))
+        R.ref_ccd_line(n, dx, dp(f.copy()), dp(c), dp(d))
+        assert np.array_equal(a, c) and np.array_equal(b, d)
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+def test_ccd_polynomial_exactness(deg):
+    """The one-sided closures are exact through degree 4 (first-derivative row)
+    and 5 (second), the interior rows beyond: the whole line solve must
+    return the exact derivatives of a quartic.  A wrong closure or interior
+    coefficient breaks this."""
+    rng = np.random.default_rng(deg)
+    n, dx = 40, 0.03
+    x = np.arange(n) * dx - 0.4
+    c = rng.normal(size=deg + 1)
+    f = np.polyval(c, x)
+    want = np.polyval(np.polyder(c), x)
+    want2 = np.polyval(np.polyder(c, 2), x)
+    fx, fxx = np.zeros(n), np.zeros(n)
+    O.lib().orc_ccd_line(n, dx, dp(f), dp(fx), dp(fxx))
+    scale = np.max(np.abs(want)) + 1.0
+    assert np.max(np.abs(fx - want)) < 1e-10 * scale
+    assert np.max(np.abs(fxx - want2)) < 1e-8 * (np.max(np.abs(want2)) + 1.0)
+
+
+def test_ccd_not_exact_for_degree5():
+    """Control for the test above: degree 5 is beyond the closure order."""
+    n, dx = 40, 0.03
+    x = np.arange(n) * dx - 0.4
+    f = x ** 5
+    fx, fxx = np.zeros(n), np.zeros(n)
+    O.lib().orc_ccd_line(n, dx, dp(f), dp(fx), dp(fxx))
+    assert np.max(np.abs(fx - 5 * x ** 4)) > 1e-9
+
+
+# ---------------------------------------------------------------- CRWENO5 / WENO-JS
+def _line(vals):
+    return np.ascontiguousarray(vals, dtype=np.float64)
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2])
+def test_crweno_exact_for_quadratics(deg):
+    """Every candidate stencil is third order, so any convex combination of
+    them, and therefore the compact system whatever its nonlinear weights,
+    reproduces face values of the primitive-matched quadratic exactly:
+    h(x) = g(x) - dx^2/24 g''.  Checks both split systems and the WENO-JS
+    boundary faces.  A dropped or wrong coefficient in A, B, C or S fails."""
+    rng = np.random.default_rng(10 + deg)
+    n, dx = 30, 0.05
+    xc = (np.arange(-3, n + 3) + 0.5) * dx       # Fortran cells -2 .. n+3 at (c - 1/2) dx
+    c = rng.normal(size=3)
+    c[: 2 - deg] = 0.0
+    g = np.polyval(c, xc)
+    f = np.polyval(c[::-1] * [1, -1, 1], xc)
+    fp, gm = np.zeros(n + 6), np.zeros(n + 6)
+    O.lib().orc_crweno_flux_split(dp(_line(f)), dp(_line(g)), dp(fp), dp(gm), n)
+    faces = np.arange(0, n + 1) * dx          # face m between Fortran cells m and m+1
+    d2 = lambda cc: 2 * cc[0]
+    want_g = np.polyval(c, faces) - dx * dx / 24 * d2(c)
+    cf = c[::-1] * [1, -1, 1]
+    want_f = np.polyval(cf, faces) - dx * dx / 24 * d2(cf)
+    got_g = gm[2:n + 3]
+    got_f = fp[2:n + 3]
+    assert np.max(np.abs(got_g - want_g)) < 1e-12 * (1 + np.max(np.abs(want_g)))
+    assert np.max(np.abs(got_f - want_f)) < 1e-12 * (1 + np.max(np.abs(want_f)))
+
+
+def test_crweno_order_of_accuracy():
+    """Smooth data: the nonlinear weights approach the scheme's constants and
+    the face error shrinks at fourth order or better."""
+    errs = []
+    for n in (40, 80, 160):
+        dx = 1.0 / n
+        xc = (np.arange(-3, n + 3) + 0.5) * dx
+        # point values g_i whose primitive-matched reconstruction at faces is
+        # the exact face value of G' where G = -cos(2 pi x)/(2 pi): choose
+        # g = cell average of sin(2 pi x) over [x-dx/2, x+dx/2]
+        g = (np.cos(2 * np.pi * (xc - dx / 2)) - np.cos(2 * np.pi * (xc + dx / 2))) / (2 * np.pi * dx)
+        f = np.zeros_like(g)
+        fp, gm = np.zeros(n + 6), np.zeros(n + 6)
+        O.lib().orc_crweno_flux_split(dp(_line(f)), dp(_line(g)), dp(fp), dp(gm), n)
+        faces = np.arange(0, n + 1) * dx
+        errs.append(np.max(np.abs(gm[2 + 10:n + 3 - 10] - np.sin(2 * np.pi * faces[10:-10]))))
+    order = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(order > 3.8), (errs, order)
+
+
+def test_crweno_step_bounded():
+    """A step is reconstructed without large over- or undershoot."""
+    n = 40
+    g = np.where(np.arange(n + 6) < 23, 1.0, 0.0)
+    fp, gm = np.zeros(n + 6), np.zeros(n + 6)
+    O.lib().orc_crweno_flux_split(dp(_line(np.zeros(n + 6))), dp(_line(g)), dp(fp), dp(gm), n)
+    faces = gm[2:n + 3]
+    assert faces.max() < 1.05 and faces.min() > -0.05
+
+
+# ---------------------------------------------------------------- WENO-HJ (redis.f90:266-369, 456-471)
+def test_phyn_zero_on_linear_data():
+    assert O.lib().orc_phyn(0.0, 0.0, 0.0, 0.0) == 0.0
+
+
+@pytest.mark.parametrize("deg", [1, 2])
+def test_weno_hj_exact_slopes(deg):
+    """D+ and D- are exact for quadratics (each candidate is third order), so
+    the Godunov |grad| of phi = q(x) + a y + b z equals the exact slope
+    magnitude away from the clamped boundary."""
+    n = 24
+    p = O.make_params(n, dim=3)
+    o = O.Oracle(p)
+    z, y, x = np.meshgrid(*[(np.arange(-3, n + 3) + 0.5) / n] * 3, indexing="ij")
+    c2 = 0.7 if deg == 2 else 0.0
+    phi = c2 * x * x + 0.3 * x - 0.4 * y + 0.5 * z
+    o.field("PHI")[:] = phi
+    o.field("SIGN_TMP")[:] = 1.0
+    o.redis_gradient()
+    g = o.interior("GRAD")
+    zi, yi, xi = np.meshgrid(*[(np.arange(n) + 0.5) / n] * 3, indexing="ij")
+    want = np.sqrt((2 * c2 * xi + 0.3) ** 2 + 0.4 ** 2 + 0.5 ** 2)
+    core = (slice(4, -4),) * 3
+    assert np.max(np.abs(g[core] - want[core])) < 1e-12
+
+
+# ---------------------------------------------------------------- ls_funs (others.f90:1-45)
+def test_ls_funs_closed_form_properties():
+    p = O.make_params(8, dim=3)
+    o = O.Oracle(p)
+    eps = p.ls_wid
+    phi = o.field("PHI")
+    rng = np.random.default_rng(5)
+    phi[:] = rng.uniform(-2 * eps, 2 * eps, size=phi.shape)
+    phi.flat[0] = 0.0
+    o.ls_funs()
+    H, D, S = o.field("HEAVY"), o.field("DELTA"), o.field("SIGN")
+    assert H.flat[0] == 0.5 and S.flat[0] == 0.0
+    assert np.all((H >= 0) & (H <= 1))
+    assert np.array_equal(S, 2 * H - 1)
+    out = np.abs(phi) >= eps
+    assert np.all(D[out] == 0) and np.all((H[out] == 0) | (H[out] == 1))
+    # symmetry H(-x) = 1 - H(x)
+    phi[:] = -phi
+    o.ls_funs()
+    H2 = o.field("HEAVY").copy()
+    phi[:] = -phi
+    o.ls_funs()
+    assert np.max(np.abs(H2 + o.field("HEAVY") - 1)) < 1e-15
+    # delta is dH/dphi: compare with a centred difference
+    base = o.field("PHI").copy()
+    h = 1e-7 * eps
+    phi[:] = base + h
+    o.ls_funs()
+    Hp = o.field("HEAVY").copy()
+    phi[:] = base - h
+    o.ls_funs()
+    Hm = o.field("HEAVY").copy()
+    phi[:] = base
+    o.ls_funs()
+    inband = np.abs(base) < 0.9 * eps
+    fd = (Hp - Hm) / (2 * h)
+    assert np.max(np.abs(fd[inband] - o.field("DELTA")[inband])) < 1e-6 / eps
+
+
+# ---------------------------------------------------------------- ghost fill (branches.f90:275-320)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bc_equals_index_clamp(dim):
+    """The x, then y, then z zero-gradient fill leaves every ghost equal to the
+    interior cell at the clamped index; the device layout relies on it."""
+    p = O.make_params(6, dim=dim, ny=5, nz=7)
+    o = O.Oracle(p)
+    a = o.field("PHI")
+    rng = np.random.default_rng(2)
+    a[:] = rng.normal(size=a.shape)
+    o.bc_field("PHI")
+    nzb, nyb, nxb = a.shape
+    kk = np.clip(np.arange(nzb) - 3, 0, (p.nz - 1)) + 3
+    jj = np.clip(np.arange(nyb) - 3, 0, p.ny - 1) + 3
+    ii = np.clip(np.arange(nxb) - 3, 0, p.nx - 1) + 3
+    want = a[np.ix_(kk, jj, ii)]
+    assert np.array_equal(a, want)
+
+
+# ---------------------------------------------------------------- advection + RK3
+def test_advection_constant_state_is_steady():
+    p = O.make_params(12, dim=3)
+    o = O.Oracle(p)
+    o.field("PHI")[:] = 0.37
+    o.field("U")[:] = 0.8; o.field("V")[:] = -0.3; o.field("W")[:] = 0.5
+    o.rk3_advect()
+    assert np.max(np.abs(o.interior("PHI") - 0.37)) < 1e-15
+
+
+def test_rk3_exact_for_quadratic_in_time():
+    """phi = (x - 0.5)^2 carried by u = 1 stays (x - u t - 0.5)^2; the flux is
+    quadratic so CRWENO is exact up to the boundary closure (whose effect
+    decays into the interior), and SSP-RK3 is exact for a solution quadratic
+    in t.  Pins the three stage combinations."""
+    n = 160
+    p = O.make_params(n, dim=3, nz=8, ny=8, dt_cfl=0.4)
+    p.dy = p.dz = 1.0 / n
+    o = O.Oracle(p)
+    xb = (np.arange(-3, n + 3) + 0.5) / n
+    o.field("PHI")[:] = ((xb - 0.5) ** 2)[None, None, :]
+    o.field("U")[:] = 1.0
+    o.rk3_advect()
+    xi = (np.arange(n) + 0.5) / n
+    want = (xi - p.dt - 0.5) ** 2
+    got = o.interior("PHI")[4, 4, :]
+    # the compact system spreads the clamped-boundary error by ~0.55 per cell
+    assert np.max(np.abs(got[60:100] - want[60:100])) < 1e-13
+
+
+# ---------------------------------------------------------------- reinit
+def test_reinit_keeps_planar_sdf():
+    """An exact planar signed distance (|grad| = 1) has l = (S0 - lambda) 1 - S0
+    with lambda = 0, so the interior does not move."""
+    n = 32
+    p = O.make_params(n, dim=3)
+    o = O.Oracle(p)
+    zb, yb, xb = np.meshgrid(*[(np.arange(-3, n + 3) + 0.5) / n] * 3, indexing="ij")
+    o.field("PHI")[:] = 0.5 - xb
+    niter = o.redis(1, -1.0)
+    assert niter == 1   # error is ~0 after one pseudo-step
+    xi = (np.arange(n) + 0.5) / n
+    core = o.interior("PHI")[6:-6, 6:-6, 6:-6]
+    assert np.max(np.abs(core - (0.5 - xi[6:-6])[None, None, :])) < 1e-14
+
+
+def test_reinit_drives_gradient_to_one():
+    n = 40
+    p = O.make_params(n, dim=2, rdt_cfl=0.5)
+    o = O.Oracle(p)
+    yb, xb = np.meshgrid(*[(np.arange(-3, n + 3) + 0.5) / n] * 2, indexing="ij")
+    r = np.sqrt((xb - 0.5) ** 2 + (yb - 0.5) ** 2)
+    o.field("PHI")[:] = ((0.25 - r) * (1 + 2 * (xb - 0.5) ** 2))[None]
+    o.redis(1, 40 * p.rdt)
+    o.field("SIGN_TMP")[:] = np.sign(o.field("PHI"))
+    o.redis_gradient()
+    g = o.interior("GRAD")[0]
+    yi, xi = np.meshgrid((np.arange(n) + 0.5) / n, (np.arange(n) + 0.5) / n, indexing="ij")
+    band = np.abs(np.sqrt((xi - 0.5) ** 2 + (yi - 0.5) ** 2) - 0.25) < 4.0 / n
+    assert np.max(np.abs(g[band] - 1.0)) < 0.05
+
+
+# ---------------------------------------------------------------- mass fix and metrics
+def test_mass_fix_restores_mass():
+    """Newton-type correction: after it, a fresh ls_mv equals the target
+    (mass_preserving_level_set.f90:46-70)."""
+    n = 24
+    p = O.make_params(n, dim=3)
+    o = O.Oracle(p)
+    o.initial_phi(2, 0)
+    o.store_initial()
+    imass = o.get("imass")
+    o.field("PHI")[:] += 0.3 * p.dx     # shrink the sphere's complement: mass grows
+    o.ls_mv()
+    assert abs(o.get("mass") - imass) / imass > 1e-3
+    it = o.mass_fix()
+    o.ls_mv()
+    assert 2 <= it <= 6
+    assert abs(o.get("mass") - imass) / imass < 1e-13
+
+
+def test_ls_mv_density_ratio_identity():
+    """With rho_12 = 1 the density is 1 and mass equals volume; with
+    rho_12 = 0 mass is sum H^2 dV (others.f90:196-201)."""
+    n = 16
+    for r12 in (1.0, 0.0):
+        p = O.make_params(n, dim=3, rho_12=r12)
+        o = O.Oracle(p)
+        o.initial_phi(2, 0)
+        o.ls_mv()
+        H = o.interior("HEAVY")
+        dv = p.dx * p.dy * p.dz
+        if r12 == 1.0:
+            assert abs(o.get("mass") - o.get("vol")) < 1e-15
+        else:
+            assert abs(o.get("mass") - np.sum(H * H) * dv) < 1e-14
+        assert abs(o.get("vol") - np.sum(H) * dv) < 1e-14
+
+
+def test_interface_l1_properties():
+    n = 20
+    p = O.make_params(n, dim=3)
+    o = O.Oracle(p)
+    o.initial_phi(2, 0)
+    o.store_initial()
+    assert o.interface_l1() == 0.0
+    o.field("PHI")[:] += p.ls_wid * 10   # everything inside: H = 1 everywhere
+    H0 = np.where(o.field("PHI0") / p.ls_wid > 1 - 1e-12, 1.0, 0.0)
+    l1 = o.interface_l1()
+    assert 0 < l1 <= 1.0
+
+
+def test_initial_sphere_and_velocity():
+    n = 16
+    p = O.make_params(n, dim=3)
+    o = O.Oracle(p)
+    o.initial_phi(2, 0)
+    xi = (np.arange(n) + 0.5) / n
+    z, y, x = np.meshgrid(xi, xi, xi, indexing="ij")
+    c = float(np.float32(0.35)); r = float(np.float32(0.15))
+    want = -np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) + r
+    assert np.max(np.abs(o.interior("PHI") - want)) < 1e-15
+    o.velocity(2, 0.75, 3.0)
+    ct = np.cos(np.pi * 0.75 / 3.0)
+    u = 2 * np.sin(np.pi * x) ** 2 * np.sin(2 * np.pi * y) * np.sin(2 * np.pi * z) * ct
+    assert np.max(np.abs(o.interior("U") - u)) < 1e-15
+    # the LeVeque field is divergence free: central differences vanish to O(dx^2)
+    U, V, W = o.interior("U"), o.interior("V"), o.interior("W")
+    div = ((U[1:-1, 1:-1, 2:] - U[1:-1, 1:-1, :-2]) + (V[1:-1, 2:, 1:-1] - V[1:-1, :-2, 1:-1]) +
+           (W[2:, 1:-1, 1:-1] - W[:-2, 1:-1, 1:-1])) * n / 2
+    assert np.max(np.abs(div)) < 0.2
+
+
+def test_full_step_conserves_mass():
+    n = 16
+    p = O.make_params(n, dim=3)
+    o = O.Oracle(p)
+    o.initial_phi(2, 0)
+    o.store_initial()
+    o.mpls_step(2, 3.0, 1)
+    assert o.get("redis_iters") >= 1 and o.get("mass_iters") >= 1
+    assert abs(o.get("mass") - o.get("imass")) / o.get("imass") < 1e-12
