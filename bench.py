@@ -1,0 +1,202 @@
+"""Throughput of the MPLS interface-transport step on one MI355X.
+
+Workload: the 3D LeVeque deformation at 256^3 -- sphere r = 0.15 at
+(0.35, 0.35, 0.35), velocity x cos(pi t / T) with T = 3, dt = rdt = 0.1 dx,
+ls_wid = 1.5 dx, reinitialisation every step, rho_12 = 1.  One step = the
+hot path of one physical step: device velocity at t^{n+1}, CRWENO5 TVD-RK3
+advection, reinitialisation (btn = 1), mass correction.  Inputs are resident
+in HBM before the timed region starts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--precision fp64|fp32]
+
+Prints one JSON line (metric, value = cell-updates/s, roofline of the dominant
+kernel from HIP-event timing, CPU baseline of the oracle on the host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "mass-preserving-level-set-method_amd")
+METRIC = "cell-updates/s per RK3 step (adv+reinit+mass fix) at 256^3; L1 shape err, |dM|/M"
+HBM_PEAK_GBS = 8000.0
+T2S = 3.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_lsmp():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("lsmp", os.path.join(PKG, "lsmp.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["lsmp"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def sphere_box(n):
+    """phi0 = 0.15 - |x - (0.35, 0.35, 0.35)| on the ghosted box with the
+    reference's single-precision literals (Vortex/src/setup/problem_init.f90:69)."""
+    xb = (np.arange(-3, n + 3) + 0.5) / n
+    c, r = float(np.float32(0.35)), float(np.float32(0.15))
+    z, y, x = np.meshgrid(xb, xb, xb, indexing="ij", sparse=True)
+    box = -np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) + r
+    # ghosts are copies of the nearest interior cell (zero-gradient fill)
+    idx = np.clip(np.arange(n + 6) - 3, 0, n - 1) + 3
+    return np.ascontiguousarray(box[np.ix_(idx, idx, idx)])
+
+
+# Algorithmic bytes per interior cell per launch, in units of V = sizeof(real):
+# the minimum each kernel must read and write (scratch of the line solves and
+# halo re-reads are not counted).
+BYTES_V = {
+    "adv_x": 3, "adv_y": 4,            # 3D y pass accumulates into R
+    "redis_stage": 11.0 / 3.0,         # stages read phi,S0[,phi0], write phi: 3, 4, 4
+    "ccd_x": 2, "ccd_y": 2, "ccd_z": 4, "mass_update": 3,
+    "velocity": 3, "sign_init": 2,
+}
+ADV_Z_V = (5 + 6 + 6) / 3.0            # last pass: R, w, phi (+ l1, l2) in, phi (+ l1/l2) out
+
+
+def run_steps(s, nsteps, t, imass, field):
+    npseudo = nmass = 0
+    for _ in range(nsteps):
+        t = t + s.params.dt
+        a, b = s.mpls_step(t, T2S, field, imass)
+        npseudo += a
+        nmass += b
+    return t, npseudo, nmass
+
+
+def cpu_baseline(n_sample=64, steps=3):
+    """The FP64 C oracle (one host thread) on a bounded sample of the same
+    workload: `steps` MPLS steps of the LeVeque sphere at n_sample^3."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    o = O.Oracle(O.make_params(n_sample, dim=3))
+    o.initial_phi(2, 0)
+    o.store_initial()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.mpls_step(2, T2S, 1)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n_sample ** 3 * steps / dt,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} MPLS steps of the same LeVeque sphere case at {n_sample}^3 "
+                  f"(scalar FP64 C oracle, {dt:.1f} s; per-cell work does not depend on the grid size: "
+                  f"{int(o.get('redis_iters'))} pseudo-steps, {int(o.get('mass_iters'))} mass iterations)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's sharding "
+              "across GPUs is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    lsmp = load_lsmp()
+    n = args.n
+    prec = lsmp.FP64 if args.precision == "fp64" else lsmp.FP32
+    s = lsmp.Solver(lsmp.make_params(n, dim=3, precision=prec))
+    s.upload_phi(sphere_box(n))
+    s.store_phi0()
+    imass, _ = s.mass_vol()
+    log(f"bench: {n}^3 {args.precision}, imass {imass:.9e}; warmup {args.warmup} steps")
+
+    t = 0.0
+    t, _, _ = run_steps(s, args.warmup, t, imass, lsmp.VEL_LEVEQUE3D)
+    s.synchronize()
+    log(f"bench: timing {args.steps} steps")
+    t0 = time.perf_counter()
+    t, npseudo, nmass = run_steps(s, args.steps, t, imass, lsmp.VEL_LEVEQUE3D)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = n ** 3 * args.steps / elapsed
+    mass, _ = s.mass_vol()
+    l1 = s.interface_l1()
+    log(f"bench: {ms_per_step:.2f} ms/step, {value:.3e} cell-updates/s, "
+        f"pseudo-steps/step {npseudo / args.steps:.1f}, mass iters/step {nmass / args.steps:.2f}")
+
+    # per-kernel device time with HIP events on the library's stream
+    s.set_profiling(True)
+    t, pp, pm = run_steps(s, args.profile_steps, t, imass, lsmp.VEL_LEVEQUE3D)
+    kernels = {}
+    for name in ("adv_x", "adv_y", "adv_z", "redis_stage", "sign_init", "ccd_x", "ccd_y", "ccd_z",
+                 "finalize", "mass_update", "velocity"):
+        cnt, ms = s.kernel_time(name)
+        if cnt:
+            kernels[name] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / cnt, 2)}
+    s.set_profiling(False)
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    V = 8 if prec == lsmp.FP64 else 4
+    bv = ADV_Z_V if dom == "adv_z" else BYTES_V.get(dom, 0.0)
+    bytes_per_launch = bv * V * n ** 3
+    avg_s = kernels[dom]["avg_us"] * 1e-6
+    achieved = bytes_per_launch / avg_s / 1e9
+    for k, v in kernels.items():
+        b = (ADV_Z_V if k == "adv_z" else BYTES_V.get(k, 0.0)) * V * n ** 3
+        v["algo_GBs"] = round(b / (v["avg_us"] * 1e-6) / 1e9, 1) if b else None
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == lsmp.FP64 else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D LeVeque deformation {n}^3, sphere r=0.15 at (0.35,0.35,0.35), T=3, "
+                        f"dt=rdt=0.1dx, ls_wid=1.5dx, reinit every step, MPLS, "
+                        f"{'FP64 plain' if prec == lsmp.FP64 else 'FP32 bulk + FP64 mass sums'}",
+            "grid": [n, n, n],
+            "timed_steps_from_t": round(t - (args.steps + args.profile_steps) * s.params.dt, 6),
+        },
+        "l1_shape_err": l1,
+        "dM_over_M": abs(mass - imass) / imass,
+        "pseudo_steps_per_step": npseudo / args.steps,
+        "mass_iters_per_step": nmass / args.steps,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "kernels": kernels,
+    }
+    s.close()
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle on the host)")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -118,6 +118,55 @@ template <typename T> LSMP_DEV T phyn(T a, T b, T c, T d)
     return w0 / T(3) * (a - T(2) * b + c) + (w2 - T(0.5)) / T(6) * (b - T(2) * c + d);
 }
 
+// The same WENO-HJ term with its five divisions folded into one reciprocal:
+// with q_k = (eps + IS_k)^2 the reference's alp0 = 1/q0, alp1 = 6/q1,
+// alp2 = 3/q2 normalise to w0 = q1 q2 / D and w2 = 3 q0 q1 / D with
+// D = q1 q2 + 6 q0 q2 + 3 q0 q1.  Equal to phyn() up to rounding of the weights
+// (q_k lies in [1e-12, 3e6], so D neither under- nor overflows in FP32).
+template <typename T> LSMP_DEV T phyn_fast(T a, T b, T c, T d)
+{
+    const T eps = T(1.0e-6);
+    T is0 = T(13) * sq(a - b) + T(3) * sq(a - T(3) * b);
+    T is1 = T(13) * sq(b - c) + T(3) * sq(b + c);
+    T is2 = T(13) * sq(c - d) + T(3) * sq(T(3) * c - d);
+    T q0 = sq(eps + is0), q1 = sq(eps + is1), q2 = sq(eps + is2);
+    T q12 = q1 * q2, q01 = q0 * q1;
+    T r = T(1) / (q12 + T(6) * (q0 * q2) + T(3) * q01);
+    T w0 = q12 * r, w2 = T(3) * q01 * r;
+    return w0 * T(1.0 / 3.0) * (a - T(2) * b + c) + (w2 - T(0.5)) * T(1.0 / 6.0) * (b - T(2) * c + d);
+}
+
+// D+ and D- from the five second differences s_m = phi(m+1) - 2 phi(m) + phi(m-1),
+// m = i-2..i+2, shared by both one-sided terms.
+template <typename T>
+LSMP_DEV void weno_hj_pm_fast(T m3, T m2, T m1, T c, T p1, T p2, T p3, T inv12d, T invd, T &dp, T &dm)
+{
+    const T a = inv12d * (-(m1 - m2) + T(7) * (c - m1) + T(7) * (p1 - c) - (p2 - p1));
+    const T sm2 = m3 - T(2) * m2 + m1, sm1 = m2 - T(2) * m1 + c, s0 = m1 - T(2) * c + p1;
+    const T sp1 = c - T(2) * p1 + p2, sp2 = p1 - T(2) * p2 + p3;
+    dp = a + invd * phyn_fast(sp2, sp1, s0, sm1);
+    dm = a - invd * phyn_fast(sm2, sm1, s0, sp1);
+}
+
+// ls_funs with the divisions by ls_wid and pi replaced by products with
+// precomputed reciprocals.
+template <typename T> LSMP_DEV void ls_funs_fast(T phi, T inv_w, T &H, T &D)
+{
+    const T eps = T(1.0e-12);
+    const T pi = T(3.141592653589793238462643383279502884);
+    const T inv_pi = T(0.318309886183790671537767526745028724);
+    T x = phi * inv_w;
+    if (x > T(1) - eps) {
+        H = T(1); D = T(0);
+    } else if (x < T(-1) + eps) {
+        H = T(0); D = T(0);
+    } else {
+        H = T(0.5) * ((T(1) + x) + sin(pi * x) * inv_pi);
+        D = T(0.5) * (T(1) + cos(pi * x)) * inv_w;
+    }
+    H = fmax(fmin(H, T(1)), T(0));
+}
+
 // Fourth-order central slope times 12 dx (level_set_rk3_redis.f90:290-293).
 template <typename T> LSMP_DEV T central4(T m2, T m1, T c, T p1, T p2)
 {

@@ -308,6 +308,7 @@ RedisArgs redis_args(const lsmp_ctx *c)
     ra.invdy = 1.0 / c->p.dy;
     ra.invdz = 1.0 / c->p.dz;
     ra.ls_wid = c->p.ls_wid;
+    ra.inv_ls_wid = 1.0 / c->p.ls_wid;
     ra.rdt = c->p.rdt;
     return ra;
 }

@@ -20,7 +20,7 @@ static_assert(REX * REY <= RNT, "reinit tile needs one thread per ring cell");
 
 struct RedisArgs {
     int nx, ny, nz;
-    double inv12dx, inv12dy, inv12dz, invdx, invdy, invdz, ls_wid, rdt;
+    double inv12dx, inv12dy, inv12dz, invdx, invdy, invdz, ls_wid, inv_ls_wid, rdt;
 };
 
 struct CcdArgs {

@@ -336,27 +336,43 @@ __global__ __launch_bounds__(64) void k_adv_x(const T *__restrict__ phi, const T
 // gradient :266-369, lambda :371-454 and ls_funs ns_solver/others.f90:1-45).
 //
 // A workgroup owns an interior tile RTX x RTY in (i, j) and marches over RKC
-// planes in k.  phi planes (tile + 4 halo) sit in an LDS ring of 7; grad,
+// planes in k.  Each thread of the tile + 1 ring keeps its own column
+// phi(k-3..k+3) in registers (the z stencil); the current plane, with a
+// 4-cell halo for the x and y stencils, is published to LDS.  grad,
 // s2 = delta S0 (|grad| - 1) and s3 = |grad| delta^2 are formed on the tile +
-// 1 ring so that the 3x3x3 lambda quadrature (centre weight 51) reads LDS.
+// 1 ring and kept for three planes in LDS, so the 3x3x3 lambda quadrature
+// (centre weight 51) reads LDS only.  The next plane's global loads are issued
+// before the current plane is computed, hiding their latency.
 // The RK stage is written in Shu-Osher form, algebraically identical to the
 // reference's increments: phi1 = phi0 - dtau l(phi0);
 // phi2 = 3/4 phi0 + 1/4 (phi1 - dtau l(phi1)); phi3 = 1/3 phi0 + 2/3 (phi2 - dtau l(phi2)).
 // Stage 3 also reduces max |phi3 - phi0| (:46-62).
 // ---------------------------------------------------------------------------
+constexpr int RRING = RPX * RPY - REX * REY;   // halo-only cells of the LDS plane
+static_assert(RRING <= RNT, "one thread per halo cell");
+
+// LDS position (row, col) of halo cell number t: three rows on top, three at
+// the bottom, three columns left and right of the tile + 1 ring.
+LSMP_DEV void ring_pos(int t, int &r, int &c)
+{
+    if (t < 3 * RPX) { r = t / RPX; c = t % RPX; }
+    else if (t < 6 * RPX) { r = RPY - 3 + (t - 3 * RPX) / RPX; c = (t - 3 * RPX) % RPX; }
+    else if (t < 6 * RPX + 3 * REY) { const int u = t - 6 * RPX; r = 3 + u / 3; c = u % 3; }
+    else { const int u = t - 6 * RPX - 3 * REY; r = 3 + u / 3; c = RPX - 3 + u % 3; }
+}
 
 template <typename T, int STAGE, int BTN>
 __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi, T *__restrict__ phi_out,
-                                                       const T *__restrict__ phi0, const T *__restrict__ S0,
-                                                       RedisArgs ra, unsigned long long *errmax)
+                                                          const T *__restrict__ phi0, const T *__restrict__ S0,
+                                                          RedisArgs ra, unsigned long long *errmax)
 {
-    __shared__ T sphi[7][RPY][RPX];
+    __shared__ T sphi[RPY][RPX];
     __shared__ T ss2[3][REY][REX];
     __shared__ T ss3[3][REY][REX];
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy, inv12dz = (T)ra.inv12dz;
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
-    const T ls_wid = (T)ra.ls_wid, rdt = (T)ra.rdt;
+    const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
     const int t = threadIdx.x;
     const int i0 = blockIdx.x * RTX, j0 = blockIdx.y * RTY, k0 = blockIdx.z * RKC;
     const int k1 = min(k0 + RKC, nz);
@@ -365,42 +381,48 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
     const int gi = clampi(i0 - 1 + ex, 0, nx - 1), gj = clampi(j0 - 1 + ey, 0, ny - 1);
     const bool inner = ext && ex >= 1 && ex <= RTX && ey >= 1 && ey <= RTY && (i0 - 1 + ex) < nx &&
                        (j0 - 1 + ey) < ny;
+    // stencil centre in LDS: the clamped cell, so out-of-domain ring cells
+    // take the values of the boundary cell (zero-gradient ghosts)
     const int pc = gi - (i0 - 4), pr = gj - (j0 - 4);
+    const bool ring = t < RRING;
+    int rr = 0, rc = 0;
+    if (ring) ring_pos(t, rr, rc);
+    const size_t plane = (size_t)nx * ny;
+    const size_t col = (size_t)gj * nx + gi;
+    const size_t rcol = (size_t)clampi(j0 - 4 + rr, 0, ny - 1) * nx + clampi(i0 - 4 + rc, 0, nx - 1);
+    auto P = [&](int p) -> size_t { return (size_t)clampi(p, 0, nz - 1) * plane; };
 
-    auto load_plane = [&](int p) {
-        const int slot = (p + 7) % 7;
-        const int pcl = clampi(p, 0, nz - 1);
-        for (int idx = t; idx < RPX * RPY; idx += RNT) {
-            int c = idx % RPX, r = idx / RPX;
-            int gx = clampi(i0 - 4 + c, 0, nx - 1), gy = clampi(j0 - 4 + r, 0, ny - 1);
-            sphi[slot][r][c] = phi[((size_t)pcl * ny + gy) * nx + gx];
-        }
-    };
-
-    for (int p = k0 - 4; p <= k0 + 1; ++p) load_plane(p);
+    T w[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d) w[d] = ext ? phi[P(k0 - 4 + d) + col] : T(0);
+    T ring_cur = ring ? phi[P(k0 - 1) + rcol] : T(0);
+    T s0_cur = ext ? S0[P(k0 - 1) + col] : T(0);
+    T p0_cur = T(0);
     T grad_q = T(0), del_q = T(0), s0_q = T(0), phi_q = T(0);
     double lerr = 0.0;
     for (int p = k0 - 1; p <= k1; ++p) {
-        load_plane(p + 3);
+        // next iteration's inputs, in flight while this plane is computed
+        const T w_next = ext ? phi[P(p + 4) + col] : T(0);
+        const T ring_next = ring ? phi[P(p + 1) + rcol] : T(0);
+        const T s0_next = ext ? S0[P(p + 1) + col] : T(0);
+        const T p0_next = (STAGE > 1 && inner && p >= k0 && p < k1) ? phi0[(size_t)p * plane + col] : T(0);
+
+        if (ext) sphi[ey + 3][ex + 3] = w[3];
+        if (ring) sphi[rr][rc] = ring_cur;
         __syncthreads();
-        T grad_p = T(0), del_p = T(0), s0_p = T(0), phi_p = T(0);
+        T grad_p = T(0), del_p = T(0);
+        const T s0_p = s0_cur, phi_p = w[3];
         if (ext && p >= 0 && p < nz) {
-            const int sp = (p + 7) % 7;
-            const T c0 = sphi[sp][pr][pc];
             T xs1, xs2, ys1, ys2, zs1, zs2;
-            weno_hj_pm(sphi[sp][pr][pc - 3], sphi[sp][pr][pc - 2], sphi[sp][pr][pc - 1], c0,
-                       sphi[sp][pr][pc + 1], sphi[sp][pr][pc + 2], sphi[sp][pr][pc + 3], inv12dx, invdx, xs1, xs2);
-            weno_hj_pm(sphi[sp][pr - 3][pc], sphi[sp][pr - 2][pc], sphi[sp][pr - 1][pc], c0,
-                       sphi[sp][pr + 1][pc], sphi[sp][pr + 2][pc], sphi[sp][pr + 3][pc], inv12dy, invdy, ys1, ys2);
-            weno_hj_pm(sphi[(p + 4) % 7][pr][pc], sphi[(p + 5) % 7][pr][pc], sphi[(p + 6) % 7][pr][pc], c0,
-                       sphi[(p + 8) % 7][pr][pc], sphi[(p + 9) % 7][pr][pc], sphi[(p + 10) % 7][pr][pc], inv12dz,
-                       invdz, zs1, zs2);
-            s0_p = S0[((size_t)p * ny + gj) * nx + gi];
+            weno_hj_pm_fast(sphi[pr][pc - 3], sphi[pr][pc - 2], sphi[pr][pc - 1], w[3], sphi[pr][pc + 1],
+                            sphi[pr][pc + 2], sphi[pr][pc + 3], inv12dx, invdx, xs1, xs2);
+            weno_hj_pm_fast(sphi[pr - 3][pc], sphi[pr - 2][pc], sphi[pr - 1][pc], w[3], sphi[pr + 1][pc],
+                            sphi[pr + 2][pc], sphi[pr + 3][pc], inv12dy, invdy, ys1, ys2);
+            weno_hj_pm_fast(w[0], w[1], w[2], w[3], w[4], w[5], w[6], inv12dz, invdz, zs1, zs2);
             grad_p = godunov<T, 3>(xs1, xs2, ys1, ys2, zs1, zs2, s0_p);
-            phi_p = c0;
             if (BTN == 1) {
                 T H;
-                ls_funs(c0, ls_wid, H, del_p);
+                ls_funs_fast(w[3], inv_w, H, del_p);
                 ss2[p % 3][ey][ex] = del_p * s0_p * (grad_p - T(1));
                 ss3[p % 3][ey][ex] = grad_p * del_p * del_p;
             }
@@ -426,19 +448,24 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
                 if (fabs(b) > T(1.0e-12)) lam = a / b * del_q;
             }
             const T l = (s0_q - lam) * grad_q - s0_q;
-            const size_t gidx = ((size_t)q * ny + gj) * nx + gi;
+            const size_t gidx = (size_t)q * plane + col;
             T pn;
             if (STAGE == 1) {
                 pn = phi_q - rdt * l;
             } else if (STAGE == 2) {
-                pn = T(0.75) * phi0[gidx] + T(0.25) * (phi_q - rdt * l);
+                pn = T(0.75) * p0_cur + T(0.25) * (phi_q - rdt * l);
             } else {
-                const T p0 = phi0[gidx];
-                pn = p0 / T(3) + T(2) * (phi_q - rdt * l) / T(3);
-                lerr = fmax(lerr, (double)fabs(pn - p0));
+                pn = p0_cur * T(1.0 / 3.0) + T(2.0 / 3.0) * (phi_q - rdt * l);
+                lerr = fmax(lerr, (double)fabs(pn - p0_cur));
             }
             phi_out[gidx] = pn;
         }
+#pragma unroll
+        for (int d = 0; d < 6; ++d) w[d] = w[d + 1];
+        w[6] = w_next;
+        ring_cur = ring_next;
+        s0_cur = s0_next;
+        p0_cur = p0_next;
         grad_q = grad_p; del_q = del_p; s0_q = s0_p; phi_q = phi_p;
     }
     if (STAGE == 3) {
@@ -460,7 +487,7 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
     const int nx = ra.nx, ny = ra.ny;
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy;
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy;
-    const T ls_wid = (T)ra.ls_wid, rdt = (T)ra.rdt;
+    const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
     const int t = threadIdx.x;
     const int i0 = blockIdx.x * RTX, j0 = blockIdx.y * RTY;
     const bool ext = t < REX * REY;
@@ -479,16 +506,16 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
     if (ext) {
         const T c0 = sphi[pr][pc];
         T xs1, xs2, ys1, ys2;
-        weno_hj_pm(sphi[pr][pc - 3], sphi[pr][pc - 2], sphi[pr][pc - 1], c0, sphi[pr][pc + 1], sphi[pr][pc + 2],
-                   sphi[pr][pc + 3], inv12dx, invdx, xs1, xs2);
-        weno_hj_pm(sphi[pr - 3][pc], sphi[pr - 2][pc], sphi[pr - 1][pc], c0, sphi[pr + 1][pc], sphi[pr + 2][pc],
-                   sphi[pr + 3][pc], inv12dy, invdy, ys1, ys2);
+        weno_hj_pm_fast(sphi[pr][pc - 3], sphi[pr][pc - 2], sphi[pr][pc - 1], c0, sphi[pr][pc + 1],
+                        sphi[pr][pc + 2], sphi[pr][pc + 3], inv12dx, invdx, xs1, xs2);
+        weno_hj_pm_fast(sphi[pr - 3][pc], sphi[pr - 2][pc], sphi[pr - 1][pc], c0, sphi[pr + 1][pc],
+                        sphi[pr + 2][pc], sphi[pr + 3][pc], inv12dy, invdy, ys1, ys2);
         s0_p = S0[(size_t)gj * nx + gi];
         grad_p = godunov<T, 2>(xs1, xs2, ys1, ys2, T(0), T(0), s0_p);
         phi_p = c0;
         if (BTN == 1) {
             T H;
-            ls_funs(c0, ls_wid, H, del_p);
+            ls_funs_fast(c0, inv_w, H, del_p);
             ss2[ey][ex] = del_p * s0_p * (grad_p - T(1));
             ss3[ey][ex] = grad_p * del_p * del_p;
         }
@@ -518,7 +545,7 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
             pn = T(0.75) * phi0[gidx] + T(0.25) * (phi_p - rdt * l);
         } else {
             const T p0 = phi0[gidx];
-            pn = p0 / T(3) + T(2) * (phi_p - rdt * l) / T(3);
+            pn = p0 * T(1.0 / 3.0) + T(2.0 / 3.0) * (phi_p - rdt * l);
             lerr = fmax(lerr, (double)fabs(pn - p0));
         }
         phi_out[gidx] = pn;
