@@ -1,0 +1,344 @@
+! lsmp_driver.f90 -- Fortran driver of the gfx950 level-set transport library.
+!
+! Plays the role of the reference's program main (Vortex/src/main.f90:10-53)
+! with problem_init (src/setup/problem_init.f90), the deck reader
+! (lib/data/tree.f90:90-184, 230-285, 352) and the per-step output
+! (src/others/output.f90:1-36, 95-143, 169-231), for the three transport
+! cases of the reference: Disk (2D Zalesak disk, rigid rotation), Vortex2D
+! (2D single vortex) and Vortex (3D LeVeque deformation).  The hot path --
+! advection, reinitialisation, mass correction, mass/volume sums and the L1
+! metric -- runs on the GPU through lsmp_mod's ISO_C_BINDING interface.
+!
+! usage:  lsmp_driver <disk|vortex2d|vortex> [deck=./input/default.txt] [max_steps]
+!
+! The deck has the reference's format (odd lines are comments).  Like the
+! reference's shipped cases it needs threads grid 1 1 1 and ghc = 3; method 1
+! (LS) and 2 (MPLS) are supported, method 3 (CLSVOF) is not.
+program lsmp_driver
+use iso_c_binding
+use lsmp
+implicit none
+
+character(len=256) :: casename, deck, arg
+character(len=30) :: name
+integer :: method, gx, gy, gz, nthreads, nplots, level, ug, ghc, how_to_paras, inverse
+integer :: t2red, iter, max_steps, dim, field, iostat, ubc(2), vbc(2), wbc(2), per(3)
+real(8) :: xstart, xend, ystart, yend, zstart, zend, t2s, t2p, dt, rdt, ls_wid
+real(8) :: p_tol, p_w1, p_w2, p_b, t_tol, t_w, rho_1, mu_1, rho_2, mu_2, sigma, g, ggx, ggy, ggz
+real(8) :: cl, cu, ct, re, fr, we, rho_12, mu_12
+integer :: btn_sf, btn_g
+integer :: nx, ny, nz, kb0, kb1
+real(8) :: dx, dy, dz, time, mass, vol, imass, ivol, red_error, l1
+real(8) :: loss_mass_avg, loss_mass_max, loss_vol_avg, loss_vol_max
+real(8), allocatable :: phi(:,:,:), xc(:), yc(:), zc(:)
+type(lsmp_params) :: prm
+type(lsmp_stats) :: st
+type(c_ptr) :: h
+integer(c_int) :: npseudo, nmass
+integer :: fmass, fvol, fen, fmom, fpos
+
+! ------------------------------------------------------------ arguments
+call get_command_argument(1, casename)
+if (len_trim(casename) == 0) then
+    write(*,*) "usage: lsmp_driver <disk|vortex2d|vortex> [deck] [max_steps]"
+    stop 1
+endif
+deck = "./input/default.txt"
+call get_command_argument(2, arg)
+if (len_trim(arg) > 0) deck = arg
+max_steps = huge(1)
+call get_command_argument(3, arg)
+if (len_trim(arg) > 0) read(arg, *) max_steps
+
+select case (trim(casename))
+case ("disk")
+    dim = 2; field = LSMP_VEL_ROTATION
+case ("vortex2d")
+    dim = 2; field = LSMP_VEL_VORTEX2D
+case ("vortex")
+    dim = 3; field = LSMP_VEL_LEVEQUE3D
+case default
+    write(*,*) "unknown case ", trim(casename), " (disk | vortex2d | vortex)"
+    stop 1
+end select
+
+! ------------------------------------------------------------ deck (tree.f90:96-157)
+open(unit=526, file=trim(deck), status='old')
+read(526,*); read(526,*) method
+read(526,*); read(526,*) name
+read(526,*); read(526,*) gx, gy, gz
+read(526,*); read(526,*) nthreads
+read(526,*); read(526,*) nplots
+read(526,*); read(526,*) level
+read(526,*); read(526,*) ug
+read(526,*); read(526,*) ghc
+read(526,*); read(526,*) xstart, xend
+read(526,*); read(526,*) ystart, yend
+read(526,*); read(526,*) zstart, zend
+read(526,*); read(526,*) t2s, t2p, t2red
+read(526,*); read(526,*) dt, rdt
+read(526,*); read(526,*) p_tol, p_w1, p_w2, p_b
+read(526,*); read(526,*) t_tol, t_w
+read(526,*); read(526,*) ls_wid
+read(526,*); read(526,*) how_to_paras
+read(526,*); read(526,*) rho_1, mu_1
+read(526,*); read(526,*) rho_2, mu_2
+read(526,*); read(526,*) sigma, btn_sf
+read(526,*); read(526,*) g, btn_g
+read(526,*); read(526,*) ggx, ggy, ggz
+read(526,*); read(526,*) cl, cu, ct
+read(526,*); read(526,*) re, fr, we
+read(526,*); read(526,*) rho_12, mu_12
+read(526,*); read(526,*) inverse
+read(526,*); read(526,*) ubc(1), ubc(2)
+read(526,*); read(526,*) vbc(1), vbc(2)
+read(526,*); read(526,*) wbc(1), wbc(2)
+read(526,*); read(526,*) per(1), per(2), per(3)
+close(526)
+
+if (method /= 1 .and. method /= 2) then
+    write(*,*) "Error >> method ", method, " is not provided by this driver (1 LS, 2 MPLS)"
+    stop 1
+endif
+if (gx /= 1 .or. gy /= 1 .or. gz /= 1) then
+    ! the reference truncates line solves at subdomain edges (branches.f90:97-136);
+    ! the device path solves whole lines
+    write(*,*) "Error >> threads grid must be 1 1 1"
+    stop 1
+endif
+if (any(per /= 0)) then
+    write(*,*) "Error >> periodic directions are not supported"
+    stop 1
+endif
+if (how_to_paras < 1 .or. how_to_paras > 5) then
+    write(*,*) "Error >> Wrong parameter selector "          ! tree.f90:304-306
+    stop 1
+endif
+if (how_to_paras > 1) rho_12 = rho_2 / rho_1                 ! tree.f90:310-312
+
+! ------------------------------------------------------------ grid (tree.f90:230-280)
+nx = int(ug * (xend - xstart))
+ny = int(ug * (yend - ystart))
+if (dim == 3) then
+    nz = int(ug * (zend - zstart))
+    dz = (zend - zstart) / nz
+    kb0 = 1 - ghc; kb1 = nz + ghc
+else
+    nz = 1
+    dz = 1.0d0
+    kb0 = 1 - ghc; kb1 = 1 + ghc
+endif
+dx = (xend - xstart) / nx
+dy = (yend - ystart) / ny
+dt = dt * dx
+rdt = rdt * dx
+ls_wid = ls_wid * dx
+if (t2red < 0) t2red = int(t2s / dt) + 1                      ! tree.f90:352
+
+allocate(xc(0:nx+1), yc(0:ny+1), zc(0:nz+1))
+call coords(xc, nx, xstart, xend, dx)
+call coords(yc, ny, ystart, yend, dy)
+call coords(zc, nz, zstart, zend, dz)
+
+write(*,*) " --- Problem information --- "
+write(*,'(A30)') name
+write(*,'(A20,F15.8)') "dx:", dx
+write(*,'(A20,F15.8)') "dt:", dt
+write(*,'(A20,I5,A3,I5,A3,I5)') "Grids:", nx, "x", ny, "x", nz
+
+prm%nx = nx; prm%ny = ny; prm%nz = nz; prm%ghc = ghc; prm%dim = dim
+prm%dx = dx; prm%dy = dy; prm%dz = dz
+prm%dt = dt; prm%rdt = rdt; prm%ls_wid = ls_wid
+prm%rho_12 = rho_12; prm%inverse = inverse; prm%precision = LSMP_FP64
+prm%xstart = xstart; prm%xend = xend; prm%ystart = ystart; prm%yend = yend
+prm%zstart = zstart; prm%zend = zend
+call lsmp_check(c_lsmp_init(prm, h), "lsmp_init")
+
+! ------------------------------------------------------------ initial condition (problem_init.f90:32-103)
+allocate(phi(1-ghc:nx+ghc, 1-ghc:ny+ghc, kb0:kb1))
+call initial_phi()
+call lsmp_check(c_lsmp_upload_phi(h, phi), "lsmp_upload_phi")
+if (trim(casename) == "disk") then
+    call lsmp_check(c_lsmp_reinit(h, 0_c_int, -1.0d0, red_error, npseudo), "lsmp_reinit(0)")  ! :103
+    write(*,'("LS Init:",I8,ES20.4)') npseudo, red_error
+endif
+call lsmp_check(c_lsmp_store_phi0(h), "lsmp_store_phi0")     ! phi%tmp2 (roots.f90:150-166)
+call lsmp_check(c_lsmp_mass_vol(h, imass, ivol), "lsmp_mass_vol")   ! problem_init.f90:140-142
+
+! ------------------------------------------------------------ outputs (tree.f90:199-217)
+call execute_command_line("mkdir -p out", exitstat=iostat)
+fmass = 15; fvol = 16; fen = 17; fmom = 18; fpos = 19
+open(unit=fmass, file="./out/"//trim(name)//"_MassLoss.plt")
+write(fmass,*) 'variables = "T" "LS" "VOF" "B1" "B2" '
+open(unit=fvol, file="./out/"//trim(name)//"_VolumeLoss.plt")
+write(fvol,*) 'variables = "T" "LS" "VOF" "B1" "B2" '
+open(unit=fen, file="./out/"//trim(name)//"_Energy.plt")
+write(fen,*) 'variables = "T" "Ek" "Ep" "Es" "Ev" "E" "E0" '
+open(unit=fmom, file="./out/"//trim(name)//"_Momentum.plt")
+write(fmom,*) 'variables = "T" "Px" "Py" "Pz" '
+open(unit=fpos, file="./out/"//trim(name)//"_Position.plt")
+write(fpos,*) 'variables = "T" "x" "y" "z" '
+
+! ------------------------------------------------------------ time loop (main.f90:17-51)
+time = 0.0d0
+iter = 0
+loss_mass_avg = 0.0d0; loss_mass_max = 0.0d0
+loss_vol_avg = 0.0d0; loss_vol_max = 0.0d0
+do
+    if (time + dt > t2s) dt = t2s - time + 1.0d-10
+    time = time + dt
+    iter = iter + 1
+    call lsmp_check(c_lsmp_set_velocity_analytic(h, int(field, c_int), time, t2s), "velocity")
+
+    ! interface_solver -> classical_ls / mpls merged branch (interface_solver.f90:15-23, 123-129)
+    call lsmp_check(c_lsmp_rk3_advect(h, dt), "lsmp_rk3_advect")
+    if (mod(iter, t2red) == 0) then
+        call lsmp_check(c_lsmp_reinit(h, 1_c_int, -1.0d0, red_error, npseudo), "lsmp_reinit")
+    endif
+    if (method == 2) then
+        call lsmp_check(c_lsmp_mass_fix(h, imass, mass, nmass), "lsmp_mass_fix")
+    endif
+    call lsmp_check(c_lsmp_mass_vol(h, mass, vol), "lsmp_mass_vol")      ! main.f90:35
+
+    ! output (output.f90:7-31); the VOF, marker and NS columns of the
+    ! reference are not transported on this path and are written as zero
+    if (iter == 1) ivol = vol
+    write(fmass,'(7ES15.4)') time, 100.0d0*(mass-imass)/imass, 0.0d0, 0.0d0, 0.0d0
+    write(fvol,'(7ES15.4)') time, 100.0d0*(vol-ivol)/ivol, 0.0d0, 0.0d0, 0.0d0
+    loss_mass_avg = loss_mass_avg + abs(imass-mass)/imass
+    loss_vol_avg = loss_vol_avg + abs(ivol-vol)/ivol
+    loss_mass_max = max(loss_mass_max, abs(imass-mass)/imass)
+    loss_vol_max = max(loss_vol_max, abs(ivol-vol)/ivol)
+
+    if (mod(iter, 5) == 0) then
+        write(*,*) "========================================="
+        write(*,'(A,",",I8,",",F15.6)') trim(name), iter, time
+        write(*,*) ''
+        write(*,'("LS,  Loss of mass  (%) :",ES15.4)') 100.0d0*(mass-imass)/imass
+        write(*,'("LS,  Loss of volume(%) :",ES15.4)') 100.0d0*(ivol-vol)/ivol
+        write(*,'("LS,  redistance error  :",ES15.4)') red_error
+        write(*,*) ''
+        call lsmp_check(c_lsmp_get_stats(h, st), "lsmp_get_stats")
+        call print_device_time(st)
+        write(*,*) "========================================="
+    endif
+
+    if (time + 1.0d-10 > t2s) exit
+    if (iter >= max_steps) exit
+enddo
+
+! ------------------------------------------------------------ calculate_interface_loss (output.f90:169-231)
+call lsmp_check(c_lsmp_interface_l1(h, l1), "lsmp_interface_l1")
+open(unit=456, file="./Vortex_Results.txt", action='write', position='append')
+if (method == 1) then
+    write(456, *) "Method: LS"
+else
+    write(456, *) "Method: MPLS"
+endif
+write(456,*) "Density Ratio:", rho_12
+write(456,*) "Nx :", nx
+write(456,'(A10, 2ES15.4)') "LS Mass:", loss_mass_avg / iter, loss_mass_max
+write(456,'(A10, 2ES15.4)') "LS Vol:", loss_vol_avg / iter, loss_vol_max
+write(456,'(A10, 2ES15.4)') "LS Int:", l1
+close(456)
+write(*,'(A,I8,A,ES24.16,A,ES24.16)') " finished: steps", iter, "  L1", l1, "  max |dM|/M", loss_mass_max
+
+close(fmass); close(fvol); close(fen); close(fmom); close(fpos)
+call c_lsmp_finalize(h)
+
+contains
+
+! cell centres x(i) = start + (i - 0.5) dx with (i - 0.5) in default real,
+! x(0) = start, x(n+1) = end (tree.f90:254-274)
+subroutine coords(x, n, s, e, d)
+    integer, intent(in) :: n
+    real(8), intent(out) :: x(0:n+1)
+    real(8), intent(in) :: s, e, d
+    integer :: i
+    do i = 1, n
+        x(i) = s + (i - 0.5) * d
+    enddo
+    x(0) = s
+    x(n+1) = e
+end subroutine
+
+! problem_init.f90:32-79 for the three cases, then the zero-gradient ghosts
+subroutine initial_phi()
+    integer :: i, j, k, ii, jj, kk, sub
+    real(8) :: x, y, z, vof
+    sub = 30
+    do k = 1, nz
+    do j = 1, ny
+    do i = 1, nx
+        select case (trim(casename))
+        case ("disk")
+            vof = 0.0d0
+            do ii = 1, sub
+            do jj = 1, sub
+            do kk = 1, sub
+                x = 0.5d0*( xc(i)+xc(i-1) ) + real(ii,8)*dx/real(sub,8)
+                y = 0.5d0*( yc(j)+yc(j-1) ) + real(jj,8)*dy/real(sub,8)
+                if (sqrt((x-0.5)**2+(y-0.75)**2) <= 0.15) then
+                    vof = vof + 1.0d0/real(sub,8)**3.0d0
+                end if
+                if (abs(x-0.5) <= 0.025 .and. y < 0.8 .and. (x-0.5)**2+(y-0.75)**2 <= 0.15**2) then
+                    vof = vof - 1.0d0/real(sub,8)**3.0d0
+                end if
+            end do
+            end do
+            end do
+            phi(i,j,k) = vof * 2.0 - 1.0
+        case ("vortex2d")
+            phi(i,j,k) = -sqrt((xc(i)-0.5)**2+(yc(j)-0.75)**2) + 0.15
+        case default
+            phi(i,j,k) = - sqrt((xc(i)-0.35)**2+(yc(j)-0.35)**2+(zc(k)-0.35)**2) + 0.15
+        end select
+    enddo
+    enddo
+    enddo
+    call bc_box(phi)
+end subroutine
+
+! zero-gradient ghost fill, x then y then z (branches.f90:283-310)
+subroutine bc_box(dat)
+    real(8), intent(inout) :: dat(1-ghc:nx+ghc, 1-ghc:ny+ghc, kb0:kb1)
+    integer :: i, j, k
+    do k = kb0, kb1
+    do j = 1-ghc, ny+ghc
+    do i = 1, ghc
+        dat(1-i,j,k) = dat(1,j,k)
+        dat(nx+i,j,k) = dat(nx,j,k)
+    enddo
+    enddo
+    enddo
+    do k = kb0, kb1
+    do i = 1-ghc, nx+ghc
+    do j = 1, ghc
+        dat(i,1-j,k) = dat(i,1,k)
+        dat(i,ny+j,k) = dat(i,ny,k)
+    enddo
+    enddo
+    enddo
+    do j = 1-ghc, ny+ghc
+    do i = 1-ghc, nx+ghc
+    do k = 1, ghc
+        dat(i,j,1-k) = dat(i,j,1)
+        dat(i,j,nz+k) = dat(i,j,nz)
+    enddo
+    enddo
+    enddo
+end subroutine
+
+! print_CPU_info (output.f90:129-143) with device time per phase
+subroutine print_device_time(s)
+    type(lsmp_stats), intent(in) :: s
+    real(8) :: total
+    total = s%t_advect_ms + s%t_reinit_ms + s%t_massfix_ms
+    if (total <= 0.0d0) return
+    write(*,'("Total device time(s) :",F15.6)') total * 1.0d-3
+    write(*,'(3A18)') "Inter. Adv.", "Inter. Recon.", "Mass corr."
+    write(*,'(F17.2,"%",F17.2,"%",F17.2,"%")') 100.0d0*s%t_advect_ms/total, &
+        & 100.0d0*s%t_reinit_ms/total, 100.0d0*s%t_massfix_ms/total
+end subroutine
+
+end program lsmp_driver

@@ -1,0 +1,151 @@
+! lsmp_mod.f90 -- ISO_C_BINDING interface of the gfx950 level-set transport
+! library (include/lsmp.h) for Fortran drivers.
+!
+! The reference reaches its hot path through argument-less internal
+! subroutines (interface_solver.f90:1-13, level_set_rk3_solver.f90:1-94,
+! level_set_rk3_redis.f90:1-79, mass_preserving_level_set.f90:1-74,
+! ns_solver/others.f90:172-247).  A driver replaces those calls with the
+! wrappers at the end of this module; each stops the program with the
+! library's message on a non-zero return code, as the reference stops on its
+! own error conditions.
+module lsmp
+use iso_c_binding
+implicit none
+private
+
+integer(c_int), parameter, public :: LSMP_FP64 = 0, LSMP_FP32 = 1
+integer(c_int), parameter, public :: LSMP_VEL_ROTATION = 0, LSMP_VEL_VORTEX2D = 1, LSMP_VEL_LEVEQUE3D = 2
+
+type, bind(C), public :: lsmp_params
+    integer(c_int) :: nx, ny, nz, ghc, dim
+    real(c_double) :: dx, dy, dz
+    real(c_double) :: dt, rdt, ls_wid
+    real(c_double) :: rho_12
+    integer(c_int) :: inverse, precision
+    real(c_double) :: xstart, xend, ystart, yend, zstart, zend
+end type lsmp_params
+
+type, bind(C), public :: lsmp_stats
+    integer(c_int) :: last_redis_iters, last_mass_iters
+    real(c_double) :: last_red_error
+    real(c_double) :: t_advect_ms, t_reinit_ms, t_massfix_ms
+    integer(c_long_long) :: n_advect, n_reinit, n_massfix
+end type lsmp_stats
+
+interface
+    integer(c_int) function c_lsmp_init(p, h) bind(C, name="lsmp_init")
+        import :: c_int, c_ptr, lsmp_params
+        type(lsmp_params), intent(in) :: p
+        type(c_ptr), intent(out) :: h
+    end function
+    subroutine c_lsmp_finalize(h) bind(C, name="lsmp_finalize")
+        import :: c_ptr
+        type(c_ptr), value :: h
+    end subroutine
+    type(c_ptr) function c_lsmp_error_string(code) bind(C, name="lsmp_error_string")
+        import :: c_int, c_ptr
+        integer(c_int), value :: code
+    end function
+    integer(c_int) function c_lsmp_upload_phi(h, box) bind(C, name="lsmp_upload_phi")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(in) :: box(*)
+    end function
+    integer(c_int) function c_lsmp_download_phi(h, box) bind(C, name="lsmp_download_phi")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(out) :: box(*)
+    end function
+    integer(c_int) function c_lsmp_store_phi0(h) bind(C, name="lsmp_store_phi0")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+    end function
+    integer(c_int) function c_lsmp_set_velocity(h, u, v, w) bind(C, name="lsmp_set_velocity")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(in) :: u(*), v(*), w(*)
+    end function
+    integer(c_int) function c_lsmp_set_velocity_analytic(h, field, t, t2s) bind(C, name="lsmp_set_velocity_analytic")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int), value :: field
+        real(c_double), value :: t, t2s
+    end function
+    integer(c_int) function c_lsmp_rk3_advect(h, dt) bind(C, name="lsmp_rk3_advect")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), value :: dt
+    end function
+    integer(c_int) function c_lsmp_reinit(h, btn, timestop, red_error, n_pseudo) bind(C, name="lsmp_reinit")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int), value :: btn
+        real(c_double), value :: timestop
+        real(c_double), intent(out) :: red_error
+        integer(c_int), intent(out) :: n_pseudo
+    end function
+    integer(c_int) function c_lsmp_mass_vol(h, mass, vol) bind(C, name="lsmp_mass_vol")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(out) :: mass, vol
+    end function
+    integer(c_int) function c_lsmp_mass_fix(h, imass, mass, n_iter) bind(C, name="lsmp_mass_fix")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), value :: imass
+        real(c_double), intent(out) :: mass
+        integer(c_int), intent(out) :: n_iter
+    end function
+    integer(c_int) function c_lsmp_interface_l1(h, l1) bind(C, name="lsmp_interface_l1")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(out) :: l1
+    end function
+    integer(c_int) function c_lsmp_get_stats(h, s) bind(C, name="lsmp_get_stats")
+        import :: c_int, c_ptr, lsmp_stats
+        type(c_ptr), value :: h
+        type(lsmp_stats), intent(out) :: s
+    end function
+    integer(c_int) function c_lsmp_synchronize(h) bind(C, name="lsmp_synchronize")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+    end function
+    integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+        import :: c_size_t, c_ptr
+        type(c_ptr), value :: s
+    end function
+end interface
+
+public :: c_lsmp_init, c_lsmp_finalize, c_lsmp_upload_phi, c_lsmp_download_phi, c_lsmp_store_phi0
+public :: c_lsmp_set_velocity, c_lsmp_set_velocity_analytic, c_lsmp_rk3_advect, c_lsmp_reinit
+public :: c_lsmp_mass_vol, c_lsmp_mass_fix, c_lsmp_interface_l1, c_lsmp_get_stats, c_lsmp_synchronize
+public :: lsmp_check, lsmp_error_message
+
+contains
+
+function lsmp_error_message(code) result(msg)
+    integer(c_int), intent(in) :: code
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    p = c_lsmp_error_string(code)
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+        msg(i:i) = chars(i)
+    end do
+end function
+
+! Stop with the library's message on failure (the reference's `stop`).
+subroutine lsmp_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    if (rc /= 0) then
+        write(*,'(A,A,A,I0,A,A)') " lsmp error in ", where, ": code ", rc, ", ", lsmp_error_message(rc)
+        stop 1
+    endif
+end subroutine
+
+end module lsmp
