@@ -123,6 +123,22 @@ template <typename T> LSMP_DEV T phyn(T a, T b, T c, T d)
 // alp2 = 3/q2 normalise to w0 = q1 q2 / D and w2 = 3 q0 q1 / D with
 // D = q1 q2 + 6 q0 q2 + 3 q0 q1.  Equal to phyn() up to rounding of the weights
 // (q_k lies in [1e-12, 3e6], so D neither under- nor overflows in FP32).
+// Reciprocal of a positive normal number: the hardware estimate refined by two
+// Newton steps (within an ulp or two of 1/x; the weights it feeds are ratios).
+LSMP_DEV double rcp_pos(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+LSMP_DEV float rcp_pos(float x)
+{
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+
 template <typename T> LSMP_DEV T phyn_fast(T a, T b, T c, T d)
 {
     const T eps = T(1.0e-6);
@@ -131,9 +147,11 @@ template <typename T> LSMP_DEV T phyn_fast(T a, T b, T c, T d)
     T is2 = T(13) * sq(c - d) + T(3) * sq(T(3) * c - d);
     T q0 = sq(eps + is0), q1 = sq(eps + is1), q2 = sq(eps + is2);
     T q12 = q1 * q2, q01 = q0 * q1;
-    T r = T(1) / (q12 + T(6) * (q0 * q2) + T(3) * q01);
-    T w0 = q12 * r, w2 = T(3) * q01 * r;
-    return w0 * T(1.0 / 3.0) * (a - T(2) * b + c) + (w2 - T(0.5)) * T(1.0 / 6.0) * (b - T(2) * c + d);
+    T r = rcp_pos(q12 + T(6) * (q0 * q2) + T(3) * q01);
+    // w0/3 = q12 r/3 and (w2 - 1/2)/6 = q01 r/2 - 1/12
+    T t0 = q12 * (r * T(1.0 / 3.0));
+    T t2 = fma(q01, r * T(0.5), T(-1.0 / 12.0));
+    return fma(t0, a - T(2) * b + c, t2 * (b - T(2) * c + d));
 }
 
 // D+ and D- from the five second differences s_m = phi(m+1) - 2 phi(m) + phi(m-1),

@@ -363,12 +363,12 @@ LSMP_DEV void ring_pos(int t, int &r, int &c)
 
 template <typename T, int STAGE, int BTN>
 __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi, T *__restrict__ phi_out,
-                                                          const T *__restrict__ phi0, const T *__restrict__ S0,
-                                                          RedisArgs ra, unsigned long long *errmax)
+                                                       const T *__restrict__ phi0, const T *__restrict__ S0,
+                                                       RedisArgs ra, unsigned long long *errmax)
 {
     __shared__ T sphi[RPY][RPX];
-    __shared__ T ss2[3][REY][REX];
-    __shared__ T ss3[3][REY][REX];
+    __shared__ T ss2[REY][REX];
+    __shared__ T ss3[REY][REX];
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy, inv12dz = (T)ra.inv12dz;
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
@@ -398,7 +398,9 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
     T ring_cur = ring ? phi[P(k0 - 1) + rcol] : T(0);
     T s0_cur = ext ? S0[P(k0 - 1) + col] : T(0);
     T p0_cur = T(0);
-    T grad_q = T(0), del_q = T(0), s0_q = T(0), phi_q = T(0);
+    T grad_q = T(0), del_q = T(0), s0_q = T(0), phi_q = T(0), s2_q = T(0), s3_q = T(0);
+    // in-plane 3x3 sums of s2 and s3 around this cell for planes p-2, p-1, p
+    T b2m = T(0), b2c = T(0), b2p = T(0), b3m = T(0), b3c = T(0), b3p = T(0);
     double lerr = 0.0;
     for (int p = k0 - 1; p <= k1; ++p) {
         // next iteration's inputs, in flight while this plane is computed
@@ -410,9 +412,10 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
         if (ext) sphi[ey + 3][ex + 3] = w[3];
         if (ring) sphi[rr][rc] = ring_cur;
         __syncthreads();
-        T grad_p = T(0), del_p = T(0);
+        const bool live = p >= 0 && p < nz;
+        T grad_p = T(0), del_p = T(0), s2_p = T(0), s3_p = T(0);
         const T s0_p = s0_cur, phi_p = w[3];
-        if (ext && p >= 0 && p < nz) {
+        if (ext && live) {
             T xs1, xs2, ys1, ys2, zs1, zs2;
             weno_hj_pm_fast(sphi[pr][pc - 3], sphi[pr][pc - 2], sphi[pr][pc - 1], w[3], sphi[pr][pc + 1],
                             sphi[pr][pc + 2], sphi[pr][pc + 3], inv12dx, invdx, xs1, xs2);
@@ -423,28 +426,38 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
             if (BTN == 1) {
                 T H;
                 ls_funs_fast(w[3], inv_w, H, del_p);
-                ss2[p % 3][ey][ex] = del_p * s0_p * (grad_p - T(1));
-                ss3[p % 3][ey][ex] = grad_p * del_p * del_p;
+                s2_p = del_p * s0_p * (grad_p - T(1));
+                s3_p = grad_p * del_p * del_p;
+                ss2[ey][ex] = s2_p;
+                ss3[ey][ex] = s3_p;
             }
         }
         __syncthreads();
+        if (BTN == 1 && inner && live) {
+            // 3x3 in-plane part of the lambda quadrature; the z direction is
+            // summed from the planes kept in registers
+            T a = T(0), b = T(0);
+#pragma unroll
+            for (int jj = -1; jj <= 1; ++jj)
+#pragma unroll
+                for (int ii = -1; ii <= 1; ++ii) {
+                    a = a + ss2[ey + jj][ex + ii];
+                    b = b + ss3[ey + jj][ex + ii];
+                }
+            b2m = b2c; b2c = b2p; b2p = a;
+            b3m = b3c; b3c = b3p; b3p = b;
+        } else if (p >= nz) {
+            // plane nz is the ghost copy of plane nz-1
+            b2m = b2c; b2c = b2p;
+            b3m = b3c; b3c = b3p;
+        }
         const int q = p - 1;
         if (inner && q >= k0 && q < k1) {
             T lam = T(0);
             if (BTN == 1) {
-                T a = T(51) * ss2[q % 3][ey][ex];
-                T b = T(51) * ss3[q % 3][ey][ex];
-#pragma unroll
-                for (int kk = -1; kk <= 1; ++kk) {
-                    const int sl = clampi(q + kk, 0, nz - 1) % 3;
-#pragma unroll
-                    for (int jj = -1; jj <= 1; ++jj)
-#pragma unroll
-                        for (int ii = -1; ii <= 1; ++ii) {
-                            a = a + ss2[sl][ey + jj][ex + ii];
-                            b = b + ss3[sl][ey + jj][ex + ii];
-                        }
-                }
+                const T lo2 = q == 0 ? b2c : b2m, lo3 = q == 0 ? b3c : b3m;   // plane -1 = plane 0
+                const T a = T(51) * s2_q + ((lo2 + b2c) + b2p);
+                const T b = T(51) * s3_q + ((lo3 + b3c) + b3p);
                 if (fabs(b) > T(1.0e-12)) lam = a / b * del_q;
             }
             const T l = (s0_q - lam) * grad_q - s0_q;
@@ -466,7 +479,7 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
         ring_cur = ring_next;
         s0_cur = s0_next;
         p0_cur = p0_next;
-        grad_q = grad_p; del_q = del_p; s0_q = s0_p; phi_q = phi_p;
+        grad_q = grad_p; del_q = del_p; s0_q = s0_p; phi_q = phi_p; s2_q = s2_p; s3_q = s3_p;
     }
     if (STAGE == 3) {
         double m = block_max(lerr);
