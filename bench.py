@@ -64,6 +64,19 @@ BYTES_V = {
 ADV_Z_V = (5 + 6 + 6) / 3.0            # last pass: R, w, phi (+ l1, l2) in, phi (+ l1/l2) out
 
 
+def measured_traffic(kernel, n, fp64):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    passes (FETCH_SIZE, WRITE_SIZE, calibrated on a copy of known size), or
+    None when no pass covers this kernel, size and precision."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{n}cube_{'fp64' if fp64 else 'fp32'}.json")
+    if not os.path.exists(path):
+        return None
+    data = json.load(open(path))["kernels"]
+    names = {"redis_stage": "k_redis_stage3d"}.get(kernel, kernel)
+    hits = [v["traffic_bytes"] for k, v in data.items() if names in k]
+    return round(sum(hits) / len(hits)) if hits else None
+
+
 def run_steps(s, nsteps, t, imass, field):
     npseudo = nmass = 0
     for _ in range(nsteps):
@@ -187,7 +200,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(dom, n, prec == lsmp.FP64),
         },
         "kernels": kernels,
     }

@@ -103,6 +103,72 @@ LSMP_DEV void crweno_f_row(T f0, T f1, T f2, T f3, T f4, T &A, T &B, T &C, T &S)
     S = (T(5) * w3 + w2) / T(6) * f1 + (w3 + T(5) * (w2 + w1)) / T(6) * f2 + w1 / T(6) * f3;
 }
 
+// The same two rows with the weight divisions folded into one reciprocal:
+// with e_k = EPS + b_k and tau = |b3 - b1| the reference's
+// a_k = c_k (1 + tau / e_k) normalise to
+// w_k = c_k (e_k + tau) prod_{m != k} e_m / sum_j c_j (e_j + tau) prod_{m != j} e_m,
+// and the divisions by 3 and 6 become products.  Equal to crweno_*_row() up to
+// rounding of the weights.  The fold needs the FP64 exponent range: e_k lies in
+// [1e-13, ~2e3], so a product of three can reach 1e-39, below FP32's normal
+// range; the FP32 form keeps one division per weight.
+LSMP_DEV void crweno_weights_fast(double b1, double b2, double b3, double &w1, double &w2, double &w3)
+{
+    const double EPS = 1.0e-13;
+    const double c1 = 0.2089141306, c2 = 0.4999999998, c3 = 0.2910858692;
+    const double tau = fabs(b3 - b1);
+    const double e1 = EPS + b1, e2 = EPS + b2, e3 = EPS + b3;
+    const double a1 = c1 * (e1 + tau) * (e2 * e3);
+    const double a2 = c2 * (e2 + tau) * (e1 * e3);
+    const double a3 = c3 * (e3 + tau) * (e1 * e2);
+    const double r = 1.0 / (a1 + a2 + a3);
+    w1 = a1 * r;
+    w2 = a2 * r;
+    w3 = a3 * r;
+}
+LSMP_DEV void crweno_weights_fast(float b1, float b2, float b3, float &w1, float &w2, float &w3)
+{
+    const float EPS = 1.0e-13f;
+    const float c1 = 0.2089141306f, c2 = 0.4999999998f, c3 = 0.2910858692f;
+    const float tau = fabsf(b3 - b1);
+    const float a1 = c1 * (1.0f + tau / (EPS + b1));
+    const float a2 = c2 * (1.0f + tau / (EPS + b2));
+    const float a3 = c3 * (1.0f + tau / (EPS + b3));
+    const float r = 1.0f / (a1 + a2 + a3);
+    w1 = a1 * r;
+    w2 = a2 * r;
+    w3 = a3 * r;
+}
+
+template <typename T>
+LSMP_DEV void crweno_g_row_fast(T g0, T g1, T g2, T g3, T g4, T &A, T &B, T &C, T &S)
+{
+    T b1 = T(13) * sq(g0 - T(2) * g1 + g2) + T(3) * sq(g0 - T(4) * g1 + T(3) * g2);
+    T b2 = T(13) * sq(g1 - T(2) * g2 + g3) + T(3) * sq(g1 - g3);
+    T b3 = T(13) * sq(g2 - T(2) * g3 + g4) + T(3) * sq(T(3) * g2 - T(4) * g3 + g4);
+    T w1, w2, w3;
+    crweno_weights_fast(b1, b2, b3, w1, w2, w3);
+    const T third = T(1.0 / 3.0), sixth = T(1.0 / 6.0);
+    A = (T(2) * w1 + w2) * third;
+    B = (w1 + T(2) * (w2 + w3)) * third;
+    C = w3 * third;
+    S = (w1 * g1 + (T(5) * (w1 + w2) + w3) * g2 + (w2 + T(5) * w3) * g3) * sixth;
+}
+
+template <typename T>
+LSMP_DEV void crweno_f_row_fast(T f0, T f1, T f2, T f3, T f4, T &A, T &B, T &C, T &S)
+{
+    T b3 = T(13) * sq(f0 - T(2) * f1 + f2) + T(3) * sq(f0 - T(4) * f1 + T(3) * f2);
+    T b2 = T(13) * sq(f1 - T(2) * f2 + f3) + T(3) * sq(f1 - f3);
+    T b1 = T(13) * sq(f2 - T(2) * f3 + f4) + T(3) * sq(T(3) * f2 - T(4) * f3 + f4);
+    T w1, w2, w3;
+    crweno_weights_fast(b1, b2, b3, w1, w2, w3);
+    const T third = T(1.0 / 3.0), sixth = T(1.0 / 6.0);
+    A = w3 * third;
+    B = (w1 + T(2) * (w2 + w3)) * third;
+    C = (w2 + T(2) * w1) * third;
+    S = ((T(5) * w3 + w2) * f1 + (w3 + T(5) * (w2 + w1)) * f2 + w1 * f3) * sixth;
+}
+
 // WENO-HJ correction term (level_set_rk3_redis.f90:456-471).
 template <typename T> LSMP_DEV T phyn(T a, T b, T c, T d)
 {

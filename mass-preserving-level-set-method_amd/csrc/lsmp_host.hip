@@ -169,7 +169,8 @@ int check_params(const lsmp_params *p)
 
 // CCD coefficient tables (ccd_solvers.f90:80-163), factorised once by the
 // 2x2 block elimination of TWIN_DEC (matrix_solver.f90:6-45), for a box line
-// of nb points with spacing dx.  Output: A, B, AA, BB, each (3, nb).
+// of nb points with spacing dx.  Output: A, B, AA, BB, each (3, nb), then the
+// nb reciprocal pivot determinants.
 void ccd_tables(int nb, double dx, std::vector<double> &out)
 {
     out.assign((size_t)12 * nb, 0.0);
@@ -203,6 +204,13 @@ void ccd_tables(int nb, double dx, std::vector<double> &out)
         M(BB, 0, i) = sc2;
         M(BB, 1, i) = M(BB, 1, i) - (sc1 * M(B, 2, i - 1) + sc2 * M(BB, 2, i - 1));
     }
+    // reciprocal of each row's 2x2 pivot determinant, the Den of TWIN_BKS
+    // (matrix_solver.f90:61, 71), which depends on the constant tables only
+    out.resize((size_t)13 * nb);
+    A = out.data(); B = A + 3 * nb; AA = A + 6 * nb; BB = A + 9 * nb;
+    double *iden = A + 12 * nb;
+    for (int i = 0; i < nb; ++i)
+        iden[i] = 1.0 / (M(A, 1, i) * M(BB, 1, i) - M(AA, 1, i) * M(B, 1, i));
 }
 
 inline long box_index(const lsmp_ctx *c, int i, int j, int k)   // 0-based interior coords

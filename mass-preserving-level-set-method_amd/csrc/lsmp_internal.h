@@ -7,7 +7,7 @@
 namespace lsmp {
 
 // x-line kernels stage XCH rows of 64 lines through LDS
-constexpr int XCH = 32;
+constexpr int XCH = 16;
 
 // reinitialisation tile: RTX x RTY interior cells, one ring for the lambda
 // box, four for the WENO-HJ stencil; RKC planes per workgroup in z

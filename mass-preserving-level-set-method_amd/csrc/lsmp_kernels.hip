@@ -90,6 +90,7 @@ LSMP_DEV void crweno_line(const Loader &ldc, const Emit &emit, T *__restrict__ s
                           size_t L, int n, T inv_unused, T dd, bool store)
 {
     (void)inv_unused;
+    const T inv_dd = T(1) / dd;
     T g[6], f[6];
     // boundary faces by WENO-JS
     T gA, fA, gB, fB, gC, fC, gD, fD, gE, fE, gF, fF;
@@ -107,29 +108,23 @@ LSMP_DEV void crweno_line(const Loader &ldc, const Emit &emit, T *__restrict__ s
     T cgp = T(0), dgp = T(0), cfp = T(0), dfp = T(0);
     for (int m = 1; m <= n - 1; ++m) {
         T A, B, C, S;
-        crweno_g_row(g[0], g[1], g[2], g[3], g[4], A, B, C, S);
+        crweno_g_row_fast(g[0], g[1], g[2], g[3], g[4], A, B, C, S);
         if (m == 1) S = S - A * gm0;
         if (m == n - 1) S = S - C * gmN;
         T cg, dg;
-        if (m == 1) {
-            cg = C / B;
-            dg = S / B;
-        } else {
-            T den = B - A * cgp;
-            cg = C / den;
-            dg = (S - A * dgp) / den;
+        {
+            const T r = T(1) / (m == 1 ? B : B - A * cgp);
+            cg = C * r;
+            dg = (m == 1 ? S : S - A * dgp) * r;
         }
-        crweno_f_row(f[1], f[2], f[3], f[4], f[5], A, B, C, S);
+        crweno_f_row_fast(f[1], f[2], f[3], f[4], f[5], A, B, C, S);
         if (m == 1) S = S - A * fp0;
         if (m == n - 1) S = S - C * fpN;
         T cf, df;
-        if (m == 1) {
-            cf = C / B;
-            df = S / B;
-        } else {
-            T den = B - A * cfp;
-            cf = C / den;
-            df = (S - A * dfp) / den;
+        {
+            const T r = T(1) / (m == 1 ? B : B - A * cfp);
+            cf = C * r;
+            df = (m == 1 ? S : S - A * dfp) * r;
         }
         if (m < n - 1 && store) {
             size_t o = (size_t)(m - 1) * 4 * nlines + L;
@@ -147,7 +142,7 @@ LSMP_DEV void crweno_line(const Loader &ldc, const Emit &emit, T *__restrict__ s
     }
     // backward sweep: X(n-1) = d'(n-1)
     T Xg = dgp, Xf = dfp;
-    emit(n, (Xg - gmN) / dd, (Xf - fpN) / dd);
+    emit(n, (Xg - gmN) * inv_dd, (Xf - fpN) * inv_dd);
     for (int m = n - 2; m >= 1; --m) {
         T cg = T(0), dg = T(0), cf = T(0), df = T(0);
         if (store) {
@@ -159,10 +154,10 @@ LSMP_DEV void crweno_line(const Loader &ldc, const Emit &emit, T *__restrict__ s
         }
         T Xg1 = dg - cg * Xg;
         T Xf1 = df - cf * Xf;
-        emit(m + 1, (Xg1 - Xg) / dd, (Xf1 - Xf) / dd);
+        emit(m + 1, (Xg1 - Xg) * inv_dd, (Xf1 - Xf) * inv_dd);
         Xg = Xg1; Xf = Xf1;
     }
-    emit(1, (gm0 - Xg) / dd, (fp0 - Xf) / dd);
+    emit(1, (gm0 - Xg) * inv_dd, (fp0 - Xf) * inv_dd);
 }
 
 // RHS accumulation and fused RK stage (level_set_rk3_solver.f90:16-88, 207-212).
@@ -274,6 +269,7 @@ __global__ __launch_bounds__(64) void k_adv_x(const T *__restrict__ phi, const T
         to[t][q % XCH] = t1 + t2;
     };
 
+    const T inv_dd = T(1) / dd;
     // boundary faces straight from global memory, the interior from the tile
     T gA, fA, gB, fB, gC, fC, gD, fD, gE, fE, gF, fF;
     ldg(-2, gA, fA); ldg(-1, gB, fB); ldg(0, gC, fC); ldg(1, gD, fD); ldg(2, gE, fE); ldg(3, gF, fF);
@@ -291,18 +287,24 @@ __global__ __launch_bounds__(64) void k_adv_x(const T *__restrict__ phi, const T
     const size_t nl = (size_t)nlines;
     for (int m = 1; m <= n - 1; ++m) {
         T A, B, C, S;
-        crweno_g_row(g[0], g[1], g[2], g[3], g[4], A, B, C, S);
+        crweno_g_row_fast(g[0], g[1], g[2], g[3], g[4], A, B, C, S);
         if (m == 1) S = S - A * gm0;
         if (m == n - 1) S = S - C * gmN;
         T cg, dg;
-        if (m == 1) { cg = C / B; dg = S / B; }
-        else { T den = B - A * cgp; cg = C / den; dg = (S - A * dgp) / den; }
-        crweno_f_row(f[1], f[2], f[3], f[4], f[5], A, B, C, S);
+        {
+            const T r = T(1) / (m == 1 ? B : B - A * cgp);
+            cg = C * r;
+            dg = (m == 1 ? S : S - A * dgp) * r;
+        }
+        crweno_f_row_fast(f[1], f[2], f[3], f[4], f[5], A, B, C, S);
         if (m == 1) S = S - A * fp0;
         if (m == n - 1) S = S - C * fpN;
         T cf, df;
-        if (m == 1) { cf = C / B; df = S / B; }
-        else { T den = B - A * cfp; cf = C / den; df = (S - A * dfp) / den; }
+        {
+            const T r = T(1) / (m == 1 ? B : B - A * cfp);
+            cf = C * r;
+            df = (m == 1 ? S : S - A * dfp) * r;
+        }
         if (m < n - 1 && act) {
             size_t o = (size_t)(m - 1) * 4 * nl + L;
             scr[o] = cg; scr[o + nl] = dg; scr[o + 2 * nl] = cf; scr[o + 3 * nl] = df;
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(64) void k_adv_x(const T *__restrict__ phi, const T
         }
     }
     T Xg = dgp, Xf = dfp;
-    emit(n, (Xg - gmN) / dd, (Xf - fpN) / dd);
+    emit(n, (Xg - gmN) * inv_dd, (Xf - fpN) * inv_dd);
     for (int m = n - 2; m >= 1; --m) {
         T cg = T(0), dg = T(0), cf = T(0), df = T(0);
         if (act) {
@@ -324,10 +326,10 @@ __global__ __launch_bounds__(64) void k_adv_x(const T *__restrict__ phi, const T
         }
         T Xg1 = dg - cg * Xg;
         T Xf1 = df - cf * Xf;
-        emit(m + 1, (Xg1 - Xg) / dd, (Xf1 - Xf) / dd);
+        emit(m + 1, (Xg1 - Xg) * inv_dd, (Xf1 - Xf) * inv_dd);
         Xg = Xg1; Xf = Xf1;
     }
-    emit(1, (gm0 - Xg) / dd, (fp0 - Xf) / dd);
+    emit(1, (gm0 - Xg) * inv_dd, (fp0 - Xf) * inv_dd);
     flush(och);
 }
 
@@ -627,7 +629,7 @@ __global__ void k_scale_by_max(T *__restrict__ phi, size_t n, const unsigned lon
 // asgn_src_ccd ccd_solvers.f90:291-310; TWIN_BKS matrix_solver.f90:47-78).
 // A box line has nb = n + 6 points; box point b holds interior cell
 // clamp(b - 3).  coef holds the factorised tables of this direction as
-// A, B, AA, BB, each (3, nb) column-major.  The forward sweep stores (S, SS)
+// A, B, AA, BB, each (3, nb) column-major, then 1/Den of each row.  The forward sweep stores (S, SS)
 // of rows 0..nb-2 in scr[(b*2 + comp)*nlines + L].
 // ---------------------------------------------------------------------------
 template <typename T, class Loader, class Emit>
@@ -635,15 +637,15 @@ LSMP_DEV void ccd_line(const Loader &ldq, const Emit &emit, const T *__restrict_
                        size_t nlines, size_t L, int n, T dx, bool store)
 {
     const int nb = n + 6;
-    const T *A = coef, *B = coef + 3 * nb, *AA = coef + 6 * nb, *BB = coef + 9 * nb;
-    const T dx2 = dx * dx;
+    const T *A = coef, *B = coef + 3 * nb, *AA = coef + 6 * nb, *BB = coef + 9 * nb, *IDEN = coef + 12 * nb;
+    const T idx = T(1) / dx, idx2 = T(1) / (dx * dx);
     // sliding window f(b-1), f(b), f(b+1)
     T fm1, f0, fp1;
     T S, SS;
     {
         T a0 = ldq(-3), a1 = ldq(-2), a2 = ldq(-1), a3 = ldq(0);
-        S = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) / dx;
-        SS = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) / dx2;
+        S = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) * idx;
+        SS = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) * idx2;
         fm1 = a0; f0 = a1; fp1 = a2;
     }
     if (store) {
@@ -651,8 +653,8 @@ LSMP_DEV void ccd_line(const Loader &ldq, const Emit &emit, const T *__restrict_
         scr[nlines + L] = SS;
     }
     for (int b = 1; b <= nb - 2; ++b) {
-        T s = T(15.0 / 16.0) * (-fm1 + fp1) / dx;
-        T ss = (T(3) * fm1 - T(6) * f0 + T(3) * fp1) / dx2;
+        T s = T(15.0 / 16.0) * (-fm1 + fp1) * idx;
+        T ss = (T(3) * fm1 - T(6) * f0 + T(3) * fp1) * idx2;
         s = s - (A[3 * b] * S + B[3 * b] * SS);
         ss = ss - (AA[3 * b] * S + BB[3 * b] * SS);
         S = s; SS = ss;
@@ -667,13 +669,13 @@ LSMP_DEV void ccd_line(const Loader &ldq, const Emit &emit, const T *__restrict_
         const int b = nb - 1;
         // one-sided closure of the last row reads box points nb-1 .. nb-4
         T e0 = ldq(b - 3), e1 = ldq(b - 4), e2 = ldq(b - 5), e3 = ldq(b - 6);
-        T s = -(T(-3.5) * e0 + T(4) * e1 - T(0.5) * e2) / dx;
-        T ss = (T(34.0 / 3.0) * e0 - T(83.0 / 4.0) * e1 + T(10) * e2 - T(7.0 / 12.0) * e3) / dx2;
+        T s = -(T(-3.5) * e0 + T(4) * e1 - T(0.5) * e2) * idx;
+        T ss = (T(34.0 / 3.0) * e0 - T(83.0 / 4.0) * e1 + T(10) * e2 - T(7.0 / 12.0) * e3) * idx2;
         s = s - (A[3 * b] * S + B[3 * b] * SS);
         ss = ss - (AA[3 * b] * S + BB[3 * b] * SS);
-        T den = A[3 * b + 1] * BB[3 * b + 1] - AA[3 * b + 1] * B[3 * b + 1];
-        T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) / den;
-        T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) / den;
+        const T iden = IDEN[b];
+        T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
+        T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
         S = solS; SS = solSS;
         if (b - 3 >= 0 && b - 3 < n) emit(b - 3, S);
     }
@@ -686,9 +688,9 @@ LSMP_DEV void ccd_line(const Loader &ldq, const Emit &emit, const T *__restrict_
         }
         s = s - (A[3 * b + 2] * S + B[3 * b + 2] * SS);
         ss = ss - (AA[3 * b + 2] * S + BB[3 * b + 2] * SS);
-        T den = A[3 * b + 1] * BB[3 * b + 1] - AA[3 * b + 1] * B[3 * b + 1];
-        T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) / den;
-        T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) / den;
+        const T iden = IDEN[b];
+        T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
+        T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
         S = solS; SS = solSS;
         if (b - 3 >= 0 && b - 3 < n) emit(b - 3, S);
     }
@@ -802,20 +804,20 @@ __global__ __launch_bounds__(64) void k_ccd_x(const T *__restrict__ phi, const T
     // the forward part of ccd_line, inlined here so that the tail reload can
     // switch to global memory
     const int nb = n + 6;
-    const T *A = coef, *B = coef + 3 * nb, *AA = coef + 6 * nb, *BB = coef + 9 * nb;
-    const T dx2 = dx * dx;
+    const T *A = coef, *B = coef + 3 * nb, *AA = coef + 6 * nb, *BB = coef + 9 * nb, *IDEN = coef + 12 * nb;
+    const T idx = T(1) / dx, idx2 = T(1) / (dx * dx);
     const size_t nl = (size_t)nlines;
     T fm1, f0, fp1, S, SS;
     {
         T a0 = ldq(-3), a1 = ldq(-2), a2 = ldq(-1), a3 = ldq(0);
-        S = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) / dx;
-        SS = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) / dx2;
+        S = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) * idx;
+        SS = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) * idx2;
         fm1 = a0; f0 = a1; fp1 = a2;
     }
     if (act) { scr[L] = S; scr[nl + L] = SS; }
     for (int b = 1; b <= nb - 2; ++b) {
-        T s = T(15.0 / 16.0) * (-fm1 + fp1) / dx;
-        T ss = (T(3) * fm1 - T(6) * f0 + T(3) * fp1) / dx2;
+        T s = T(15.0 / 16.0) * (-fm1 + fp1) * idx;
+        T ss = (T(3) * fm1 - T(6) * f0 + T(3) * fp1) * idx2;
         s = s - (A[3 * b] * S + B[3 * b] * SS);
         ss = ss - (AA[3 * b] * S + BB[3 * b] * SS);
         S = s; SS = ss;
@@ -829,13 +831,13 @@ __global__ __launch_bounds__(64) void k_ccd_x(const T *__restrict__ phi, const T
     {
         const int b = nb - 1;
         T e0 = ldq(b - 3), e1 = ldq(b - 4), e2 = ldq(b - 5), e3 = ldq(b - 6);
-        T s = -(T(-3.5) * e0 + T(4) * e1 - T(0.5) * e2) / dx;
-        T ss = (T(34.0 / 3.0) * e0 - T(83.0 / 4.0) * e1 + T(10) * e2 - T(7.0 / 12.0) * e3) / dx2;
+        T s = -(T(-3.5) * e0 + T(4) * e1 - T(0.5) * e2) * idx;
+        T ss = (T(34.0 / 3.0) * e0 - T(83.0 / 4.0) * e1 + T(10) * e2 - T(7.0 / 12.0) * e3) * idx2;
         s = s - (A[3 * b] * S + B[3 * b] * SS);
         ss = ss - (AA[3 * b] * S + BB[3 * b] * SS);
-        T den = A[3 * b + 1] * BB[3 * b + 1] - AA[3 * b + 1] * B[3 * b + 1];
-        S = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) / den;
-        SS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) / den;
+        const T iden = IDEN[b];
+        S = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
+        SS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
     }
     for (int b = nb - 2; b >= 0; --b) {
         T s = T(0), ss = T(0);
@@ -845,9 +847,9 @@ __global__ __launch_bounds__(64) void k_ccd_x(const T *__restrict__ phi, const T
         }
         s = s - (A[3 * b + 2] * S + B[3 * b + 2] * SS);
         ss = ss - (AA[3 * b + 2] * S + BB[3 * b + 2] * SS);
-        T den = A[3 * b + 1] * BB[3 * b + 1] - AA[3 * b + 1] * B[3 * b + 1];
-        T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) / den;
-        T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) / den;
+        const T iden = IDEN[b];
+        T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
+        T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
         S = solS; SS = solSS;
         if (b - 3 >= 0 && b - 3 < n) emit(b - 3, S);
     }
