@@ -125,6 +125,55 @@ int lsmp_kernel_time(lsmp_handle h, const char *name, long long *launches, doubl
 /* Block until all work queued by the handle has finished. */
 int lsmp_synchronize(lsmp_handle h);
 
+/* ------------------------------------------------------------------------
+ * Learned reinitialisation.  No counterpart in the reference: it replaces
+ * most pseudo-steps of level_set_rk3_redis (level_set_rk3_redis.f90:25-74)
+ * by one pass of a small network on the narrow band, followed by a few plain
+ * pseudo-steps and a residual check that falls back to the plain method.
+ *
+ * Network: 128 inputs (the 5x5x5 stencil (phi(nb) - phi(c)) / h with the
+ * x offset fastest, then phi(c) / h, then two zeros) -> 64 SiLU -> 64 SiLU
+ * -> 1, output = (phi_reinit(c) - phi(c)) / h.  Parameters are FP32 masters
+ * (W1[128][64], b1[64], W2[64][64], b2[64], w3[64], b3, row-major in that
+ * order); products run in BF16 on the matrix cores with FP32 accumulation.
+ * h = dx; 3-D grids need dx = dy = dz.
+ * ------------------------------------------------------------------------ */
+typedef struct lsmp_net *lsmp_net_handle;
+
+/* capacity: training pairs the net can hold (0 for inference only);
+ * parameters start from a seeded uniform initialisation. */
+int lsmp_net_create(int capacity, unsigned seed, lsmp_net_handle *out);
+void lsmp_net_destroy(lsmp_net_handle net);
+int lsmp_net_param_count(void);
+int lsmp_net_set_params(lsmp_net_handle net, const float *params, int n);  /* resets Adam */
+int lsmp_net_get_params(lsmp_net_handle net, float *params, int n);
+int lsmp_net_dataset_size(lsmp_net_handle net, int *n);
+/* Run the plain reinitialisation (btn = 1, default timestop) on the solver's
+ * phi and store one (pre, post) pair per cell with |phi_pre| < band_cells dx
+ * until the capacity is reached.  phi is left reinitialised. */
+int lsmp_net_collect(lsmp_net_handle net, lsmp_handle h, double band_cells, int *n_added);
+/* Append n host pairs x[n][128] (rounded to BF16), y[n]; drop all pairs. */
+int lsmp_net_add_samples(lsmp_net_handle net, const float *x, const float *y, int n);
+int lsmp_net_clear_samples(lsmp_net_handle net);
+/* phi <- phi + dx * net(stencil) on |phi| < band_cells dx, nothing else
+ * (the network pass of lsmp_reinit_learned on its own). */
+int lsmp_net_apply(lsmp_net_handle net, lsmp_handle h, double band_cells, int *n_band);
+/* steps Adam steps (lr, beta 0.9/0.999, eps 1e-8) on minibatches of `batch`
+ * pairs drawn with replacement; loss = mean squared error of the last step. */
+int lsmp_net_train(lsmp_net_handle net, int steps, int batch, double lr, unsigned seed, double *loss);
+/* Loss and its gradient for the n host samples x[n][128], y[n] (the rows are
+ * rounded to BF16 first); grad has lsmp_net_param_count() entries. */
+int lsmp_net_eval_grad(lsmp_net_handle net, const float *x, const float *y, int n, double *loss, float *grad);
+/* Learned reinitialisation of the solver's phi: network on |phi| < band_cells dx,
+ * then `sweeps` plain pseudo-steps with the sign of the input field; if the
+ * band residual mean | |grad phi| - 1 | over |phi| < 3 dx exceeds residual_tol,
+ * the input field is reinitialised by the plain method instead (*fallback = 1). */
+int lsmp_reinit_learned(lsmp_handle h, lsmp_net_handle net, double band_cells, int sweeps, double residual_tol,
+                        double *residual, int *fallback, int *n_pseudo);
+/* The band residual mean | |grad phi| - 1 | over |phi| < band_cells dx, with the
+ * Godunov gradient of the current sign field. */
+int lsmp_reinit_residual(lsmp_handle h, double band_cells, double *residual);
+
 #ifdef __cplusplus
 }
 #endif

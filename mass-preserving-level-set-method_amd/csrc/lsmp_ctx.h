@@ -1,0 +1,48 @@
+// lsmp_ctx.h -- state of one solver instance, shared by the host translation
+// units of the library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/lsmp.h"
+#include "lsmp_internal.h"
+
+struct Pending {
+    int phase;            // -1: kernel timing only
+    std::string kname;    // empty: phase timing only
+    hipEvent_t e0, e1;
+};
+
+struct lsmp_ctx {
+    lsmp_params p{};
+    int dim = 3, nx = 0, ny = 0, nz = 1;
+    size_t ncell = 0;
+    bool fp32 = false;
+    size_t esz = 8;
+    hipStream_t st = nullptr;
+    // device fields (element type double or float)
+    void *phi = nullptr, *phiB = nullptr, *phiC = nullptr;
+    void *u = nullptr, *v = nullptr, *w = nullptr;
+    void *R = nullptr, *L1 = nullptr, *L2 = nullptr, *S0 = nullptr, *phi0 = nullptr;
+    void *dxv = nullptr, *dyv = nullptr, *gr = nullptr, *scr = nullptr;
+    void *coef[3] = {nullptr, nullptr, nullptr};
+    double *partial = nullptr, *scal = nullptr;
+    unsigned long long *u64 = nullptr;
+    double *hbuf = nullptr;   // pinned readback
+    int npartial = 0;
+    int *band_list = nullptr;        // narrow-band cell indices (learned reinitialisation)
+    unsigned *band_count = nullptr;
+    bool vel_set = false, phi0_set = false;
+    lsmp_stats stats{};
+    bool profiling = false;
+    std::map<std::string, std::pair<long long, double>> ktime;
+    std::vector<Pending> pend;
+    std::vector<hipEvent_t> pool;
+};
+
+// Plain reinitialisation (lsmp_host.hip), also used by the learned path.
+int lsmp_reinit_run(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, double *err, int *n);
+lsmp::RedisArgs lsmp_redis_args(const lsmp_ctx *c);

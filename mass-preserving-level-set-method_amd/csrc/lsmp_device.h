@@ -312,4 +312,52 @@ template <typename T> LSMP_DEV T heavyside(T x)
     return T(0.5) * ((T(1) + x) + sin(pi * x) / pi);
 }
 
+// Wave- and block-wide reductions (64-lane wavefronts).
+LSMP_DEV double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+LSMP_DEV double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide sum of NV doubles; result valid in thread 0.  blockDim.x must be
+// a multiple of 64 and at most 1024.
+template <int NV> LSMP_DEV void block_sum(double (&v)[NV])
+{
+    __shared__ double red[16][NV];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+    if (lane == 0)
+        for (int q = 0; q < NV; ++q) red[wid][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < NV; ++q) {
+            double s = red[0][q];
+            for (int w = 1; w < nw; ++w) s += red[w][q];
+            v[q] = s;
+        }
+    }
+    __syncthreads();
+}
+
+LSMP_DEV double block_max(double v)
+{
+    __shared__ double red[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    v = wave_max(v);
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < nw; ++w) v = fmax(v, red[w]);
+    __syncthreads();
+    return v;
+}
+
 }  // namespace lsmp

@@ -14,46 +14,15 @@
 
 #include "../../include/lsmp.h"
 #include "lsmp_internal.h"
+#include "lsmp_ctx.h"
 
 using namespace lsmp;
 
 namespace {
 
 constexpr int GHC = 3;
-constexpr int NPHASE = 3;   // advect, reinit, mass fix
-
-struct Pending {
-    int phase;            // -1: kernel timing only
-    std::string kname;    // empty: phase timing only
-    hipEvent_t e0, e1;
-};
 
 }  // namespace
-
-struct lsmp_ctx {
-    lsmp_params p{};
-    int dim = 3, nx = 0, ny = 0, nz = 1;
-    size_t ncell = 0;
-    bool fp32 = false;
-    size_t esz = 8;
-    hipStream_t st = nullptr;
-    // device fields (element type double or float)
-    void *phi = nullptr, *phiB = nullptr, *phiC = nullptr;
-    void *u = nullptr, *v = nullptr, *w = nullptr;
-    void *R = nullptr, *L1 = nullptr, *L2 = nullptr, *S0 = nullptr, *phi0 = nullptr;
-    void *dxv = nullptr, *dyv = nullptr, *gr = nullptr, *scr = nullptr;
-    void *coef[3] = {nullptr, nullptr, nullptr};
-    double *partial = nullptr, *scal = nullptr;
-    unsigned long long *u64 = nullptr;
-    double *hbuf = nullptr;   // pinned readback
-    int npartial = 0;
-    bool vel_set = false, phi0_set = false;
-    lsmp_stats stats{};
-    bool profiling = false;
-    std::map<std::string, std::pair<long long, double>> ktime;
-    std::vector<Pending> pend;
-    std::vector<hipEvent_t> pool;
-};
 
 namespace {
 
@@ -321,13 +290,17 @@ RedisArgs redis_args(const lsmp_ctx *c)
     return ra;
 }
 
-template <typename T> int reinit(lsmp_ctx *c, int btn, double timestop, double *red_error, int *n_pseudo)
+template <typename T>
+int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, double *red_error, int *n_pseudo)
 {
     const RedisArgs ra = redis_args(c);
     const size_t n = c->ncell;
     T *S0 = (T *)c->S0;
     // level_set_redis_init / level_set_redis_stable (redis.f90:81-174)
-    if (btn == 1) {
+    if (!init_s0) {
+        // S0 was set by the caller (the learned path keeps the sign of the
+        // field before the network's correction)
+    } else if (btn == 1) {
         KT(c, "sign_init", (launch_sign_init<T>(c->st, (const T *)c->phi, S0, n, (T)c->p.ls_wid, 0)));
     } else {
         KT(c, "sign_init", (launch_sign_init<T>(c->st, (const T *)c->phi, S0, n, (T)c->p.ls_wid, 1)));
@@ -365,6 +338,7 @@ template <typename T> int reinit(lsmp_ctx *c, int btn, double timestop, double *
         c->phiB = c->phi;
         c->phi = b;
         if (time > timestop || error <= 1.0e-6) break;
+        if (max_iter > 0 && iter >= max_iter) break;
     }
     if (std::isnan(error)) return LSMP_E_NAN;
     c->stats.last_redis_iters = iter;
@@ -490,7 +464,7 @@ template <typename T> int velocity_analytic(lsmp_ctx *c, int field, double t, do
 void free_all(lsmp_ctx *c)
 {
     void *bufs[] = {c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
-                    c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64};
+                    c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
@@ -500,6 +474,16 @@ void free_all(lsmp_ctx *c)
 }
 
 }  // namespace
+
+// Plain reinitialisation, shared with the learned path: max_iter > 0 caps the
+// pseudo-steps, init_s0 = false keeps the sign field already in S0.
+int lsmp_reinit_run(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, double *err, int *n)
+{
+    return c->fp32 ? reinit<float>(c, btn, timestop, max_iter, init_s0, err, n)
+                   : reinit<double>(c, btn, timestop, max_iter, init_s0, err, n);
+}
+
+RedisArgs lsmp_redis_args(const lsmp_ctx *c) { return redis_args(c); }
 
 // ====================================================================== ABI
 extern "C" {
@@ -647,8 +631,7 @@ int lsmp_reinit(lsmp_handle h, int btn, double timestop, double *red_error, int 
 {
     if (!h || (btn != 0 && btn != 1)) return LSMP_E_INVALID;
     Timer tm(h, 1, nullptr, true);
-    int rc = h->fp32 ? reinit<float>(h, btn, timestop, red_error, n_pseudo)
-                     : reinit<double>(h, btn, timestop, red_error, n_pseudo);
+    int rc = lsmp_reinit_run(h, btn, timestop, 0, true, red_error, n_pseudo);
     tm.stop();
     h->stats.n_reinit++;
     resolve(h, false);

@@ -1,0 +1,160 @@
+"""Training set and training loop of the learned reinitialisation.
+
+Pairs come from the plain method on random seeded shapes and flows:
+
+* shapes: unions of 1 to 8 ellipsoids, semi-axes uniform in [0.05, 0.2],
+  centres uniform in [0.25, 0.75]^3, random orientation, phi > 0 inside;
+* flows: divergence-free random Fourier fields u = curl A with integer
+  wavenumber vectors of max-norm 1..4, scaled to max |u| = 1 on the grid;
+* each shape is reinitialised by the plain method, then advected a random
+  number of RK3 steps without reinitialisation (CFL 0.1 of the unit speed),
+  and the next plain reinitialisation gives one (pre, post) pair per band
+  cell; this repeats along the trajectory.
+
+Everything runs on the GPU through the library (advection, plain
+reinitialisation, pair collection, BF16 forward/backward and Adam kernels);
+numpy only draws the random shapes and flows as host boxes.
+
+    python mass-preserving-level-set-method_amd/learned.py --out weights.npy
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+import lsmp  # noqa: E402
+
+WEIGHTS = os.path.join(HERE, "weights", "reinit_mlp.npy")
+
+
+def cell_coords(n):
+    """Cell-centre coordinates of the box indices -3 .. n+2 on the unit interval."""
+    return (np.arange(-3, n + 3) + 0.5) / n
+
+
+def random_rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.diag(r))
+
+
+def ellipsoid_union(n, rng, kmin=1, kmax=8):
+    """Box (n+6)^3 of max_e (1 - |A^-1 R^T (x - c)|) min(a): positive inside the union."""
+    x = cell_coords(n)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    pts = np.stack([X, Y, Z], axis=-1)
+    phi = np.full(X.shape, -np.inf)
+    for _ in range(int(rng.integers(kmin, kmax + 1))):
+        c = rng.uniform(0.25, 0.75, size=3)
+        a = rng.uniform(0.05, 0.2, size=3)
+        R = random_rotation(rng)
+        loc = (pts - c) @ R
+        r = np.sqrt(np.sum((loc / a) ** 2, axis=-1))
+        phi = np.maximum(phi, (1.0 - r) * a.min())
+    return np.ascontiguousarray(phi)
+
+
+def fourier_velocity(n, rng, kmax=4, modes=6):
+    """Divergence-free u = curl A, A = sum_m a_m sin(2 pi k_m . x + theta_m), max |u| = 1."""
+    x = cell_coords(n)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    u = np.zeros((3,) + X.shape)
+    for _ in range(modes):
+        while True:
+            k = rng.integers(-kmax, kmax + 1, size=3)
+            if np.abs(k).max() >= 1:
+                break
+        a = rng.normal(size=3)
+        th = rng.uniform(0, 2 * np.pi)
+        arg = 2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z) + th
+        # curl(a sin(arg)) = 2 pi cos(arg) (k x a)
+        kxa = np.cross(k, a)
+        c = 2 * np.pi * np.cos(arg)
+        for d in range(3):
+            u[d] += c * kxa[d]
+    u /= np.sqrt((u ** 2).sum(0)).max()
+    return [np.ascontiguousarray(u[d]) for d in range(3)]
+
+
+def generate(net, sizes=(96, 128), shapes=24, chain=4, max_adv=8, band_cells=6.0, seed=0,
+             precision=lsmp.FP32, log=print):
+    """Fill the net's training set; returns the number of pairs."""
+    rng = np.random.default_rng(seed)
+    total = 0
+    for si in range(shapes):
+        n = int(sizes[si % len(sizes)])
+        with lsmp.Solver(lsmp.make_params(n, dim=3, precision=precision)) as s:
+            s.upload_phi(ellipsoid_union(n, rng))
+            s.reinit(1)
+            s.reinit(1)
+            s.set_velocity(*fourier_velocity(n, rng))
+            for _ in range(chain):
+                for _ in range(int(rng.integers(1, max_adv + 1))):
+                    s.rk3_advect()
+                got = net.collect(s, band_cells)
+                total += got
+                if got == 0:
+                    break
+        log(f"  shape {si + 1}/{shapes}: n={n} pairs so far {total}")
+    return total
+
+
+def train(net, steps=6000, batch=16384, lr=2e-3, lr_end=2e-5, chunk=200, seed=0, log=print):
+    """Adam with a geometric learning-rate decay; returns the loss history."""
+    hist = []
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        frac = done / max(steps - 1, 1)
+        cur = lr * (lr_end / lr) ** frac
+        loss = net.train(k, batch=batch, lr=cur, seed=seed)
+        done += k
+        hist.append((done, cur, loss))
+        log(f"  step {done}: lr {cur:.2e} loss {loss:.4e}")
+    return hist
+
+
+def load_net(path=WEIGHTS, capacity=0):
+    """A network with the shipped (or given) trained parameters."""
+    net = lsmp.Net(capacity=capacity)
+    net.load(path)
+    return net
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--capacity", type=int, default=3_000_000)
+    ap.add_argument("--shapes", type=int, default=24)
+    ap.add_argument("--chain", type=int, default=4)
+    ap.add_argument("--sizes", default="96,128")
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    t0 = time.time()
+    net = lsmp.Net(capacity=args.capacity, seed=args.seed + 1)
+    sizes = [int(v) for v in args.sizes.split(",")]
+    n = generate(net, sizes=sizes, shapes=args.shapes, chain=args.chain, seed=args.seed)
+    print(f"pairs {n} ({time.time() - t0:.1f} s)", flush=True)
+    t1 = time.time()
+    hist = train(net, steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed)
+    print(f"trained {args.steps} steps in {time.time() - t1:.1f} s", flush=True)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    net.save(args.out)
+    meta = dict(pairs=n, steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed, sizes=sizes,
+                shapes=args.shapes, chain=args.chain, final_loss=hist[-1][2])
+    with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

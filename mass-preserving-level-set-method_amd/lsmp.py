@@ -29,7 +29,12 @@ EXPORTS = [
     "lsmp_store_phi0", "lsmp_set_velocity", "lsmp_set_velocity_analytic", "lsmp_rk3_advect",
     "lsmp_reinit", "lsmp_mass_vol", "lsmp_mass_fix", "lsmp_interface_l1", "lsmp_get_stats",
     "lsmp_set_profiling", "lsmp_kernel_time", "lsmp_synchronize",
+    "lsmp_net_create", "lsmp_net_destroy", "lsmp_net_param_count", "lsmp_net_set_params",
+    "lsmp_net_get_params", "lsmp_net_dataset_size", "lsmp_net_collect", "lsmp_net_train",
+    "lsmp_net_eval_grad", "lsmp_reinit_learned", "lsmp_reinit_residual", "lsmp_net_add_samples",
+    "lsmp_net_clear_samples", "lsmp_net_apply",
 ]
+NO_STATUS = ("lsmp_finalize", "lsmp_error_string", "lsmp_net_destroy")
 
 
 class Params(C.Structure):
@@ -84,8 +89,25 @@ def load_library(path=LIB_PATH):
     L.lsmp_set_profiling.argtypes = [H, C.c_int]
     L.lsmp_kernel_time.argtypes = [H, C.c_char_p, C.POINTER(C.c_longlong), _DP]
     L.lsmp_synchronize.argtypes = [H]
+    _FP = C.POINTER(C.c_float)
+    _IP = C.POINTER(C.c_int)
+    L.lsmp_net_create.argtypes = [C.c_int, C.c_uint, C.POINTER(H)]
+    L.lsmp_net_destroy.argtypes = [H]
+    L.lsmp_net_destroy.restype = None
+    L.lsmp_net_param_count.argtypes = []
+    L.lsmp_net_set_params.argtypes = [H, _FP, C.c_int]
+    L.lsmp_net_get_params.argtypes = [H, _FP, C.c_int]
+    L.lsmp_net_dataset_size.argtypes = [H, _IP]
+    L.lsmp_net_collect.argtypes = [H, H, C.c_double, _IP]
+    L.lsmp_net_train.argtypes = [H, C.c_int, C.c_int, C.c_double, C.c_uint, _DP]
+    L.lsmp_net_eval_grad.argtypes = [H, _FP, _FP, C.c_int, _DP, _FP]
+    L.lsmp_reinit_learned.argtypes = [H, H, C.c_double, C.c_int, C.c_double, _DP, _IP, _IP]
+    L.lsmp_reinit_residual.argtypes = [H, C.c_double, _DP]
+    L.lsmp_net_add_samples.argtypes = [H, _FP, _FP, C.c_int]
+    L.lsmp_net_clear_samples.argtypes = [H]
+    L.lsmp_net_apply.argtypes = [H, H, C.c_double, _IP]
     for name in EXPORTS:
-        if name not in ("lsmp_finalize", "lsmp_error_string"):
+        if name not in NO_STATUS:
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -221,14 +243,137 @@ class Solver:
     def synchronize(self):
         _check(self.lib.lsmp_synchronize(self.h))
 
-    def mpls_step(self, t, t2s, field, imass, t2red=1, iteration=1):
+    def reinit_learned(self, net, band_cells=6.0, sweeps=2, residual_tol=1e-2):
+        """Learned reinitialisation (no reference counterpart): network on the
+        band, `sweeps` plain pseudo-steps, plain fallback above residual_tol.
+        Returns (residual, fallback, n_pseudo)."""
+        r = C.c_double()
+        fb = C.c_int()
+        n = C.c_int()
+        _check(self.lib.lsmp_reinit_learned(self.h, net.h, band_cells, sweeps, residual_tol, C.byref(r),
+                                            C.byref(fb), C.byref(n)))
+        return r.value, bool(fb.value), n.value
+
+    def reinit_residual(self, band_cells=3.0):
+        """Mean | |grad phi| - 1 | over |phi| < band_cells dx."""
+        r = C.c_double()
+        _check(self.lib.lsmp_reinit_residual(self.h, band_cells, C.byref(r)))
+        return r.value
+
+    def mpls_step(self, t, t2s, field, imass, t2red=1, iteration=1, net=None, sweeps=2, residual_tol=1e-2):
         """One MPLS step as the merged branch of mpls runs it
         (interface_solver.f90:123-129) after the driver set the velocity at
-        t^{n+1} (main.f90:23-29).  Returns (n_pseudo, n_mass)."""
+        t^{n+1} (main.f90:23-29).  With a trained `net` the reinitialisation
+        is the learned one.  Returns (n_pseudo, n_mass)."""
         self.set_velocity_analytic(field, t, t2s)
         self.rk3_advect()
         npseudo = 0
         if iteration % t2red == 0:
-            _, npseudo = self.reinit(1)
+            if net is None:
+                _, npseudo = self.reinit(1)
+            else:
+                _, _, npseudo = self.reinit_learned(net, sweeps=sweeps, residual_tol=residual_tol)
         _, nm = self.mass_fix(imass)
         return npseudo, nm
+
+
+NF, NH = 128, 64
+PARAM_SLICES = {"W1": (0, (NF, NH)), "b1": (NF * NH, (NH,)), "W2": (NF * NH + NH, (NH, NH)),
+                "b2": (NF * NH + NH + NH * NH, (NH,)), "w3": (NF * NH + 2 * NH + NH * NH, (NH,)),
+                "b3": (NF * NH + 3 * NH + NH * NH, (1,))}
+
+
+def unpack_params(flat):
+    """Split the flat parameter vector into the named arrays (views)."""
+    return {k: flat[o:o + int(np.prod(sh))].reshape(sh) for k, (o, sh) in PARAM_SLICES.items()}
+
+
+class Net:
+    """The learned-reinitialisation network and its training set on the GPU."""
+
+    def __init__(self, capacity=0, seed=1):
+        self.lib = load_library()
+        self.h = C.c_void_p()
+        _check(self.lib.lsmp_net_create(int(capacity), int(seed), C.byref(self.h)))
+        self.nparam = self.lib.lsmp_net_param_count()
+
+    def close(self):
+        if self.h:
+            self.lib.lsmp_net_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get_params(self):
+        out = np.empty(self.nparam, dtype=np.float32)
+        _check(self.lib.lsmp_net_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), self.nparam))
+        return out
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        if flat.size != self.nparam:
+            raise ValueError(f"expected {self.nparam} parameters, got {flat.size}")
+        _check(self.lib.lsmp_net_set_params(self.h, flat.ctypes.data_as(C.POINTER(C.c_float)), self.nparam))
+
+    def save(self, path):
+        np.save(path, self.get_params())
+
+    def load(self, path):
+        self.set_params(np.load(path, allow_pickle=False))
+
+    def dataset_size(self):
+        n = C.c_int()
+        _check(self.lib.lsmp_net_dataset_size(self.h, C.byref(n)))
+        return n.value
+
+    def collect(self, solver, band_cells=6.0):
+        """Plain-reinitialise the solver's phi and store the band pairs."""
+        n = C.c_int()
+        _check(self.lib.lsmp_net_collect(self.h, solver.h, band_cells, C.byref(n)))
+        return n.value
+
+    def add_samples(self, x, y):
+        """Append host pairs x (n, 128), y (n,) to the training set."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        if x.ndim != 2 or x.shape[1] != NF or y.shape != (x.shape[0],):
+            raise ValueError("x must be (n, 128) and y (n,)")
+        _check(self.lib.lsmp_net_add_samples(self.h, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                             y.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0]))
+
+    def clear_samples(self):
+        _check(self.lib.lsmp_net_clear_samples(self.h))
+
+    def apply(self, solver, band_cells=6.0):
+        """The network pass alone on the solver's phi; returns the band size."""
+        n = C.c_int()
+        _check(self.lib.lsmp_net_apply(self.h, solver.h, band_cells, C.byref(n)))
+        return n.value
+
+    def train(self, steps, batch=4096, lr=1e-3, seed=0):
+        loss = C.c_double()
+        _check(self.lib.lsmp_net_train(self.h, int(steps), int(batch), float(lr), int(seed), C.byref(loss)))
+        return loss.value
+
+    def eval_grad(self, x, y):
+        """Loss and gradient on host samples x (n, 128), y (n,)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        if x.ndim != 2 or x.shape[1] != NF or y.shape != (x.shape[0],):
+            raise ValueError("x must be (n, 128) and y (n,)")
+        g = np.empty(self.nparam, dtype=np.float32)
+        loss = C.c_double()
+        _check(self.lib.lsmp_net_eval_grad(self.h, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                           y.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0], C.byref(loss),
+                                           g.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss.value, g
