@@ -77,14 +77,33 @@ def measured_traffic(kernel, n, fp64):
     return round(sum(hits) / len(hits)) if hits else None
 
 
-def run_steps(s, nsteps, t, imass, field):
+def run_steps(s, nsteps, t, imass, field, net=None):
     npseudo = nmass = 0
     for _ in range(nsteps):
         t = t + s.params.dt
-        a, b = s.mpls_step(t, T2S, field, imass)
+        a, b = s.mpls_step(t, T2S, field, imass, net=net)
         npseudo += a
         nmass += b
     return t, npseudo, nmass
+
+
+def form_window(lsmp, n, prec, steps, warmup, net=None):
+    """Cell-updates/s of one form over the same window of steps."""
+    s = lsmp.Solver(lsmp.make_params(n, dim=3, precision=prec))
+    s.upload_phi(sphere_box(n))
+    s.store_phi0()
+    imass, _ = s.mass_vol()
+    t, _, _ = run_steps(s, warmup, 0.0, imass, lsmp.VEL_LEVEQUE3D, net)
+    s.synchronize()
+    t0 = time.perf_counter()
+    t, npseudo, _ = run_steps(s, steps, t, imass, lsmp.VEL_LEVEQUE3D, net)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    mass, _ = s.mass_vol()
+    out = {"value": n ** 3 * steps / el, "ms_per_step": el / steps * 1e3,
+           "pseudo_steps_per_step": npseudo / steps, "dM_over_M": abs(mass - imass) / imass}
+    s.close()
+    return out
 
 
 def cpu_baseline(n_sample=64, steps=3):
@@ -119,6 +138,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP32 and learned-form windows")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's sharding "
@@ -205,6 +225,19 @@ def main():
         "kernels": kernels,
     }
     s.close()
+    if not args.no_forms:
+        # the chosen forms beside the plain one, over the same window of steps
+        forms = {}
+        log("bench: FP32 bulk plain window")
+        forms["fp32-plain"] = form_window(lsmp, n, lsmp.FP32, args.steps, args.warmup)
+        log("bench: FP32 bulk + learned reinitialisation window")
+        net = lsmp.Net()
+        net.load(os.path.join(PKG, "weights", "reinit_mlp.npy"))
+        forms["fp32-learned"] = form_window(lsmp, n, lsmp.FP32, args.steps, args.warmup, net)
+        net.close()
+        for f in forms.values():
+            f["vs_fp64_plain"] = round(f["value"] / value, 3)
+        out["forms"] = forms
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle on the host)")
         out["cpu_baseline"] = cpu_baseline()

@@ -135,7 +135,9 @@ int lsmp_synchronize(lsmp_handle h);
  * x offset fastest, then phi(c) / h, then two zeros) -> 64 SiLU -> 64 SiLU
  * -> 1, output = (phi_reinit(c) - phi(c)) / h.  Parameters are FP32 masters
  * (W1[128][64], b1[64], W2[64][64], b2[64], w3[64], b3, row-major in that
- * order); products run in BF16 on the matrix cores with FP32 accumulation.
+ * order).  Products run on the matrix cores in split BF16: each FP32 operand
+ * is hi + lo with hi = bf16(v), lo = bf16(v - hi), and a product is
+ * hi*hi + lo*hi + hi*lo with FP32 accumulation (about 16 significant bits).
  * h = dx; 3-D grids need dx = dy = dz.
  * ------------------------------------------------------------------------ */
 typedef struct lsmp_net *lsmp_net_handle;
@@ -149,27 +151,42 @@ int lsmp_net_set_params(lsmp_net_handle net, const float *params, int n);  /* re
 int lsmp_net_get_params(lsmp_net_handle net, float *params, int n);
 int lsmp_net_dataset_size(lsmp_net_handle net, int *n);
 /* Run the plain reinitialisation (btn = 1, default timestop) on the solver's
- * phi and store one (pre, post) pair per cell with |phi_pre| < band_cells dx
- * until the capacity is reached.  phi is left reinitialised. */
-int lsmp_net_collect(lsmp_net_handle net, lsmp_handle h, double band_cells, int *n_added);
-/* Append n host pairs x[n][128] (rounded to BF16), y[n]; drop all pairs. */
+ * phi and store one (pre, post) pair for every stride-th cell (ascending
+ * index order) with |phi_pre| < band_cells dx until the capacity is reached.
+ * phi is left reinitialised. */
+int lsmp_net_collect(lsmp_net_handle net, lsmp_handle h, double band_cells, int stride, int *n_added);
+/* Append n host pairs x[n][128], y[n] (FP32); drop all pairs. */
 int lsmp_net_add_samples(lsmp_net_handle net, const float *x, const float *y, int n);
 int lsmp_net_clear_samples(lsmp_net_handle net);
-/* phi <- phi + dx * net(stencil) on |phi| < band_cells dx, nothing else
- * (the network pass of lsmp_reinit_learned on its own). */
+/* phi <- phi + dx * net(stencil) * clamp((|phi| - ls_wid) / ls_wid, 0, 1) on
+ * |phi| < band_cells dx, nothing else (the network pass of
+ * lsmp_reinit_learned on its own).  The network leaves the interface region
+ * |phi| <= ls_wid to the plain sweeps and the mass correction. */
 int lsmp_net_apply(lsmp_net_handle net, lsmp_handle h, double band_cells, int *n_band);
 /* steps Adam steps (lr, beta 0.9/0.999, eps 1e-8) on minibatches of `batch`
- * pairs drawn with replacement; loss = mean squared error of the last step. */
-int lsmp_net_train(lsmp_net_handle net, int steps, int batch, double lr, unsigned seed, double *loss);
-/* Loss and its gradient for the n host samples x[n][128], y[n] (the rows are
- * rounded to BF16 first); grad has lsmp_net_param_count() entries. */
+ * pairs drawn with replacement, loss = mean of w (o - y)^2 with
+ * w = c^2 / (y^2 + c^2) for weight_c = c > 0 (relative error for |y| >> c)
+ * and w = 1 for weight_c = 0; *loss = that mean over the last minibatch. */
+int lsmp_net_train(lsmp_net_handle net, int steps, int batch, double lr, unsigned seed, double weight_c,
+                   double *loss);
+/* Loss and its gradient for the n host samples x[n][128], y[n]; grad has
+ * lsmp_net_param_count() entries. */
 int lsmp_net_eval_grad(lsmp_net_handle net, const float *x, const float *y, int n, double *loss, float *grad);
 /* Learned reinitialisation of the solver's phi: network on |phi| < band_cells dx,
- * then `sweeps` plain pseudo-steps with the sign of the input field; if the
- * band residual mean | |grad phi| - 1 | over |phi| < 3 dx exceeds residual_tol,
- * the input field is reinitialised by the plain method instead (*fallback = 1). */
+ * then `sweeps` plain pseudo-steps with the sign of the input field.  If the
+ * residual mean | |grad phi| - 1 | over |phi| < residual_band dx then exceeds
+ * residual_tol (*fallback = 1):
+ *   LSMP_FALLBACK_RESTART  -- the input field is reinitialised by the plain
+ *                             method instead (its result exactly);
+ *   LSMP_FALLBACK_CONTINUE -- plain pseudo-steps continue from the current
+ *                             field, `sweeps` at a time, until the residual
+ *                             passes, the plain stop (max change <= 1e-6) is
+ *                             met or the plain pseudo-time budget is spent.
+ * *residual is the final field's residual, *n_pseudo all pseudo-steps run. */
+#define LSMP_FALLBACK_RESTART 0
+#define LSMP_FALLBACK_CONTINUE 1
 int lsmp_reinit_learned(lsmp_handle h, lsmp_net_handle net, double band_cells, int sweeps, double residual_tol,
-                        double *residual, int *fallback, int *n_pseudo);
+                        double residual_band, int fallback_mode, double *residual, int *fallback, int *n_pseudo);
 /* The band residual mean | |grad phi| - 1 | over |phi| < band_cells dx, with the
  * Godunov gradient of the current sign field. */
 int lsmp_reinit_residual(lsmp_handle h, double band_cells, double *residual);

@@ -57,21 +57,50 @@ LSMP_DEV float across_groups_sum(float v)   // sum over the 4 lanes sharing lane
 
 struct Geom {
     int nx, ny, nz;
-    float inv_h;
+    float inv_h;      // 1 / dx
+    float inv_12h;    // 1 / (12 dx)
 };
 
-// Feature q of the cell at (i, j, k): q < 125 -> (phi(nb) - phi_c)/h over the
-// 5^3 stencil (di fastest, clamped indices), q = 125 -> phi_c/h, else 0.
+template <typename T> LSMP_DEV T load_clamped(const T *__restrict__ phi, const Geom &g, int a, int b, int c)
+{
+    a = clampi(a, 0, g.nx - 1);
+    b = clampi(b, 0, g.ny - 1);
+    c = clampi(c, 0, g.nz - 1);
+    return phi[((size_t)c * g.ny + b) * g.nx + a];
+}
+
+// |grad phi| at the cell from fourth-order central differences (clamped indices).
+template <typename T> LSMP_DEV float grad_norm4(const T *__restrict__ phi, const Geom &g, int i, int j, int k)
+{
+    auto d4 = [&](int di, int dj, int dk) {
+        const T m2 = load_clamped(phi, g, i - 2 * di, j - 2 * dj, k - 2 * dk);
+        const T m1 = load_clamped(phi, g, i - di, j - dj, k - dk);
+        const T p1 = load_clamped(phi, g, i + di, j + dj, k + dk);
+        const T p2 = load_clamped(phi, g, i + 2 * di, j + 2 * dj, k + 2 * dk);
+        return (float)((m2 - p2) + T(8) * (p1 - m1)) * g.inv_12h;
+    };
+    const float gx = d4(1, 0, 0), gy = d4(0, 1, 0), gz = d4(0, 0, 1);
+    return sqrtf(gx * gx + gy * gy + gz * gz);
+}
+
+// Feature q of the cell at (i, j, k):
+//   q < 125:  (phi(nb) - phi_c) / h over the 5^3 stencil (x offset fastest,
+//             clamped indices: the zero-gradient ghost rule);
+//   q = 125:  phi_c / h;
+//   q = 126:  |grad phi| - 1 (fourth-order central differences);
+//   q = 127:  (phi_c / h) (1 / max(|grad phi|, 0.1) - 1), the change that
+//             the first-order distance estimate phi / |grad phi| would make.
 template <typename T>
 LSMP_DEV float feature(const T *__restrict__ phi, const Geom &g, int i, int j, int k, T pc, int q)
 {
     if (q < 125) {
         const int di = q % 5 - 2, dj = (q / 5) % 5 - 2, dk = q / 25 - 2;
-        const int a = clampi(i + di, 0, g.nx - 1), b = clampi(j + dj, 0, g.ny - 1), c = clampi(k + dk, 0, g.nz - 1);
-        return (float)(phi[((size_t)c * g.ny + b) * g.nx + a] - pc) * g.inv_h;
+        return (float)(load_clamped(phi, g, i + di, j + dj, k + dk) - pc) * g.inv_h;
     }
     if (q == 125) return (float)pc * g.inv_h;
-    return 0.0f;
+    const float gm = grad_norm4(phi, g, i, j, k);
+    if (q == 126) return gm - 1.0f;
+    return (float)pc * g.inv_h * (1.0f / fmaxf(gm, 0.1f) - 1.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -155,19 +184,20 @@ __global__ __launch_bounds__(256) void k_band_write(const T *__restrict__ phi, s
         if (f[r]) list[pos++] = (int)(q0 + r);
 }
 
+// Pair c takes band entry first + c * stride.
 template <typename T>
 __global__ void k_collect(const T *__restrict__ pre, const T *__restrict__ post, const int *__restrict__ list,
-                          int nlist, Geom g, __bf16 *__restrict__ X, float *__restrict__ Y, unsigned base,
-                          unsigned cap)
+                          int nlist, int first, int stride, Geom g, float *__restrict__ X, float *__restrict__ Y,
+                          unsigned base, unsigned cap)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nlist) return;
     const size_t s = (size_t)base + c;
     if (s >= cap) return;
-    const int idx = list[c];
+    const int idx = list[first + (size_t)c * stride];
     const int i = idx % g.nx, j = (idx / g.nx) % g.ny, k = idx / (g.nx * g.ny);
     const T pc = pre[idx];
-    for (int q = 0; q < NF; ++q) X[s * NF + q] = (__bf16)feature(pre, g, i, j, k, pc, q);
+    for (int q = 0; q < NF; ++q) X[s * NF + q] = feature(pre, g, i, j, k, pc, q);
     Y[s] = (float)(post[idx] - pc) * g.inv_h;
 }
 
@@ -208,24 +238,36 @@ __global__ void k_residual(const T *__restrict__ phi, const T *__restrict__ S0, 
 // ---------------------------------------------------------------------------
 // network
 // ---------------------------------------------------------------------------
+// Split-BF16 products: an FP32 value v is carried as hi = bf16(v) and
+// lo = bf16(v - hi) (v - hi is exact in FP32), and a product of two such
+// operands is hi*hi + lo*hi + hi*lo, three BF16 MFMAs accumulated in FP32
+// (the dropped lo*lo term is below 2^-16 relative).
+LSMP_DEV void split_bf16(float v, __bf16 &hi, __bf16 &lo)
+{
+    hi = (__bf16)v;
+    lo = (__bf16)(v - (float)hi);
+}
+
 struct NetLds {
-    __attribute__((aligned(16))) __bf16 w1t[NH][NF];   // W1 transposed: [n][k]
-    __attribute__((aligned(16))) __bf16 w2t[NH][NH];   // W2 transposed: [n2][n1]
+    __attribute__((aligned(16))) __bf16 w1h[NH][NF];   // W1 transposed, [n][k], hi part
+    __attribute__((aligned(16))) __bf16 w1l[NH][NF];   // lo part
+    __attribute__((aligned(16))) __bf16 w2h[NH][NH];   // W2 transposed, [n2][n1]
+    __attribute__((aligned(16))) __bf16 w2l[NH][NH];
     float b1[NH], b2[NH], w3[NH];
     float b3;
 };
 
+// w2: optional [n1][n2] copy of the hi part of W2 for the backward pass.
 LSMP_DEV void stage_weights(NetLds &L, const float *__restrict__ P, __bf16 (*w2)[NH])
 {
     for (int idx = threadIdx.x; idx < NF * NH; idx += blockDim.x) {
         const int k = idx / NH, n = idx % NH;
-        L.w1t[n][k] = (__bf16)P[OFF_W1 + idx];
+        split_bf16(P[OFF_W1 + idx], L.w1h[n][k], L.w1l[n][k]);
     }
     for (int idx = threadIdx.x; idx < NH * NH; idx += blockDim.x) {
         const int k = idx / NH, n = idx % NH;
-        const __bf16 v = (__bf16)P[OFF_W2 + idx];
-        L.w2t[n][k] = v;
-        if (w2) w2[k][n] = v;
+        split_bf16(P[OFF_W2 + idx], L.w2h[n][k], L.w2l[n][k]);
+        if (w2) w2[k][n] = L.w2h[n][k];
     }
     for (int idx = threadIdx.x; idx < NH; idx += blockDim.x) {
         L.b1[idx] = P[OFF_B1 + idx];
@@ -235,38 +277,52 @@ LSMP_DEV void stage_weights(NetLds &L, const float *__restrict__ P, __bf16 (*w2)
     if (threadIdx.x == 0) L.b3 = P[OFF_B3];
 }
 
-// Forward pass of one wave's 16 samples.  a1: the input rows in A layout.
-// Returns z1, z2 (pre-activations, D layout) and the outputs o[i] of rows
-// 4*(lane/16)+i; sh1 is this wave's 16x64 LDS tile for the H1 relayout.
-LSMP_DEV void net_forward(const NetLds &L, const bf16x8 (&a1)[4], __bf16 (*sh1)[NH], f32x4 (&z1)[4],
-                          f32x4 (&z2)[4], float (&o)[4])
+// Forward pass of one wave's 16 samples.  xh/xl: the input rows in A layout
+// (hi and lo parts).  Returns z1, z2 (pre-activations, D layout) and the
+// outputs o[i] of rows 4*(lane/16)+i; sh/sl are this wave's 16x64 LDS tiles
+// for the H1 relayout (hi and lo parts; sh is kept for the backward pass).
+LSMP_DEV void net_forward(const NetLds &L, const bf16x8 (&xh)[4], const bf16x8 (&xl)[4], __bf16 (*sh)[NH],
+                          __bf16 (*sl)[NH], f32x4 (&z1)[4], f32x4 (&z2)[4], float (&o)[4])
 {
     const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-            acc = mma(a1[ks], *(const bf16x8 *)&L.w1t[16 * nt + lr][32 * ks + 8 * lg], acc);
+        for (int ks = 0; ks < 4; ++ks) {
+            const bf16x8 wh = *(const bf16x8 *)&L.w1h[16 * nt + lr][32 * ks + 8 * lg];
+            const bf16x8 wl = *(const bf16x8 *)&L.w1l[16 * nt + lr][32 * ks + 8 * lg];
+            acc = mma(xl[ks], wh, acc);
+            acc = mma(xh[ks], wl, acc);
+            acc = mma(xh[ks], wh, acc);
+        }
         const float b = L.b1[16 * nt + lr];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             acc[i] += b;
-            sh1[4 * lg + i][16 * nt + lr] = (__bf16)silu(acc[i]);
+            split_bf16(silu(acc[i]), sh[4 * lg + i][16 * nt + lr], sl[4 * lg + i][16 * nt + lr]);
         }
         z1[nt] = acc;
     }
     __syncthreads();
-    bf16x8 a2[2];
+    bf16x8 ah[2], al[2];
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) a2[ks] = *(const bf16x8 *)&sh1[lr][32 * ks + 8 * lg];
+    for (int ks = 0; ks < 2; ++ks) {
+        ah[ks] = *(const bf16x8 *)&sh[lr][32 * ks + 8 * lg];
+        al[ks] = *(const bf16x8 *)&sl[lr][32 * ks + 8 * lg];
+    }
     float p[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks)
-            acc = mma(a2[ks], *(const bf16x8 *)&L.w2t[16 * nt + lr][32 * ks + 8 * lg], acc);
+        for (int ks = 0; ks < 2; ++ks) {
+            const bf16x8 wh = *(const bf16x8 *)&L.w2h[16 * nt + lr][32 * ks + 8 * lg];
+            const bf16x8 wl = *(const bf16x8 *)&L.w2l[16 * nt + lr][32 * ks + 8 * lg];
+            acc = mma(al[ks], wh, acc);
+            acc = mma(ah[ks], wl, acc);
+            acc = mma(ah[ks], wh, acc);
+        }
         const float b = L.b2[16 * nt + lr], w = L.w3[16 * nt + lr];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -279,15 +335,25 @@ LSMP_DEV void net_forward(const NetLds &L, const bf16x8 (&a1)[4], __bf16 (*sh1)[
     for (int i = 0; i < 4; ++i) o[i] = group16_sum(p[i]) + L.b3;
 }
 
-// Rows of X for this wave in A layout (bf16x8 loads of 8 features).
-LSMP_DEV void load_rows(const __bf16 *__restrict__ X, long srow, bool valid, bf16x8 (&a1)[4])
+// Rows of X for this wave in A layout, split into hi and lo parts.
+LSMP_DEV void load_rows(const float *__restrict__ X, long srow, bool valid, bf16x8 (&xh)[4], bf16x8 (&xl)[4])
 {
     const int l = threadIdx.x & 63, lg = l >> 4;
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-        if (valid) a1[ks] = *(const bf16x8 *)&X[srow * NF + 32 * ks + 8 * lg];
-        else
-            for (int j = 0; j < 8; ++j) a1[ks][j] = (__bf16)0.0f;
+        float4 a = {0.f, 0.f, 0.f, 0.f}, b = {0.f, 0.f, 0.f, 0.f};
+        if (valid) {
+            a = *(const float4 *)&X[srow * NF + 32 * ks + 8 * lg];
+            b = *(const float4 *)&X[srow * NF + 32 * ks + 8 * lg + 4];
+        }
+        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            __bf16 hi, lo;
+            split_bf16(v[j], hi, lo);
+            xh[ks][j] = hi;
+            xl[ks][j] = lo;
+        }
     }
 }
 
@@ -299,17 +365,19 @@ LSMP_DEV unsigned hash32(unsigned x)
 
 // One training step's forward and backward pass over a minibatch: each
 // workgroup handles 64 samples and writes its gradient of
-// L = (1/B) sum (o - y)^2 to part[blockIdx.x][NPARAM] and its loss sum to
-// lpart[blockIdx.x].  Samples are drawn with replacement by a hash of
-// (seed, position), or taken in order when `ordered` is set.
-__global__ __launch_bounds__(256) void k_net_grad(const float *__restrict__ P, const __bf16 *__restrict__ X,
+// L = (1/B) sum w (o - y)^2, w = c^2 / (y^2 + c^2) (w = 1 for c = 0), to
+// part[blockIdx.x][NPARAM] and its loss sum to lpart[blockIdx.x].  Samples
+// are drawn with replacement by a hash of (seed, position), or taken in
+// order when `ordered` is set.
+__global__ __launch_bounds__(256) void k_net_grad(const float *__restrict__ P, const float *__restrict__ X,
                                                   const float *__restrict__ Y, unsigned nsamp, int batch,
-                                                  unsigned seed, int ordered, float *__restrict__ part,
+                                                  unsigned seed, int ordered, float wc, float *__restrict__ part,
                                                   double *__restrict__ lpart)
 {
     __shared__ NetLds L;
     __shared__ __attribute__((aligned(16))) __bf16 w2[NH][NH];        // [n1][n2]
-    __shared__ __attribute__((aligned(16))) __bf16 sh1[4][16][NH];     // per wave [row][n1]
+    __shared__ __attribute__((aligned(16))) __bf16 sh1[4][16][NH];     // per wave [row][n1], hi part
+    __shared__ __attribute__((aligned(16))) __bf16 sl1[4][16][NH];     // lo part
     __shared__ __attribute__((aligned(16))) __bf16 sdz2[4][16][NH];    // per wave [row][n2]
     __shared__ __attribute__((aligned(16))) __bf16 h1t[NH][TILE];      // [n1][sample]
     __shared__ __attribute__((aligned(16))) __bf16 dz2t[NH][TILE];     // [n2][sample]
@@ -324,15 +392,15 @@ __global__ __launch_bounds__(256) void k_net_grad(const float *__restrict__ P, c
     const bool valid = pos < batch;
     unsigned s = 0;
     if (valid) s = ordered ? (unsigned)pos : hash32(seed * 0x9e3779b9u + (unsigned)pos) % nsamp;
-    bf16x8 a1[4];
-    load_rows(X, (long)s, valid, a1);
+    bf16x8 xh[4], xl[4];
+    load_rows(X, (long)s, valid, xh, xl);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) xt[32 * ks + 8 * lg + j][wv * 16 + lr] = a1[ks][j];
+        for (int j = 0; j < 8; ++j) xt[32 * ks + 8 * lg + j][wv * 16 + lr] = xh[ks][j];
     f32x4 z1[4], z2[4];
     float o[4];
-    net_forward(L, a1, sh1[wv], z1, z2, o);
+    net_forward(L, xh, xl, sh1[wv], sl1[wv], z1, z2, o);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
@@ -344,13 +412,15 @@ __global__ __launch_bounds__(256) void k_net_grad(const float *__restrict__ P, c
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int rpos = blockIdx.x * TILE + wv * 16 + 4 * lg + i;
-        float e = 0.f;
+        float e = 0.f, w = 1.f;
         if (rpos < batch) {
             const unsigned rs = ordered ? (unsigned)rpos : hash32(seed * 0x9e3779b9u + (unsigned)rpos) % nsamp;
-            e = o[i] - Y[rs];
+            const float y = Y[rs];
+            e = o[i] - y;
+            if (wc > 0.f) w = wc * wc / (y * y + wc * wc);
         }
-        dO[i] = 2.0f * e * invB;
-        lsum += e * e;
+        dO[i] = 2.0f * w * e * invB;
+        lsum += w * e * e;
     }
     float gw3[4], gb2[4];
 #pragma unroll
@@ -472,14 +542,18 @@ __global__ void k_net_update(float *__restrict__ P, float *__restrict__ m, float
     P[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + 1.0e-8f);
 }
 
-// Inference on the band cells: phi_out(cell) = phi_c + h * net(features).
+// Inference on the band cells: phi_out = phi_c + h * net(features) * gate,
+// gate = clamp((|phi_c| - w) / w, 0, 1) with w = ls_wid: the network leaves
+// the interface region |phi| <= ls_wid to the plain sweeps and ramps in over
+// [ls_wid, 2 ls_wid].
 template <typename T>
 __global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                    const int *__restrict__ list, int nlist, Geom g,
-                                                   const float *__restrict__ P, T h)
+                                                   const float *__restrict__ P, T h, T w)
 {
     __shared__ NetLds L;
     __shared__ __attribute__((aligned(16))) __bf16 sh1[4][16][NH];
+    __shared__ __attribute__((aligned(16))) __bf16 sl1[4][16][NH];
     __shared__ float sout[4][16];
     stage_weights(L, P, nullptr);
     __syncthreads();
@@ -493,19 +567,26 @@ __global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T 
         i = idx % g.nx; j = (idx / g.nx) % g.ny; k = idx / (g.nx * g.ny);
         pc = phi[idx];
     }
-    bf16x8 a1[4];
+    bf16x8 xh[4], xl[4];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj)
-            a1[ks][jj] = (__bf16)(valid ? feature(phi, g, i, j, k, pc, 32 * ks + 8 * lg + jj) : 0.0f);
+        for (int jj = 0; jj < 8; ++jj) {
+            __bf16 hi, lo;
+            split_bf16(valid ? feature(phi, g, i, j, k, pc, 32 * ks + 8 * lg + jj) : 0.0f, hi, lo);
+            xh[ks][jj] = hi;
+            xl[ks][jj] = lo;
+        }
     f32x4 z1[4], z2[4];
     float o[4];
-    net_forward(L, a1, sh1[wv], z1, z2, o);
+    net_forward(L, xh, xl, sh1[wv], sl1[wv], z1, z2, o);
     if (lr == 0)
         for (int q = 0; q < 4; ++q) sout[wv][4 * lg + q] = o[q];
     __syncthreads();
-    if (valid && lg == 0) phi_out[idx] = pc + h * (T)sout[wv][lr];
+    if (valid && lg == 0) {
+        const T gate = fmin(fmax((fabs(pc) - w) / w, T(0)), T(1));
+        phi_out[idx] = pc + h * (T)sout[wv][lr] * gate;
+    }
 }
 
 template <typename TO, typename TI>
@@ -539,9 +620,8 @@ struct lsmp_net {
     hipStream_t st = nullptr;
     float *P = nullptr, *m = nullptr, *v = nullptr, *part = nullptr, *gsum = nullptr;
     double *lpart = nullptr, *loss = nullptr;
-    __bf16 *X = nullptr;
+    float *X = nullptr;   // training inputs [capacity][NF]
     float *Y = nullptr;
-    unsigned *count = nullptr;
     unsigned capacity = 0, nsamp = 0;
     int part_blocks = 0;
     int step = 0;
@@ -611,10 +691,11 @@ Geom geom(const lsmp_ctx *c)
     Geom g;
     g.nx = c->nx; g.ny = c->ny; g.nz = c->nz;
     g.inv_h = (float)(1.0 / c->p.dx);
+    g.inv_12h = (float)(1.0 / (12.0 * c->p.dx));
     return g;
 }
 
-template <typename T> int collect(lsmp_net *net, lsmp_ctx *c, double band_cells, int *n_added)
+template <typename T> int collect(lsmp_net *net, lsmp_ctx *c, double band_cells, int stride, int *n_added)
 {
     const size_t n = c->ncell;
     T *pre = (T *)c->R;   // the advection RHS is free between steps
@@ -626,11 +707,15 @@ template <typename T> int collect(lsmp_net *net, lsmp_ctx *c, double band_cells,
     int np;
     rc = lsmp_reinit_run(c, 1, -1.0, 0, true, &err, &np);
     if (rc) return rc;
+    // every stride-th band cell, starting at an offset that varies between calls
+    const unsigned first = nb ? net->nsamp % (unsigned)stride % nb : 0;
+    const unsigned avail = nb > first ? (nb - first + stride - 1) / stride : 0;
     const unsigned room = net->capacity - net->nsamp;
-    const unsigned take = nb < room ? nb : room;
+    const unsigned take = avail < room ? avail : room;
     if (take) {
         hipLaunchKernelGGL((k_collect<T>), dim3(cdiv(take, 256)), dim3(256), 0, c->st, pre, (const T *)c->phi,
-                           c->band_list, (int)take, geom(c), net->X, net->Y, net->nsamp, net->capacity);
+                           c->band_list, (int)take, (int)first, stride, geom(c), net->X, net->Y, net->nsamp,
+                           net->capacity);
         NET_HIP(hipGetLastError());
     }
     NET_HIP(hipStreamSynchronize(c->st));
@@ -671,7 +756,7 @@ template <typename T> int apply(lsmp_ctx *c, lsmp_net *net, double band_cells, i
     if (rc) return rc;
     if (nb) {
         hipLaunchKernelGGL((k_net_infer<T>), dim3(cdiv(nb, TILE)), dim3(256), 0, c->st, phi, out, c->band_list,
-                           (int)nb, geom(c), net->P, (T)c->p.dx);
+                           (int)nb, geom(c), net->P, (T)c->p.dx, (T)c->p.ls_wid);
         NET_HIP(hipGetLastError());
     }
     c->phiB = c->phi;
@@ -681,16 +766,31 @@ template <typename T> int apply(lsmp_ctx *c, lsmp_net *net, double band_cells, i
     return LSMP_OK;
 }
 
+// Pseudo-steps the plain reinitialisation runs without an early exit: the
+// loop of level_set_rk3_redis advances time by rdt and stops once time > timestop.
+int plain_budget(const lsmp_ctx *c)
+{
+    const double timestop = 2.0 * c->p.ls_wid;
+    double t = 0.0;
+    int k = 0;
+    do {
+        ++k;
+        t += c->p.rdt;
+    } while (!(t > timestop) && k < 100000);
+    return k;
+}
+
 template <typename T>
-int reinit_learned(lsmp_ctx *c, lsmp_net *net, double band_cells, int sweeps, double tol, double *res_out,
-                   int *fallback, int *n_pseudo)
+int reinit_learned(lsmp_ctx *c, lsmp_net *net, double band_cells, int sweeps, double tol, double res_band, int mode,
+                   double *res_out, int *fallback, int *n_pseudo)
 {
     const size_t n = c->ncell;
     T *phi = (T *)c->phi, *backup = (T *)c->R;
     // the sign of the field before the correction, as level_set_redis_init sets it
     launch_sign_init<T>(c->st, phi, (T *)c->S0, n, (T)c->p.ls_wid, 0);
     NET_HIP(hipGetLastError());
-    NET_HIP(hipMemcpyAsync(backup, phi, n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+    if (mode == LSMP_FALLBACK_RESTART)
+        NET_HIP(hipMemcpyAsync(backup, phi, n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
     int rc = apply<T>(c, net, band_cells, nullptr);
     if (rc) return rc;
     double err = 0.0;
@@ -698,19 +798,35 @@ int reinit_learned(lsmp_ctx *c, lsmp_net *net, double band_cells, int sweeps, do
     rc = lsmp_reinit_run(c, 1, -1.0, sweeps, false, &err, &np);
     if (rc) return rc;
     double res = 0.0;
-    rc = residual<T>(c, 3.0, &res);
+    rc = residual<T>(c, res_band, &res);
     if (rc) return rc;
     *fallback = 0;
     if (!(res <= tol)) {
-        // the network's result is rejected: the plain method from the input field
-        NET_HIP(hipMemcpyAsync(c->phi, backup, n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
-        int np2 = 0;
-        rc = lsmp_reinit_run(c, 1, -1.0, 0, true, &err, &np2);
-        if (rc) return rc;
-        np += np2;
-        rc = residual<T>(c, 3.0, &res);
-        if (rc) return rc;
         *fallback = 1;
+        if (mode == LSMP_FALLBACK_RESTART) {
+            // the network's result is rejected: the plain method from the input field
+            NET_HIP(hipMemcpyAsync(c->phi, backup, n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+            int np2 = 0;
+            rc = lsmp_reinit_run(c, 1, -1.0, 0, true, &err, &np2);
+            if (rc) return rc;
+            np += np2;
+            rc = residual<T>(c, res_band, &res);
+            if (rc) return rc;
+        } else {
+            // more plain pseudo-steps from the current field, `sweeps` at a time,
+            // until the residual passes, the plain method's own stop (max change
+            // <= 1e-6) is met, or its pseudo-time budget is spent
+            int budget = plain_budget(c) - np;
+            while (budget > 0 && !(res <= tol) && err > 1.0e-6) {
+                int np2 = 0;
+                rc = lsmp_reinit_run(c, 1, -1.0, sweeps < budget ? sweeps : budget, false, &err, &np2);
+                if (rc) return rc;
+                np += np2;
+                budget -= np2;
+                rc = residual<T>(c, res_band, &res);
+                if (rc) return rc;
+            }
+        }
     }
     *res_out = res;
     *n_pseudo = np;
@@ -739,7 +855,7 @@ int lsmp_net_create(int capacity, unsigned seed, lsmp_net_handle *out)
         return fail(LSMP_E_HIP);
     net->capacity = (unsigned)capacity;
     if (capacity > 0) {
-        if (hipMalloc(&net->X, (size_t)capacity * NF * sizeof(__bf16)) != hipSuccess ||
+        if (hipMalloc(&net->X, (size_t)capacity * NF * sizeof(float)) != hipSuccess ||
             hipMalloc(&net->Y, (size_t)capacity * sizeof(float)) != hipSuccess)
             return fail(LSMP_E_HIP);
     }
@@ -794,22 +910,23 @@ int lsmp_net_dataset_size(lsmp_net_handle net, int *n)
     return LSMP_OK;
 }
 
-int lsmp_net_collect(lsmp_net_handle net, lsmp_handle h, double band_cells, int *n_added)
+int lsmp_net_collect(lsmp_net_handle net, lsmp_handle h, double band_cells, int stride, int *n_added)
 {
-    if (!net || !h || !(band_cells > 0) || net->capacity == 0) return LSMP_E_INVALID;
-    return h->fp32 ? collect<float>(net, h, band_cells, n_added) : collect<double>(net, h, band_cells, n_added);
+    if (!net || !h || !(band_cells > 0) || stride < 1 || net->capacity == 0) return LSMP_E_INVALID;
+    return h->fp32 ? collect<float>(net, h, band_cells, stride, n_added)
+                   : collect<double>(net, h, band_cells, stride, n_added);
 }
 
-int lsmp_net_train(lsmp_net_handle net, int steps, int batch, double lr, unsigned seed, double *loss)
+int lsmp_net_train(lsmp_net_handle net, int steps, int batch, double lr, unsigned seed, double weight_c, double *loss)
 {
-    if (!net || steps < 0 || batch <= 0 || net->nsamp == 0) return LSMP_E_INVALID;
+    if (!net || steps < 0 || batch <= 0 || net->nsamp == 0 || !(weight_c >= 0)) return LSMP_E_INVALID;
     const int nblk = (int)cdiv((size_t)batch, TILE);
     int rc = ensure_parts(net, nblk);
     if (rc) return rc;
     for (int s = 0; s < steps; ++s) {
         net->step += 1;
         hipLaunchKernelGGL(k_net_grad, dim3(nblk), dim3(256), 0, net->st, net->P, net->X, net->Y, net->nsamp, batch,
-                           seed * 2654435761u + (unsigned)net->step, 0, net->part, net->lpart);
+                           seed * 2654435761u + (unsigned)net->step, 0, (float)weight_c, net->part, net->lpart);
         const float bc1 = 1.0f - powf(0.9f, (float)net->step), bc2 = 1.0f - powf(0.999f, (float)net->step);
         hipLaunchKernelGGL(k_net_update, dim3(cdiv(NPARAM, 256)), dim3(256), 0, net->st, net->P, net->m, net->v,
                            net->part, nblk, (float)lr, bc1, bc2, 1, (float *)nullptr, net->lpart, net->loss);
@@ -824,23 +941,22 @@ int lsmp_net_train(lsmp_net_handle net, int steps, int batch, double lr, unsigne
 int lsmp_net_eval_grad(lsmp_net_handle net, const float *x, const float *y, int n, double *loss, float *grad)
 {
     if (!net || !x || !y || n <= 0) return LSMP_E_INVALID;
-    std::vector<__bf16> xb((size_t)n * NF);
-    for (size_t q = 0; q < xb.size(); ++q) xb[q] = (__bf16)x[q];
-    __bf16 *dX = nullptr;
+    const size_t nx = (size_t)n * NF;
+    float *dX = nullptr;
     float *dY = nullptr;
-    NET_HIP(hipMalloc(&dX, xb.size() * sizeof(__bf16)));
+    NET_HIP(hipMalloc(&dX, nx * sizeof(float)));
     if (hipMalloc(&dY, (size_t)n * sizeof(float)) != hipSuccess) {
         (void)hipFree(dX);
         return LSMP_E_HIP;
     }
     const int nblk = (int)cdiv((size_t)n, TILE);
     int rc = ensure_parts(net, nblk);
-    if (!rc && (hipMemcpyAsync(dX, xb.data(), xb.size() * sizeof(__bf16), hipMemcpyHostToDevice, net->st) != hipSuccess ||
+    if (!rc && (hipMemcpyAsync(dX, x, nx * sizeof(float), hipMemcpyHostToDevice, net->st) != hipSuccess ||
                 hipMemcpyAsync(dY, y, (size_t)n * sizeof(float), hipMemcpyHostToDevice, net->st) != hipSuccess))
         rc = LSMP_E_HIP;
     if (!rc) {
         hipLaunchKernelGGL(k_net_grad, dim3(nblk), dim3(256), 0, net->st, net->P, dX, dY, (unsigned)n, n, 0u, 1,
-                           net->part, net->lpart);
+                           0.0f, net->part, net->lpart);
         hipLaunchKernelGGL(k_net_update, dim3(cdiv(NPARAM, 256)), dim3(256), 0, net->st, net->P, net->m, net->v,
                            net->part, nblk, 0.0f, 1.0f, 1.0f, 0, net->gsum, net->lpart, net->loss);
         if (hipGetLastError() != hipSuccess ||
@@ -857,14 +973,17 @@ int lsmp_net_eval_grad(lsmp_net_handle net, const float *x, const float *y, int 
 }
 
 int lsmp_reinit_learned(lsmp_handle h, lsmp_net_handle net, double band_cells, int sweeps, double residual_tol,
-                        double *residual_out, int *fallback, int *n_pseudo)
+                        double residual_band, int fallback_mode, double *residual_out, int *fallback, int *n_pseudo)
 {
-    if (!h || !net || !(band_cells > 0) || sweeps < 1) return LSMP_E_INVALID;
+    if (!h || !net || !(band_cells > 0) || sweeps < 1 || !(residual_band > 0)) return LSMP_E_INVALID;
+    if (fallback_mode != LSMP_FALLBACK_RESTART && fallback_mode != LSMP_FALLBACK_CONTINUE) return LSMP_E_INVALID;
     if (h->dim == 3 && (h->p.dy != h->p.dx || h->p.dz != h->p.dx)) return LSMP_E_INVALID;
     double r = 0.0;
     int fb = 0, np = 0;
-    int rc = h->fp32 ? reinit_learned<float>(h, net, band_cells, sweeps, residual_tol, &r, &fb, &np)
-                     : reinit_learned<double>(h, net, band_cells, sweeps, residual_tol, &r, &fb, &np);
+    int rc = h->fp32 ? reinit_learned<float>(h, net, band_cells, sweeps, residual_tol, residual_band, fallback_mode,
+                                             &r, &fb, &np)
+                     : reinit_learned<double>(h, net, band_cells, sweeps, residual_tol, residual_band, fallback_mode,
+                                              &r, &fb, &np);
     if (residual_out) *residual_out = r;
     if (fallback) *fallback = fb;
     if (n_pseudo) *n_pseudo = np;
@@ -876,9 +995,7 @@ int lsmp_net_add_samples(lsmp_net_handle net, const float *x, const float *y, in
 {
     if (!net || !x || !y || n < 0 || (unsigned)n > net->capacity - net->nsamp) return LSMP_E_INVALID;
     if (n == 0) return LSMP_OK;
-    std::vector<__bf16> xb((size_t)n * NF);
-    for (size_t q = 0; q < xb.size(); ++q) xb[q] = (__bf16)x[q];
-    NET_HIP(hipMemcpyAsync(net->X + (size_t)net->nsamp * NF, xb.data(), xb.size() * sizeof(__bf16),
+    NET_HIP(hipMemcpyAsync(net->X + (size_t)net->nsamp * NF, x, (size_t)n * NF * sizeof(float),
                            hipMemcpyHostToDevice, net->st));
     NET_HIP(hipMemcpyAsync(net->Y + net->nsamp, y, (size_t)n * sizeof(float), hipMemcpyHostToDevice, net->st));
     NET_HIP(hipStreamSynchronize(net->st));

@@ -6,10 +6,11 @@ Pairs come from the plain method on random seeded shapes and flows:
   centres uniform in [0.25, 0.75]^3, random orientation, phi > 0 inside;
 * flows: divergence-free random Fourier fields u = curl A with integer
   wavenumber vectors of max-norm 1..4, scaled to max |u| = 1 on the grid;
-* each shape is reinitialised by the plain method, then advected a random
-  number of RK3 steps without reinitialisation (CFL 0.1 of the unit speed),
-  and the next plain reinitialisation gives one (pre, post) pair per band
-  cell; this repeats along the trajectory.
+* each shape is reinitialised by the plain method and then follows the MPLS
+  loop under its flow: a few RK3 advection steps (CFL 0.1 of the unit
+  speed; one step with probability p_one, else 2..max_adv), the plain
+  reinitialisation -- whose input and output give the (pre, post) pairs of
+  every stride-th band cell -- and the mass correction.
 
 Everything runs on the GPU through the library (advection, plain
 reinitialisation, pair collection, BF16 forward/backward and Adam kernels);
@@ -81,9 +82,16 @@ def fourier_velocity(n, rng, kmax=4, modes=6):
     return [np.ascontiguousarray(u[d]) for d in range(3)]
 
 
-def generate(net, sizes=(96, 128), shapes=24, chain=4, max_adv=8, band_cells=6.0, seed=0,
-             precision=lsmp.FP32, log=print):
-    """Fill the net's training set; returns the number of pairs."""
+def generate(net, sizes=(96, 128), shapes=24, chain=4, max_adv=8, p_one=0.0, band_cells=6.0, seed=0,
+             precision=lsmp.FP32, stride=1, every=1, log=print):
+    """Fill the net's training set; returns the number of pairs.
+
+    Each shape runs `chain` rounds of: `every` MPLS steps (1 to max_adv RK3
+    advection steps, 1 with probability p_one, then the plain
+    reinitialisation and the mass correction), the pairs being collected
+    from the last reinitialisation of the round.  A large `every` lets the
+    flow stretch the shape into sheets and filaments before most pairs are
+    taken."""
     rng = np.random.default_rng(seed)
     total = 0
     for si in range(shapes):
@@ -92,11 +100,19 @@ def generate(net, sizes=(96, 128), shapes=24, chain=4, max_adv=8, band_cells=6.0
             s.upload_phi(ellipsoid_union(n, rng))
             s.reinit(1)
             s.reinit(1)
+            imass, _ = s.mass_vol()
             s.set_velocity(*fourier_velocity(n, rng))
             for _ in range(chain):
-                for _ in range(int(rng.integers(1, max_adv + 1))):
-                    s.rk3_advect()
-                got = net.collect(s, band_cells)
+                got = 0
+                for e in range(every):
+                    k = 1 if rng.random() < p_one else int(rng.integers(1, max_adv + 1))
+                    for _ in range(k):
+                        s.rk3_advect()
+                    if e == every - 1:
+                        got = net.collect(s, band_cells, stride)
+                    else:
+                        s.reinit(1)
+                    s.mass_fix(imass)
                 total += got
                 if got == 0:
                     break
@@ -104,7 +120,7 @@ def generate(net, sizes=(96, 128), shapes=24, chain=4, max_adv=8, band_cells=6.0
     return total
 
 
-def train(net, steps=6000, batch=16384, lr=2e-3, lr_end=2e-5, chunk=200, seed=0, log=print):
+def train(net, steps=6000, batch=16384, lr=2e-3, lr_end=2e-5, chunk=200, seed=0, weight_c=0.0, log=print):
     """Adam with a geometric learning-rate decay; returns the loss history."""
     hist = []
     done = 0
@@ -112,7 +128,7 @@ def train(net, steps=6000, batch=16384, lr=2e-3, lr_end=2e-5, chunk=200, seed=0,
         k = min(chunk, steps - done)
         frac = done / max(steps - 1, 1)
         cur = lr * (lr_end / lr) ** frac
-        loss = net.train(k, batch=batch, lr=cur, seed=seed)
+        loss = net.train(k, batch=batch, lr=cur, seed=seed, weight_c=weight_c)
         done += k
         hist.append((done, cur, loss))
         log(f"  step {done}: lr {cur:.2e} loss {loss:.4e}")
@@ -129,27 +145,34 @@ def load_net(path=WEIGHTS, capacity=0):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", default=WEIGHTS)
-    ap.add_argument("--capacity", type=int, default=3_000_000)
+    ap.add_argument("--capacity", type=int, default=16_000_000)
     ap.add_argument("--shapes", type=int, default=24)
-    ap.add_argument("--chain", type=int, default=4)
+    ap.add_argument("--chain", type=int, default=64)
     ap.add_argument("--sizes", default="96,128")
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--stride", type=int, default=32)
+    ap.add_argument("--max-adv", type=int, default=1)
+    ap.add_argument("--p-one", type=float, default=1.0)
+    ap.add_argument("--every", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=30000)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--weight-c", type=float, default=0.003)
     args = ap.parse_args(argv)
     t0 = time.time()
     net = lsmp.Net(capacity=args.capacity, seed=args.seed + 1)
     sizes = [int(v) for v in args.sizes.split(",")]
-    n = generate(net, sizes=sizes, shapes=args.shapes, chain=args.chain, seed=args.seed)
+    n = generate(net, sizes=sizes, shapes=args.shapes, chain=args.chain, seed=args.seed, stride=args.stride,
+                 max_adv=args.max_adv, p_one=args.p_one, every=args.every)
     print(f"pairs {n} ({time.time() - t0:.1f} s)", flush=True)
     t1 = time.time()
-    hist = train(net, steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed)
+    hist = train(net, steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed, weight_c=args.weight_c)
     print(f"trained {args.steps} steps in {time.time() - t1:.1f} s", flush=True)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     net.save(args.out)
     meta = dict(pairs=n, steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed, sizes=sizes,
-                shapes=args.shapes, chain=args.chain, final_loss=hist[-1][2])
+                shapes=args.shapes, chain=args.chain, stride=args.stride, max_adv=args.max_adv,
+                p_one=args.p_one, every=args.every, weight_c=args.weight_c, final_loss=hist[-1][2])
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
         json.dump(meta, f, indent=1)
     print(json.dumps(meta), flush=True)

@@ -1,0 +1,98 @@
+"""The 3D LeVeque deformation run to T (default 3) on the GPU, in any form:
+FP64 plain, FP32 bulk plain, or FP32 bulk with the learned reinitialisation.
+
+Case: sphere r = 0.15 at (0.35, 0.35, 0.35) in the unit cube, velocity of
+Vortex/src/setup/velocity.f90 with t2s = T, dt = rdt = 0.1 dx, ls_wid =
+1.5 dx, MPLS every step (Vortex/src/main.f90 loop with t2red = 1).  Prints
+one JSON line: ms/step, cell-updates/s, L1 shape error (output.f90 metric),
+|dM|/M, pseudo-steps per step and fallbacks of the learned form.
+
+    python mass-preserving-level-set-method_amd/leveque.py --n 256 --form fp32-learned
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+import lsmp  # noqa: E402
+
+FORMS = ("fp64-plain", "fp32-plain", "fp32-learned", "fp64-learned")
+
+
+def sphere_box(n):
+    """phi0 = 0.15 - |x - (0.35, 0.35, 0.35)| with the reference's single-precision
+    literals (Vortex/src/setup/problem_init.f90:69), zero-gradient ghosts."""
+    xb = (np.arange(-3, n + 3) + 0.5) / n
+    c, r = float(np.float32(0.35)), float(np.float32(0.15))
+    z, y, x = np.meshgrid(xb, xb, xb, indexing="ij", sparse=True)
+    box = -np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) + r
+    idx = np.clip(np.arange(n + 6) - 3, 0, n - 1) + 3
+    return np.ascontiguousarray(box[np.ix_(idx, idx, idx)])
+
+
+def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsmp.FALLBACK_CONTINUE,
+        log=None):
+    prec = lsmp.FP64 if form.startswith("fp64") else lsmp.FP32
+    learned = form.endswith("learned")
+    net = None
+    if learned:
+        net = lsmp.Net()
+        net.load(weights or os.path.join(HERE, "weights", "reinit_mlp.npy"))
+    s = lsmp.Solver(lsmp.make_params(n, dim=3, precision=prec))
+    s.upload_phi(sphere_box(n))
+    s.store_phi0()
+    imass, _ = s.mass_vol()
+    dt = s.params.dt
+    nsteps = int(round(T / dt * frac))
+    t = 0.0
+    pseudo = fallbacks = 0
+    s.synchronize()
+    t0 = time.perf_counter()
+    for it in range(nsteps):
+        t += dt
+        s.set_velocity_analytic(lsmp.VEL_LEVEQUE3D, t, T)
+        s.rk3_advect()
+        if learned:
+            _, fb, k = s.reinit_learned(net, 6.0, sweeps, tol, 1.5, fallback)
+            fallbacks += fb
+        else:
+            _, k = s.reinit(1)
+        pseudo += k
+        s.mass_fix(imass)
+        if log and (it + 1) % max(1, nsteps // 10) == 0:
+            log(f"  {form} {n}^3 step {it + 1}/{nsteps} t={t:.3f}")
+    s.synchronize()
+    el = time.perf_counter() - t0
+    mass, _ = s.mass_vol()
+    out = dict(form=form, n=n, T=T, t_end=t, steps=nsteps, ms_per_step=el / nsteps * 1e3,
+               cell_updates_per_s=n ** 3 * nsteps / el, l1=s.interface_l1(), dM_over_M=abs(mass - imass) / imass,
+               pseudo_per_step=pseudo / nsteps, fallbacks=fallbacks)
+    s.close()
+    if net is not None:
+        net.close()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--form", choices=FORMS, default="fp32-learned")
+    ap.add_argument("--T", type=float, default=3.0)
+    ap.add_argument("--frac", type=float, default=1.0, help="run this fraction of T")
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--tol", type=float, default=1e-2)
+    args = ap.parse_args(argv)
+    r = run(args.n, args.form, T=args.T, frac=args.frac, weights=args.weights, tol=args.tol,
+            log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps(r), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "liblsmp.so")
 
 FP64, FP32 = 0, 1
+FALLBACK_RESTART, FALLBACK_CONTINUE = 0, 1
 VEL_ROTATION, VEL_VORTEX2D, VEL_LEVEQUE3D = 0, 1, 2
 
 ERRORS = {0: "ok", -1: "invalid parameter or size", -2: "HIP runtime error",
@@ -98,10 +99,10 @@ def load_library(path=LIB_PATH):
     L.lsmp_net_set_params.argtypes = [H, _FP, C.c_int]
     L.lsmp_net_get_params.argtypes = [H, _FP, C.c_int]
     L.lsmp_net_dataset_size.argtypes = [H, _IP]
-    L.lsmp_net_collect.argtypes = [H, H, C.c_double, _IP]
-    L.lsmp_net_train.argtypes = [H, C.c_int, C.c_int, C.c_double, C.c_uint, _DP]
+    L.lsmp_net_collect.argtypes = [H, H, C.c_double, C.c_int, _IP]
+    L.lsmp_net_train.argtypes = [H, C.c_int, C.c_int, C.c_double, C.c_uint, C.c_double, _DP]
     L.lsmp_net_eval_grad.argtypes = [H, _FP, _FP, C.c_int, _DP, _FP]
-    L.lsmp_reinit_learned.argtypes = [H, H, C.c_double, C.c_int, C.c_double, _DP, _IP, _IP]
+    L.lsmp_reinit_learned.argtypes = [H, H, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _DP, _IP, _IP]
     L.lsmp_reinit_residual.argtypes = [H, C.c_double, _DP]
     L.lsmp_net_add_samples.argtypes = [H, _FP, _FP, C.c_int]
     L.lsmp_net_clear_samples.argtypes = [H]
@@ -243,15 +244,17 @@ class Solver:
     def synchronize(self):
         _check(self.lib.lsmp_synchronize(self.h))
 
-    def reinit_learned(self, net, band_cells=6.0, sweeps=2, residual_tol=1e-2):
+    def reinit_learned(self, net, band_cells=6.0, sweeps=2, residual_tol=1e-2, residual_band=1.5,
+                       fallback=FALLBACK_CONTINUE):
         """Learned reinitialisation (no reference counterpart): network on the
-        band, `sweeps` plain pseudo-steps, plain fallback above residual_tol.
-        Returns (residual, fallback, n_pseudo)."""
+        band, `sweeps` plain pseudo-steps, plain fallback (restart from the
+        input, or continuation) when the residual over |phi| < residual_band
+        dx exceeds residual_tol.  Returns (residual, fallback_taken, n_pseudo)."""
         r = C.c_double()
         fb = C.c_int()
         n = C.c_int()
-        _check(self.lib.lsmp_reinit_learned(self.h, net.h, band_cells, sweeps, residual_tol, C.byref(r),
-                                            C.byref(fb), C.byref(n)))
+        _check(self.lib.lsmp_reinit_learned(self.h, net.h, band_cells, sweeps, residual_tol, residual_band,
+                                            int(fallback), C.byref(r), C.byref(fb), C.byref(n)))
         return r.value, bool(fb.value), n.value
 
     def reinit_residual(self, band_cells=3.0):
@@ -260,7 +263,8 @@ class Solver:
         _check(self.lib.lsmp_reinit_residual(self.h, band_cells, C.byref(r)))
         return r.value
 
-    def mpls_step(self, t, t2s, field, imass, t2red=1, iteration=1, net=None, sweeps=2, residual_tol=1e-2):
+    def mpls_step(self, t, t2s, field, imass, t2red=1, iteration=1, net=None, sweeps=2, residual_tol=1e-2,
+                  fallback=FALLBACK_CONTINUE):
         """One MPLS step as the merged branch of mpls runs it
         (interface_solver.f90:123-129) after the driver set the velocity at
         t^{n+1} (main.f90:23-29).  With a trained `net` the reinitialisation
@@ -272,7 +276,8 @@ class Solver:
             if net is None:
                 _, npseudo = self.reinit(1)
             else:
-                _, _, npseudo = self.reinit_learned(net, sweeps=sweeps, residual_tol=residual_tol)
+                _, _, npseudo = self.reinit_learned(net, sweeps=sweeps, residual_tol=residual_tol,
+                                                    fallback=fallback)
         _, nm = self.mass_fix(imass)
         return npseudo, nm
 
@@ -336,10 +341,10 @@ class Net:
         _check(self.lib.lsmp_net_dataset_size(self.h, C.byref(n)))
         return n.value
 
-    def collect(self, solver, band_cells=6.0):
-        """Plain-reinitialise the solver's phi and store the band pairs."""
+    def collect(self, solver, band_cells=6.0, stride=1):
+        """Plain-reinitialise the solver's phi and store every stride-th band pair."""
         n = C.c_int()
-        _check(self.lib.lsmp_net_collect(self.h, solver.h, band_cells, C.byref(n)))
+        _check(self.lib.lsmp_net_collect(self.h, solver.h, band_cells, int(stride), C.byref(n)))
         return n.value
 
     def add_samples(self, x, y):
@@ -360,9 +365,11 @@ class Net:
         _check(self.lib.lsmp_net_apply(self.h, solver.h, band_cells, C.byref(n)))
         return n.value
 
-    def train(self, steps, batch=4096, lr=1e-3, seed=0):
+    def train(self, steps, batch=4096, lr=1e-3, seed=0, weight_c=0.0):
+        """Adam steps; weight_c > 0 weights each pair by c^2 / (y^2 + c^2)."""
         loss = C.c_double()
-        _check(self.lib.lsmp_net_train(self.h, int(steps), int(batch), float(lr), int(seed), C.byref(loss)))
+        _check(self.lib.lsmp_net_train(self.h, int(steps), int(batch), float(lr), int(seed), float(weight_c),
+                                       C.byref(loss)))
         return loss.value
 
     def eval_grad(self, x, y):

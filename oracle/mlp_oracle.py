@@ -5,12 +5,15 @@ HIP kernels compute so the tests can check them:
 
 * features  -- the 5x5x5 stencil (phi(nb) - phi(c)) / h, x offset fastest,
   indices clamped to the interior (the zero-gradient ghost rule of
-  branches.f90:275-320), then phi(c) / h, then two zeros;
-* forward   -- z1 = bf16(x) bf16(W1) + b1, h1 = bf16(silu(z1)),
-  z2 = h1 bf16(W2) + b2, o = silu(z2) . w3 + b3 (products of BF16 values
-  accumulated in FP32);
-* backward  -- the gradient of L = mean (o - y)^2 with the same BF16
-  roundings the kernels apply (dZ2 and dZ1 rounded before their products);
+  branches.f90:275-320), then phi(c) / h, then |grad phi| - 1 from
+  fourth-order central differences, then (phi(c) / h)(1 / max(|grad phi|,
+  0.1) - 1);
+* forward   -- split-BF16 products: each FP32 operand v is hi = bf16(v) plus
+  lo = bf16(v - hi) and a product is hi*hi + lo*hi + hi*lo accumulated in
+  FP32; z1 = x W1 + b1, h1 = silu(z1), z2 = h1 W2 + b2, o = silu(z2) . w3 + b3;
+* backward  -- the gradient of L = mean (o - y)^2 with the roundings the
+  kernels apply: the weight gradients multiply the hi parts of x and h1 by
+  dZ rounded to BF16, and dH1 = bf16(dZ2) hi(W2)^T;
 * adam      -- beta 0.9 / 0.999, eps 1e-8, bias-corrected;
 * hash32 / minibatch -- the sample draw of a training step;
 * band      -- ordered |phi| < thr cell list.
@@ -57,15 +60,32 @@ def dsilu(z):
     return s * (1 + z * (1 - s))
 
 
+def split(v):
+    """(hi, lo) BF16 parts of float32 values, as float64 arrays."""
+    v = np.asarray(v, dtype=np.float32)
+    hi = bf16(v)
+    lo = bf16(v - hi)
+    return hi.astype(np.float64), lo.astype(np.float64)
+
+
+def split_matmul(a, b):
+    """hi(a) hi(b) + lo(a) hi(b) + hi(a) lo(b), summed exactly (float64)."""
+    ah, al = split(a)
+    bh, bl = split(b)
+    return ah @ bh + al @ bh + ah @ bl
+
+
 def forward(P, X, emulate=True):
     """Network output o (n,) and the cache for backward."""
     W1, b1, W2, b2, w3, b3 = unpack(P)
     if emulate:
         f32 = np.float32
-        Xb, W1b, W2b = bf16(X), bf16(W1), bf16(W2)
-        z1 = (Xb.astype(np.float64) @ W1b.astype(np.float64)).astype(f32) + b1.astype(f32)
-        h1 = bf16(silu(z1).astype(f32))
-        z2 = (h1.astype(np.float64) @ W2b.astype(np.float64)).astype(f32) + b2.astype(f32)
+        Xb = bf16(np.asarray(X, f32))
+        W2b = bf16(np.asarray(W2, f32))
+        z1 = split_matmul(X, W1).astype(f32) + b1.astype(f32)
+        h1f = silu(z1).astype(f32)
+        h1 = bf16(h1f)
+        z2 = split_matmul(h1f, W2).astype(f32) + b2.astype(f32)
         h2 = silu(z2).astype(f32)
         o = (h2.astype(np.float64) @ w3.astype(np.float64)).astype(f32) + f32(b3)
     else:
@@ -78,13 +98,16 @@ def forward(P, X, emulate=True):
     return o, (Xb, W2b, z1, h1, z2, h2, w3)
 
 
-def loss_grad(P, X, Y, emulate=True):
-    """L = mean (o - y)^2 and dL/dP (flat, float64)."""
+def loss_grad(P, X, Y, emulate=True, weight_c=0.0):
+    """L = mean w (o - y)^2, w = c^2 / (y^2 + c^2) (1 for c = 0), and dL/dP
+    (flat, float64)."""
     n = X.shape[0]
     o, (Xb, W2b, z1, h1, z2, h2, w3) = forward(P, X, emulate)
-    e = o.astype(np.float64) - np.asarray(Y, np.float64)
-    L = float(np.mean(e * e))
-    dO = 2.0 * e / n
+    y = np.asarray(Y, np.float64)
+    e = o.astype(np.float64) - y
+    w = weight_c ** 2 / (y * y + weight_c ** 2) if weight_c > 0 else np.ones_like(y)
+    L = float(np.mean(w * e * e))
+    dO = 2.0 * w * e / n
     g = np.zeros(NPARAM)
     g[OFF["w3"]:OFF["b3"]] = h2.astype(np.float64).T @ dO
     g[OFF["b3"]] = dO.sum()
@@ -140,34 +163,51 @@ def band(phi, thr):
     return np.flatnonzero(np.abs(phi.ravel()) < thr)
 
 
-def features(phi, cells, inv_h):
-    """Rows of the network input for the flat cell indices (phi: (nz, ny, nx))."""
+def features(phi, cells, inv_h, inv_12h=None):
+    """Rows of the network input for the flat cell indices (phi: (nz, ny, nx)).
+    inv_h = float32(1/h); inv_12h = float32(1/(12 h)) (default: from inv_h)."""
     nz, ny, nx = phi.shape
+    if inv_12h is None:
+        inv_12h = np.float32(1.0 / (12.0 / float(inv_h)))
     k, r = np.divmod(cells, nx * ny)
     j, i = np.divmod(r, nx)
     X = np.zeros((cells.size, NF), dtype=np.float32)
     pc = phi.ravel()[cells]
+
+    def at(di, dj, dk):
+        return phi[np.clip(k + dk, 0, nz - 1), np.clip(j + dj, 0, ny - 1), np.clip(i + di, 0, nx - 1)]
+
     q = 0
     for dk in range(-2, 3):
         for dj in range(-2, 3):
             for di in range(-2, 3):
-                a = np.clip(i + di, 0, nx - 1)
-                b = np.clip(j + dj, 0, ny - 1)
-                c = np.clip(k + dk, 0, nz - 1)
-                d = phi[c, b, a] - pc
-                X[:, q] = d.astype(np.float32) * np.float32(inv_h)
+                X[:, q] = (at(di, dj, dk) - pc).astype(np.float32) * np.float32(inv_h)
                 q += 1
     X[:, 125] = pc.astype(np.float32) * np.float32(inv_h)
+    g2 = np.zeros(cells.size, dtype=np.float32)
+    for e in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+        m2, m1 = at(*(-2 * np.array(e))), at(*(-np.array(e)))
+        p1, p2 = at(*e), at(*(2 * np.array(e)))
+        d = ((m2 - p2) + phi.dtype.type(8) * (p1 - m1)).astype(np.float32) * np.float32(inv_12h)
+        g2 += d * d
+    gm = np.sqrt(g2)
+    X[:, 126] = gm - np.float32(1)
+    X[:, 127] = X[:, 125] * (np.float32(1) / np.maximum(gm, np.float32(0.1)) - np.float32(1))
     return X
 
 
-def apply(P, phi, h, band_cells):
+def apply(P, phi, h, band_cells, ls_wid):
     """phi after the network pass on |phi| < band_cells h (h: the FP64 spacing;
-    the threshold and h are rounded to phi's type, 1/h to FP32)."""
+    the threshold and h are rounded to phi's type, 1/h to FP32): the
+    correction h o is scaled by clamp((|phi| - ls_wid) / ls_wid, 0, 1)."""
     h = float(h)
     cells = band(phi, np.asarray(band_cells * h, dtype=phi.dtype))
-    X = features(phi, cells, np.float32(1.0 / h))
+    X = features(phi, cells, np.float32(1.0 / h), np.float32(1.0 / (12.0 * h)))
     o, _ = forward(P, X)
+    t = phi.dtype.type
+    pc = phi.ravel()[cells]
+    w = t(ls_wid)
+    gate = np.minimum(np.maximum((np.abs(pc) - w) / w, t(0)), t(1))
     out = phi.copy().ravel()
-    out[cells] = phi.ravel()[cells] + np.asarray(h, phi.dtype) * o.astype(phi.dtype)
+    out[cells] = pc + t(h) * o.astype(phi.dtype) * gate
     return out.reshape(phi.shape)

@@ -2,10 +2,11 @@
 (oracle/mlp_oracle.py) and against the plain reinitialisation.
 
 Tolerances: the kernels multiply BF16 values exactly and accumulate in FP32
-in an order the oracle does not copy, so sums agree to FP32 rounding; a BF16
-rounding that lands on the other side of a tie then moves one term by one
-BF16 ulp (2^-8 relative).  Gradients: norm-wise 2e-3 relative, per entry
-1e-3 of the largest entry.  Network outputs: 1e-3 h per cell.
+in an order the oracle does not copy, so sums agree to FP32 rounding; in the
+backward pass a BF16 rounding that lands on the other side of a tie then
+moves one term by one BF16 ulp (2^-8 relative).  Gradients: norm-wise 2e-3
+relative, per entry 1e-3 of the largest entry.  The forward pass is split
+BF16 (about 16 significant bits): network outputs within 1e-5 h per cell.
 """
 import os
 import sys
@@ -96,8 +97,9 @@ def test_forward_backward_match_oracle(n):
 
 
 def test_train_steps_match_oracle_adam():
-    """Two training steps (hashed minibatches, Adam with bias correction)
-    against the oracle's minibatch draw, backward pass and Adam step."""
+    """Two training steps (hashed minibatches, Adam with bias correction, the
+    second with the relative weighting) against the oracle's minibatch draw,
+    backward pass and Adam step."""
     lsmp = load_lsmp()
     rng = np.random.default_rng(11)
     nsamp, batch, lr, seed = 300, 128, 1e-3, 5
@@ -112,15 +114,14 @@ def test_train_steps_match_oracle_adam():
         assert net.dataset_size() == nsamp
         loss1 = net.train(1, batch=batch, lr=lr, seed=seed)
         P1 = net.get_params()
-        loss2 = net.train(1, batch=batch, lr=lr, seed=seed)
+        loss2 = net.train(1, batch=batch, lr=lr, seed=seed, weight_c=0.1)
         P2 = net.get_params()
-    Xb = M.bf16(X)
     m = np.zeros(M.NPARAM, np.float32)
     v = np.zeros(M.NPARAM, np.float32)
     Po = P.copy()
-    for step, (loss_gpu, P_gpu) in enumerate(((loss1, P1), (loss2, P2)), start=1):
+    for step, (loss_gpu, P_gpu, wc) in enumerate(((loss1, P1, 0.0), (loss2, P2, 0.1)), start=1):
         idx = M.minibatch(M.step_seed(seed, step), nsamp, batch)
-        L, g = M.loss_grad(Po.astype(np.float64), Xb[idx], Y[idx], emulate=True)
+        L, g = M.loss_grad(Po.astype(np.float64), X[idx], Y[idx], emulate=True, weight_c=wc)
         Pn, m, v = M.adam(Po, m, v, g, lr, step)
         say(f"  step {step}: loss gpu {loss_gpu:.6e} oracle {L:.6e}")
         assert abs(loss_gpu - L) <= 1e-4 * L
@@ -151,14 +152,16 @@ def test_network_pass_matches_oracle(precision):
         out = interior(s.download_phi())
     h = 1.0 / n
     pre_t = pre if precision == 0 else pre.astype(np.float32)
-    want = M.apply(P.astype(np.float64), pre_t, h, 6.0)
-    assert nb == M.band(pre_t, np.asarray(6.0 * h, dtype=pre_t.dtype)).size
+    want = M.apply(P.astype(np.float64), pre_t, h, 6.0, 1.5 * h)
+    cells = M.band(pre_t, np.asarray(6.0 * h, dtype=pre_t.dtype))
+    assert nb == cells.size
     d = np.abs(out - want)
-    moved = np.abs(want - pre) > 0
-    say(f"  band {nb}, max |gpu - oracle| / h = {d.max() / h:.2e}, cells moved {moved.sum()}")
-    assert moved.sum() == nb
-    assert d.max() <= 1e-3 * h
-    assert np.array_equal(out[~moved], pre[~moved])
+    inband = np.zeros(pre.size, dtype=bool)
+    inband[cells] = True
+    inband = inband.reshape(pre.shape)
+    say(f"  band {nb}, max |gpu - oracle| / h = {d.max() / h:.2e}")
+    assert d.max() <= 1e-5 * h
+    assert np.array_equal(out[~inband], pre[~inband])
 
 
 def test_collected_pairs_match_plain_reinit():
@@ -183,7 +186,7 @@ def test_collected_pairs_match_plain_reinit():
     h = 1.0 / n
     cells = M.band(pre, 6.0 * h)
     assert got == cells.size
-    X = M.features(pre, cells, np.float32(1.0 / h))
+    X = M.features(pre, cells, np.float32(1.0 / h), np.float32(1.0 / (12.0 * h)))
     Y = ((post.ravel()[cells] - pre.ravel()[cells]).astype(np.float32) * np.float32(1.0 / h))
     idx = M.minibatch(M.step_seed(3, 1), got, 2048)
     L, _ = M.loss_grad(P.astype(np.float64), X[idx], Y[idx], emulate=True)
@@ -224,11 +227,11 @@ def test_fallback_equals_plain_reinit():
         a.upload_phi(box)
         b.upload_phi(box)
         _, n_plain = a.reinit(1)
-        res, fb, n_l = b.reinit_learned(net, 6.0, 2, 1e-12)
+        res, fb, n_l = b.reinit_learned(net, 6.0, 2, 1e-12, 3.0, lsmp.FALLBACK_RESTART)
         assert fb
         assert n_l == n_plain + 2
         assert np.array_equal(a.download_phi(), b.download_phi())
-        res2, fb2, n2 = b.reinit_learned(net, 6.0, 2, 10.0)
+        res2, fb2, n2 = b.reinit_learned(net, 6.0, 2, 10.0, 3.0, lsmp.FALLBACK_RESTART)
         assert not fb2 and n2 == 2
 
 
@@ -248,3 +251,86 @@ def test_training_reduces_loss():
         l0 = net.train(1, batch=4096, lr=0.0, seed=9)
         hist = L.train(net, steps=600, batch=4096, lr=2e-3, lr_end=2e-4, chunk=200, seed=4, log=say)
     assert hist[-1][2] < 0.2 * l0
+
+
+def test_continue_fallback_stops_within_plain_budget():
+    """The continuation fallback runs plain pseudo-steps in blocks until the
+    residual passes: with an unreachable threshold it spends exactly the
+    plain method's pseudo-step count, with a loose one only the 2 sweeps."""
+    lsmp = load_lsmp()
+    n = 32
+    box = sphere_box(n, r=0.3, warp=0.2)
+    with lsmp.Solver(lsmp.make_params(n, dim=3)) as a, lsmp.Solver(lsmp.make_params(n, dim=3)) as b, \
+            lsmp.Net(seed=3) as net:
+        net.set_params(np.zeros(net.nparam, np.float32))
+        a.upload_phi(box)
+        _, n_plain = a.reinit(1)
+        b.upload_phi(box)
+        res, fb, n_c = b.reinit_learned(net, 6.0, 2, 1e-12, 1.5, lsmp.FALLBACK_CONTINUE)
+        say(f"continue: plain {n_plain} pseudo-steps, continuation {n_c}, residual {res:.3e}")
+        assert fb and n_c == n_plain
+        # a zero network leaves phi unchanged, so the continuation with the full
+        # budget is the plain method itself (same sign field, same steps)
+        assert np.array_equal(a.download_phi(), b.download_phi())
+        b.upload_phi(box)
+        res, fb, n_c = b.reinit_learned(net, 6.0, 2, 10.0, 1.5, lsmp.FALLBACK_CONTINUE)
+        assert not fb and n_c == 2
+
+
+def _learned_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "learned", os.path.join(ROOT, "mass-preserving-level-set-method_amd", "learned.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.mark.parametrize("seed", [1001, 1002])
+def test_ablation_network_gains_over_sweeps_alone(seed):
+    """What the trained network adds: on held-out random shapes and flows
+    (seeds the training never drew) the MPLS loop runs 200 steps, then the
+    field after 1 and after 3 more advection steps is reinitialised three
+    ways.  Network + 2 sweeps must land closer to the full plain
+    reinitialisation than the 2 sweeps alone (band rms error over
+    |phi| < 6h, in units of h)."""
+    lsmp = load_lsmp()
+    L = _learned_module()
+    rng = np.random.default_rng(seed)
+    n = 96
+    h = 1.0 / n
+    with L.load_net() as net, lsmp.Net() as zero:
+        zero.set_params(np.zeros(zero.nparam, np.float32))
+        with lsmp.Solver(lsmp.make_params(n, 3, precision=lsmp.FP32)) as s:
+            s.upload_phi(L.ellipsoid_union(n, rng))
+            s.reinit(1)
+            s.reinit(1)
+            imass, _ = s.mass_vol()
+            s.set_velocity(*L.fourier_velocity(n, rng))
+            for _ in range(200):
+                s.rk3_advect()
+                s.reinit(1)
+                s.mass_fix(imass)
+            for nadv in (1, 3):
+                for _ in range(nadv):
+                    s.rk3_advect()
+                pre = s.download_phi()
+                out = {}
+                for kind in ("plain", "sweeps", "net"):
+                    with lsmp.Solver(lsmp.make_params(n, 3, precision=lsmp.FP32)) as t:
+                        t.upload_phi(pre)
+                        if kind == "plain":
+                            t.reinit(1)
+                        else:
+                            t.reinit_learned(net if kind == "net" else zero, 6.0, 2, 1e9, 1.5,
+                                             lsmp.FALLBACK_RESTART)
+                        out[kind] = interior(t.download_phi())
+                ref = out["plain"]
+                band = np.abs(ref) < 6 * h
+                e_s = np.sqrt(np.mean(((out["sweeps"] - ref)[band] / h) ** 2))
+                e_n = np.sqrt(np.mean(((out["net"] - ref)[band] / h) ** 2))
+                say(f"seed {seed} +{nadv} steps: rms vs plain: sweeps alone {e_s:.4e}, network + sweeps {e_n:.4e}"
+                    f" (ratio {e_n / e_s:.3f})")
+                assert e_n < 0.97 * e_s
+                s.reinit(1)
+                s.mass_fix(imass)
