@@ -103,6 +103,23 @@ LSMP_DEV void crweno_f_row(T f0, T f1, T f2, T f3, T f4, T &A, T &B, T &C, T &S)
     S = (T(5) * w3 + w2) / T(6) * f1 + (w3 + T(5) * (w2 + w1)) / T(6) * f2 + w1 / T(6) * f3;
 }
 
+// Reciprocal of a normal nonzero number (either sign): the hardware estimate
+// refined by Newton steps (two in FP64, one in FP32), within an ulp or two of
+// the correctly rounded 1/x.
+LSMP_DEV double rcp_pos(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+LSMP_DEV float rcp_pos(float x)
+{
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+
 // The same two rows with the weight divisions folded into one reciprocal:
 // with e_k = EPS + b_k and tau = |b3 - b1| the reference's
 // a_k = c_k (1 + tau / e_k) normalise to
@@ -110,7 +127,7 @@ LSMP_DEV void crweno_f_row(T f0, T f1, T f2, T f3, T f4, T &A, T &B, T &C, T &S)
 // and the divisions by 3 and 6 become products.  Equal to crweno_*_row() up to
 // rounding of the weights.  The fold needs the FP64 exponent range: e_k lies in
 // [1e-13, ~2e3], so a product of three can reach 1e-39, below FP32's normal
-// range; the FP32 form keeps one division per weight.
+// range; the FP32 form keeps one reciprocal per weight.
 LSMP_DEV void crweno_weights_fast(double b1, double b2, double b3, double &w1, double &w2, double &w3)
 {
     const double EPS = 1.0e-13;
@@ -120,7 +137,7 @@ LSMP_DEV void crweno_weights_fast(double b1, double b2, double b3, double &w1, d
     const double a1 = c1 * (e1 + tau) * (e2 * e3);
     const double a2 = c2 * (e2 + tau) * (e1 * e3);
     const double a3 = c3 * (e3 + tau) * (e1 * e2);
-    const double r = 1.0 / (a1 + a2 + a3);
+    const double r = rcp_pos(a1 + a2 + a3);
     w1 = a1 * r;
     w2 = a2 * r;
     w3 = a3 * r;
@@ -130,10 +147,10 @@ LSMP_DEV void crweno_weights_fast(float b1, float b2, float b3, float &w1, float
     const float EPS = 1.0e-13f;
     const float c1 = 0.2089141306f, c2 = 0.4999999998f, c3 = 0.2910858692f;
     const float tau = fabsf(b3 - b1);
-    const float a1 = c1 * (1.0f + tau / (EPS + b1));
-    const float a2 = c2 * (1.0f + tau / (EPS + b2));
-    const float a3 = c3 * (1.0f + tau / (EPS + b3));
-    const float r = 1.0f / (a1 + a2 + a3);
+    const float a1 = c1 * (1.0f + tau * rcp_pos(EPS + b1));
+    const float a2 = c2 * (1.0f + tau * rcp_pos(EPS + b2));
+    const float a3 = c3 * (1.0f + tau * rcp_pos(EPS + b3));
+    const float r = rcp_pos(a1 + a2 + a3);
     w1 = a1 * r;
     w2 = a2 * r;
     w3 = a3 * r;
@@ -189,22 +206,6 @@ template <typename T> LSMP_DEV T phyn(T a, T b, T c, T d)
 // alp2 = 3/q2 normalise to w0 = q1 q2 / D and w2 = 3 q0 q1 / D with
 // D = q1 q2 + 6 q0 q2 + 3 q0 q1.  Equal to phyn() up to rounding of the weights
 // (q_k lies in [1e-12, 3e6], so D neither under- nor overflows in FP32).
-// Reciprocal of a positive normal number: the hardware estimate refined by two
-// Newton steps (within an ulp or two of 1/x; the weights it feeds are ratios).
-LSMP_DEV double rcp_pos(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
-LSMP_DEV float rcp_pos(float x)
-{
-    float r = __builtin_amdgcn_rcpf(x);
-    return fmaf(r, fmaf(-x, r, 1.0f), r);
-}
-
 template <typename T> LSMP_DEV T phyn_fast(T a, T b, T c, T d)
 {
     const T eps = T(1.0e-6);

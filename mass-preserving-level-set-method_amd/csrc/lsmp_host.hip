@@ -527,13 +527,14 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
                        &c->S0, &c->phi0, &c->dxv, &c->dyv, &c->gr};
     for (void **f : fields)
         if (alloc(f, fb)) return fail(LSMP_E_HIP);
-    // scratch: CRWENO (c', d') of 2 systems for n-2 rows per line, or CCD
-    // (S, SS) for n+5 rows per line, whichever direction needs most
+    // scratch: CRWENO (c', d') for n-2 rows of 2 systems per line (64 slots
+    // per 32 lines), or CCD (S, SS) for n+5 rows per line, whichever
+    // direction needs most
     size_t sc = 0;
     const int ns[3] = {c->nx, c->ny, c->nz};
     for (int d = 0; d < c->dim; ++d) {
         const size_t nlines = c->ncell / (size_t)ns[d];
-        sc = std::max(sc, (size_t)4 * (size_t)(ns[d] - 2) * nlines);
+        sc = std::max(sc, (size_t)2 * (size_t)(ns[d] - 2) * ((nlines + 31) / 32 * 64));
         sc = std::max(sc, (size_t)2 * (size_t)(ns[d] + 5) * nlines);
     }
     if (alloc(&c->scr, sc * c->esz)) return fail(LSMP_E_HIP);

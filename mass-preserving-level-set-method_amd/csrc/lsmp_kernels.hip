@@ -28,262 +28,448 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
 }
 
 // ---------------------------------------------------------------------------
-// Advection: one CRWENO5 line solve per lane (level_set_rk3_solver.f90:143-219,
-// crweno_flux_split :347-414, wenojs_flux_split :288-345, Thomas TDMA.f90:1-36).
+// Advection (level_set_rk3_solver.f90:1-221): CRWENO5 line solves
+// (crweno_flux_split :347-414, wenojs_flux_split :288-345, Thomas
+// TDMA.f90:1-36) along x, y and z, accumulated into the RHS; the last
+// direction applies the TVD-RK3 stage (:16-88) in place.
 //
 // Fortran cells c = 1..n of a line map to 0-based q = c-1; interface m
 // (0..n) is the face between cells m and m+1.  The two boundary faces come
-// from WENO-JS, faces 1..n-1 from the two tridiagonal systems.  The Thomas
-// forward sweep stores (c', d') of both systems for rows 1..n-2 in scr at
-// scr[((m-1)*4 + comp)*nlines + L]; the backward sweep emits the flux
-// difference of each cell, from cell n down to cell 1.
+// from WENO-JS, faces 1..n-1 from the two tridiagonal systems.
 // ---------------------------------------------------------------------------
-template <typename T, class Loader, class Emit>
-LSMP_DEV void crweno_line(const Loader &ldc, const Emit &emit, T *__restrict__ scr, size_t nlines,
-                          size_t L, int n, T inv_unused, T dd, bool store)
-{
-    (void)inv_unused;
-    const T inv_dd = T(1) / dd;
-    T g[6], f[6];
-    // boundary faces by WENO-JS
-    T gA, fA, gB, fB, gC, fC, gD, fD, gE, fE, gF, fF;
-    ldc(-2, gA, fA); ldc(-1, gB, fB); ldc(0, gC, fC); ldc(1, gD, fD); ldc(2, gE, fE); ldc(3, gF, fF);
-    const T gm0 = wenojs_g(gA, gB, gC, gD, gE);
-    const T fp0 = wenojs_f(fB, fC, fD, fE, fF);
-    ldc(n - 2, gA, fA); ldc(n - 1, gB, fB); ldc(n, gC, fC); ldc(n + 1, gD, fD); ldc(n + 2, gE, fE);
-    ldc(n + 3, gF, fF);
-    const T gmN = wenojs_g(gA, gB, gC, gD, gE);
-    const T fpN = wenojs_f(fB, fC, fD, fE, fF);
+// ---------------------------------------------------------------------------
+// Paired line solve: two lanes per line.  Lane l < 32 of a wave solves the
+// positive-flux (g) system of line 32 w + l, lane l + 32 the negative-flux (f)
+// system of the same line.  An f row is the g row of the mirrored stencil
+// f(i+3..i-1) with A and C exchanged (crweno_flux_split :388-407 against
+// :362-381), so both halves run one instruction stream; the two boundary
+// faces come from WENO-JS per system (:383-384, 409-410).  Each lane runs the
+// reference's Thomas recurrence (TDMA.f90:1-36) on its system: the forward
+// sweep keeps (c', d') of rows 1..n-2 in scr[((m-1)*2 + comp)*nslot + slot],
+// slot = 64 w + lane (coalesced), and the backward sweep forms the flux
+// difference of each cell; the f lane hands its half to the g lane by a
+// cross-half shuffle and the g lane applies the RHS or the RK stage.
+//
+// Memory latency is hidden by prefetching into registers: the forward sweep
+// loads the AXC cells it will need a chunk ahead, the backward sweep the
+// scratch rows and RHS operands of the next AXB rows.
+//
+// Line L starts at (L / lw) * ostride + (L % lw) * istride with element
+// stride estride (x lines: lw = nlines, istride = nx, estride = 1).
+// ---------------------------------------------------------------------------
+constexpr int AXC = 16;
+constexpr int AXB = 8;
 
-    // window: g[t], f[t] hold cell m-2+t
-#pragma unroll
-    for (int t = 0; t < 6; ++t) ldc(-1 + t, g[t], f[t]);
-    T cgp = T(0), dgp = T(0), cfp = T(0), dfp = T(0);
-    for (int m = 1; m <= n - 1; ++m) {
-        T A, B, C, S;
-        crweno_g_row_fast(g[0], g[1], g[2], g[3], g[4], A, B, C, S);
-        if (m == 1) S = S - A * gm0;
-        if (m == n - 1) S = S - C * gmN;
-        T cg, dg;
-        {
-            const T r = T(1) / (m == 1 ? B : B - A * cgp);
-            cg = C * r;
-            dg = (m == 1 ? S : S - A * dgp) * r;
-        }
-        crweno_f_row_fast(f[1], f[2], f[3], f[4], f[5], A, B, C, S);
-        if (m == 1) S = S - A * fp0;
-        if (m == n - 1) S = S - C * fpN;
-        T cf, df;
-        {
-            const T r = T(1) / (m == 1 ? B : B - A * cfp);
-            cf = C * r;
-            df = (m == 1 ? S : S - A * dfp) * r;
-        }
-        if (m < n - 1 && store) {
-            size_t o = (size_t)(m - 1) * 4 * nlines + L;
-            scr[o] = cg;
-            scr[o + nlines] = dg;
-            scr[o + 2 * nlines] = cf;
-            scr[o + 3 * nlines] = df;
-        }
-        cgp = cg; dgp = dg; cfp = cf; dfp = df;
-        if (m < n - 1) {
-#pragma unroll
-            for (int t = 0; t < 5; ++t) { g[t] = g[t + 1]; f[t] = f[t + 1]; }
-            ldc(m + 4, g[5], f[5]);
-        }
-    }
-    // backward sweep: X(n-1) = d'(n-1)
-    T Xg = dgp, Xf = dfp;
-    emit(n, (Xg - gmN) * inv_dd, (Xf - fpN) * inv_dd);
-    for (int m = n - 2; m >= 1; --m) {
-        T cg = T(0), dg = T(0), cf = T(0), df = T(0);
-        if (store) {
-            size_t o = (size_t)(m - 1) * 4 * nlines + L;
-            cg = scr[o];
-            dg = scr[o + nlines];
-            cf = scr[o + 2 * nlines];
-            df = scr[o + 3 * nlines];
-        }
-        T Xg1 = dg - cg * Xg;
-        T Xf1 = df - cf * Xf;
-        emit(m + 1, (Xg1 - Xg) * inv_dd, (Xf1 - Xf) * inv_dd);
-        Xg = Xg1; Xf = Xf1;
-    }
-    emit(1, (gm0 - Xg) * inv_dd, (fp0 - Xf) * inv_dd);
-}
-
-// RHS accumulation and fused RK stage (level_set_rk3_solver.f90:16-88, 207-212).
-// MODE 0: R = t1 + t2 (first direction); 1: R = (R + t1) + t2;
-// 2: last direction, R complete -> stage update of phi in place.
 template <typename T, int MODE, int STAGE>
-LSMP_DEV void adv_emit(T *__restrict__ R, T *phi, T *__restrict__ L1, T *__restrict__ L2, size_t a, T t1,
-                       T t2, T dt)
+__global__ __launch_bounds__(256) void k_adv_pair(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
+                                                   T *__restrict__ scr, int nlines, int n, int lw, long ostride,
+                                                   long istride, long estride, T dd, T dt)
 {
-    if (MODE == 0) {
-        R[a] = t1 + t2;
-    } else if (MODE == 1) {
-        R[a] = (R[a] + t1) + t2;
-    } else {
-        T r = (R[a] + t1) + t2;
-        if (STAGE == 1) {
-            L1[a] = r;
-            phi[a] = phi[a] + r * dt;
-        } else if (STAGE == 2) {
-            L2[a] = r;
-            T src = (T(-3) * L1[a] + r) / T(4);
-            phi[a] = phi[a] + src * dt;
-        } else {
-            T src = (-L1[a] - L2[a] + T(8) * r) / T(12);
-            phi[a] = phi[a] + src * dt;
-        }
-    }
-}
-
-// Lines along y or z: one lane per line, lanes on consecutive i (coalesced).
-// Line L -> base (L / nx) * ostride + L % nx, element stride estride.
-template <typename T, int MODE, int STAGE>
-__global__ __launch_bounds__(256) void k_adv_strided(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
-                                                      T *__restrict__ scr, int nx, int nlines, int n,
-                                                      long ostride, long estride, T dd, T dt)
-{
-    const int L = blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= nlines) return;
-    const size_t base = (size_t)(L / nx) * (size_t)ostride + (size_t)(L % nx);
-    auto ldc = [&](int c, T &g, T &f) {
-        int q = clampi(c - 1, 0, n - 1);
-        size_t a = base + (size_t)q * (size_t)estride;
-        flux_split(phi[a], vel[a], g, f);
-    };
-    auto emit = [&](int c, T t1, T t2) {
-        size_t a = base + (size_t)(c - 1) * (size_t)estride;
-        adv_emit<T, MODE, STAGE>(R, phi, L1, L2, a, t1, t2, dt);
-    };
-    crweno_line<T>(ldc, emit, scr, (size_t)nlines, (size_t)L, n, T(0), dd, true);
-}
-
-// Lines along x: 64 lanes = 64 lines (consecutive j); the contiguous x data is
-// staged through an LDS tile of XCH rows so that global reads and writes stay
-// coalesced.  Only the first direction, so MODE 0.
-template <typename T>
-__global__ __launch_bounds__(64) void k_adv_x(const T *__restrict__ phi, const T *__restrict__ vel,
-                                               T *__restrict__ R, T *__restrict__ scr, int nx, int ny,
-                                               int nlines, T dd)
-{
-    __shared__ T tp[64][XCH + 1];
-    __shared__ T tu[64][XCH + 1];
-    __shared__ T to[64][XCH + 1];
-    const int t = threadIdx.x;
-    const int L0 = blockIdx.x * 64;
-    const int L = L0 + t;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int nwaves = (nlines + 31) >> 5;
+    if (wave >= nwaves) return;   // whole waves only: the shuffles below need every lane
+    const bool isf = lane >= 32;
+    const int L = wave * 32 + (lane & 31);
     const bool act = L < nlines;
-    const int n = nx;
-    auto base_of = [&](int l) -> size_t { return (size_t)(l / ny) * (size_t)nx * ny + (size_t)(l % ny) * nx; };
-    const size_t base = act ? base_of(L) : 0;
-    int cur = -1;
-    auto load_chunk = [&](int ch) {
-        __syncthreads();
-        for (int idx = t; idx < 64 * XCH; idx += 64) {
-            int b = idx / XCH, e = idx % XCH;
-            int lb = L0 + b, q = ch * XCH + e;
-            if (lb < nlines && q < n) {
-                size_t a = base_of(lb) + q;
-                tp[b][e] = phi[a];
-                tu[b][e] = vel[a];
+    const int Lc = act ? L : nlines - 1;   // idle lanes shadow the last line and store nothing
+    const size_t base = (size_t)(Lc / lw) * (size_t)ostride + (size_t)(Lc % lw) * (size_t)istride;
+    const size_t nslot = (size_t)nwaves * 64, slot = (size_t)wave * 64 + lane;
+    const T inv_dd = T(1) / dd;
+    // Fortran cell c = 1..n -> element c - 1, clamped (zero-gradient ghosts)
+    auto at = [&](int c) -> size_t { return base + (size_t)clampi(c - 1, 0, n - 1) * (size_t)estride; };
+    auto flux = [&](T p, T u) {
+        T g, f;
+        flux_split(p, u, g, f);
+        return isf ? f : g;
+    };
+    auto ld = [&](int c) {
+        const size_t a = at(c);
+        return flux(phi[a], vel[a]);
+    };
+    auto emit = [&](size_t a, T t1, T t2, T Rv, T Pv, T L1v, T L2v) {
+        if (MODE == 0) {
+            R[a] = t1 + t2;
+        } else if (MODE == 1) {
+            R[a] = (Rv + t1) + t2;
+        } else {
+            const T r = (Rv + t1) + t2;
+            if (STAGE == 1) {
+                L1[a] = r;
+                phi[a] = Pv + r * dt;
+            } else if (STAGE == 2) {
+                L2[a] = r;
+                phi[a] = Pv + (T(-3) * L1v + r) / T(4) * dt;
+            } else {
+                phi[a] = Pv + (-L1v - L2v + T(8) * r) / T(12) * dt;
             }
         }
-        __syncthreads();
     };
-    auto ldg = [&](int c, T &g, T &f) {   // direct load, boundary stencils only
-        int q = clampi(c - 1, 0, n - 1);
-        if (act) flux_split(phi[base + q], vel[base + q], g, f);
-        else { g = T(0); f = T(0); }
-    };
-    auto ldc = [&](int c, T &g, T &f) {
-        int q = clampi(c - 1, 0, n - 1);
-        int ch = q / XCH;
-        if (ch != cur) { load_chunk(ch); cur = ch; }
-        flux_split(tp[t][q % XCH], tu[t][q % XCH], g, f);
-    };
-    int och = (n - 1) / XCH;
-    auto flush = [&](int ch) {
-        __syncthreads();
-        for (int idx = t; idx < 64 * XCH; idx += 64) {
-            int b = idx / XCH, e = idx % XCH;
-            int lb = L0 + b, q = ch * XCH + e;
-            if (lb < nlines && q < n) R[base_of(lb) + q] = to[b][e];
-        }
-        __syncthreads();
-    };
-    auto emit = [&](int c, T t1, T t2) {
-        int q = c - 1;
-        int ch = q / XCH;
-        if (ch != och) { flush(och); och = ch; }
-        to[t][q % XCH] = t1 + t2;
+    auto emit_direct = [&](int c, T t) {   // boundary cells: operands loaded here
+        const T t2 = __shfl_xor(t, 32, 64);
+        if (isf || !act) return;
+        const size_t a = at(c);
+        T Rv = T(0), Pv = T(0), L1v = T(0), L2v = T(0);
+        if (MODE >= 1) Rv = R[a];
+        if (MODE == 2) Pv = phi[a];
+        if (MODE == 2 && STAGE >= 2) L1v = L1[a];
+        if (MODE == 2 && STAGE == 3) L2v = L2[a];
+        emit(a, t, t2, Rv, Pv, L1v, L2v);
     };
 
-    const T inv_dd = T(1) / dd;
-    // boundary faces straight from global memory, the interior from the tile
-    T gA, fA, gB, fB, gC, fC, gD, fD, gE, fE, gF, fF;
-    ldg(-2, gA, fA); ldg(-1, gB, fB); ldg(0, gC, fC); ldg(1, gD, fD); ldg(2, gE, fE); ldg(3, gF, fF);
-    const T gm0 = wenojs_g(gA, gB, gC, gD, gE);
-    const T fp0 = wenojs_f(fB, fC, fD, fE, fF);
-    ldg(n - 2, gA, fA); ldg(n - 1, gB, fB); ldg(n, gC, fC); ldg(n + 1, gD, fD); ldg(n + 2, gE, fE);
-    ldg(n + 3, gF, fF);
-    const T gmN = wenojs_g(gA, gB, gC, gD, gE);
-    const T fpN = wenojs_f(fB, fC, fD, fE, fF);
-
-    T g[6], f[6];
+    // boundary faces of this lane's system
+    T bc0, bcN;
+    {
+        T v[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) ldc(-1 + q, g[q], f[q]);
-    T cgp = T(0), dgp = T(0), cfp = T(0), dfp = T(0);
-    const size_t nl = (size_t)nlines;
-    for (int m = 1; m <= n - 1; ++m) {
+        for (int t = 0; t < 6; ++t) v[t] = ld(-2 + t);
+        bc0 = isf ? wenojs_f(v[1], v[2], v[3], v[4], v[5]) : wenojs_g(v[0], v[1], v[2], v[3], v[4]);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) v[t] = ld(n - 2 + t);
+        bcN = isf ? wenojs_f(v[1], v[2], v[3], v[4], v[5]) : wenojs_g(v[0], v[1], v[2], v[3], v[4]);
+    }
+
+    // forward sweep; w[t] holds this system's flux at cell m - 2 + t.  Row 1
+    // and row n - 1 carry the boundary faces; rows 2 .. n - 2 run in unrolled
+    // chunks (n >= 4, so there is at least one).
+    T w[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) w[t] = ld(-1 + t);
+    auto coeffs = [&](T &A, T &B, T &C, T &S) {
+        const T x0 = isf ? w[5] : w[0], x1 = isf ? w[4] : w[1], x2 = isf ? w[3] : w[2];
+        const T x3 = isf ? w[2] : w[3], x4 = isf ? w[1] : w[4];
+        T Ag, Cg;
+        crweno_g_row_fast(x0, x1, x2, x3, x4, Ag, B, Cg, S);
+        A = isf ? Cg : Ag;
+        C = isf ? Ag : Cg;
+    };
+    auto shift = [&](T next) {
+#pragma unroll
+        for (int t = 0; t < 5; ++t) w[t] = w[t + 1];
+        w[5] = next;
+    };
+    T cp, dp;
+    size_t o = slot;   // scratch offset of the current row: (m - 1) * 2 * nslot + slot
+    {
         T A, B, C, S;
-        crweno_g_row_fast(g[0], g[1], g[2], g[3], g[4], A, B, C, S);
-        if (m == 1) S = S - A * gm0;
-        if (m == n - 1) S = S - C * gmN;
-        T cg, dg;
-        {
-            const T r = T(1) / (m == 1 ? B : B - A * cgp);
-            cg = C * r;
-            dg = (m == 1 ? S : S - A * dgp) * r;
-        }
-        crweno_f_row_fast(f[1], f[2], f[3], f[4], f[5], A, B, C, S);
-        if (m == 1) S = S - A * fp0;
-        if (m == n - 1) S = S - C * fpN;
-        T cf, df;
-        {
-            const T r = T(1) / (m == 1 ? B : B - A * cfp);
-            cf = C * r;
-            df = (m == 1 ? S : S - A * dfp) * r;
-        }
-        if (m < n - 1 && act) {
-            size_t o = (size_t)(m - 1) * 4 * nl + L;
-            scr[o] = cg; scr[o + nl] = dg; scr[o + 2 * nl] = cf; scr[o + 3 * nl] = df;
-        }
-        cgp = cg; dgp = dg; cfp = cf; dfp = df;
-        if (m < n - 1) {
-#pragma unroll
-            for (int q = 0; q < 5; ++q) { g[q] = g[q + 1]; f[q] = f[q + 1]; }
-            ldc(m + 4, g[5], f[5]);
-        }
-    }
-    T Xg = dgp, Xf = dfp;
-    emit(n, (Xg - gmN) * inv_dd, (Xf - fpN) * inv_dd);
-    for (int m = n - 2; m >= 1; --m) {
-        T cg = T(0), dg = T(0), cf = T(0), df = T(0);
+        coeffs(A, B, C, S);
+        S = S - A * bc0;
+        const T rr = rcp_pos(B);
+        cp = C * rr;
+        dp = S * rr;
         if (act) {
-            size_t o = (size_t)(m - 1) * 4 * nl + L;
-            cg = scr[o]; dg = scr[o + nl]; cf = scr[o + 2 * nl]; df = scr[o + 3 * nl];
+            scr[o] = cp;
+            scr[o + nslot] = dp;
         }
-        T Xg1 = dg - cg * Xg;
-        T Xf1 = df - cf * Xf;
-        emit(m + 1, (Xg1 - Xg) * inv_dd, (Xf1 - Xf) * inv_dd);
-        Xg = Xg1; Xf = Xf1;
+        o += 2 * nslot;
+        shift(ld(5));
     }
-    emit(1, (gm0 - Xg) * inv_dd, (fp0 - Xf) * inv_dd);
-    flush(och);
+    auto mid = [&](T next) {
+        T A, B, C, S;
+        coeffs(A, B, C, S);
+        const T rr = rcp_pos(B - A * cp);
+        const T c = C * rr;
+        const T d = (S - A * dp) * rr;
+        if (act) {
+            scr[o] = c;
+            scr[o + nslot] = d;
+        }
+        o += 2 * nslot;
+        cp = c;
+        dp = d;
+        shift(next);
+    };
+    // row m consumes cell m + 4; slot r of the rolling buffer serves rows
+    // m = 2 + r (mod AXC) and is reloaded AXC rows ahead as soon as it is used
+    T pp[AXC], pu[AXC];
+    int m0 = 2;
+    if (m0 + AXC - 1 <= n - 2) {
+#pragma unroll
+        for (int r = 0; r < AXC; ++r) {
+            const size_t ad = at(m0 + 4 + r);
+            pp[r] = phi[ad];
+            pu[r] = vel[ad];
+        }
+    }
+    for (; m0 + AXC - 1 <= n - 2; m0 += AXC) {
+#pragma unroll
+        for (int r = 0; r < AXC; ++r) {
+            const T next = flux(pp[r], pu[r]);
+            const size_t ad = at(m0 + AXC + 4 + r);   // clamped: reads past the line end are harmless
+            pp[r] = phi[ad];
+            pu[r] = vel[ad];
+            mid(next);
+        }
+    }
+    for (int m = m0; m <= n - 2; ++m) mid(ld(m + 4));
+    {
+        T A, B, C, S;
+        coeffs(A, B, C, S);
+        S = S - C * bcN;
+        dp = (S - A * dp) * rcp_pos(B - A * cp);
+    }
+
+    // backward sweep: X(n-1) = d'(n-1), then X(m) = d'(m) - c'(m) X(m+1) for
+    // m = n - 2 .. 1, emitting cell m + 1 (element m of the line)
+    T X = dp;
+    emit_direct(n, (X - bcN) * inv_dd);
+    size_t ae = base + (size_t)(n - 2) * (size_t)estride;   // element of the row being emitted
+    auto back = [&](T c, T d, T Rv, T Pv, T L1v, T L2v) {
+        const T X1 = d - c * X;
+        const T t = (X1 - X) * inv_dd;
+        const T t2 = __shfl_xor(t, 32, 64);
+        if (!isf && act) emit(ae, t, t2, Rv, Pv, L1v, L2v);
+        ae -= (size_t)estride;
+        X = X1;
+    };
+    // rolling buffer of the scratch rows and RHS operands, AXB rows ahead
+    T pc[AXB], pd[AXB], pR[AXB], pP[AXB], pL1[AXB], pL2[AXB];
+    auto load_b = [&](int r, int m) {   // row m >= 1 into slot r
+        const size_t ob = (size_t)(m - 1) * 2 * nslot + slot;
+        pc[r] = scr[ob];
+        pd[r] = scr[ob + nslot];
+        if (MODE >= 1 && !isf) {
+            const size_t ad = base + (size_t)m * (size_t)estride;
+            pR[r] = R[ad];
+            if (MODE == 2) pP[r] = phi[ad];
+            if (MODE == 2 && STAGE >= 2) pL1[r] = L1[ad];
+            if (MODE == 2 && STAGE == 3) pL2[r] = L2[ad];
+        }
+    };
+    m0 = n - 2;
+    if (m0 - AXB + 1 >= 1) {
+#pragma unroll
+        for (int r = 0; r < AXB; ++r) load_b(r, m0 - r);
+    }
+    for (; m0 - AXB + 1 >= 1; m0 -= AXB) {
+        const bool more = m0 - 2 * AXB + 1 >= 1;   // the next chunk is full: reload
+#pragma unroll
+        for (int r = 0; r < AXB; ++r) {
+            const T c = pc[r], d = pd[r], Rv = pR[r], Pv = pP[r], L1v = pL1[r], L2v = pL2[r];
+            if (more) load_b(r, m0 - AXB - r);
+            back(c, d, Rv, Pv, L1v, L2v);
+        }
+    }
+    for (int m = m0; m >= 1; --m) {   // tail
+        const size_t ob = (size_t)(m - 1) * 2 * nslot + slot;
+        T Rv = T(0), Pv = T(0), L1v = T(0), L2v = T(0);
+        if (MODE >= 1 && !isf) {
+            Rv = R[ae];
+            if (MODE == 2) Pv = phi[ae];
+            if (MODE == 2 && STAGE >= 2) L1v = L1[ae];
+            if (MODE == 2 && STAGE == 3) L2v = L2[ae];
+        }
+        back(scr[ob], scr[ob + nslot], Rv, Pv, L1v, L2v);
+    }
+    emit_direct(1, (bc0 - X) * inv_dd);
+}
+
+// The same paired solve for lines along x (contiguous in memory), first
+// direction only (R = t1 + t2).  Per wave, the 32 lines' cells arrive in
+// chunks of XW through an LDS tile that the wave loads cooperatively (16
+// consecutive elements per 128-byte segment, coalesced); the next chunk is
+// loaded into registers while the current one is consumed.  The backward
+// sweep writes its flux differences into the same tile and flushes each
+// chunk with coalesced stores.  Every wave of the workgroup runs the same
+// chunk loop (idle waves shadow a valid line and store nothing), so the
+// workgroup barriers are uniform.
+constexpr int XW = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_adv_pair_x(const T *__restrict__ phi, const T *__restrict__ vel,
+                                                     T *__restrict__ R, T *__restrict__ scr, int nlines, int n,
+                                                     T dd)
+{
+    __shared__ T tp[4][32][XW + 1];
+    __shared__ T tu[4][32][XW + 1];
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, wv = threadIdx.x >> 6;
+    const int nwaves = (nlines + 31) >> 5;
+    const bool isf = lane >= 32;
+    const int ln = lane & 31;
+    const int L = wave * 32 + ln;
+    const bool act = L < nlines;
+    const size_t base = (size_t)(act ? L : nlines - 1) * (size_t)n;
+    const size_t nslot = (size_t)nwaves * 64;
+    const size_t slot = (size_t)(wave < nwaves ? wave : nwaves - 1) * 64 + lane;
+    const T inv_dd = T(1) / dd;
+    auto at = [&](int c) -> size_t { return base + (size_t)clampi(c - 1, 0, n - 1); };
+    auto flux = [&](T p, T u) {
+        T g, f;
+        flux_split(p, u, g, f);
+        return isf ? f : g;
+    };
+    auto ld = [&](int c) {
+        const size_t a = at(c);
+        return flux(phi[a], vel[a]);
+    };
+    // cooperative chunk access: element e of line l of chunk k is item
+    // lane + 64 i, l = item / XW, e = item % XW
+    constexpr int PER = 32 * XW / 64;
+    auto chunk_addr = [&](int k, int i, int &l, int &e) -> size_t {
+        const int item = lane + 64 * i;
+        l = item / XW;
+        e = item % XW;
+        int line = wave * 32 + l;
+        line = line < nlines ? line : nlines - 1;
+        return (size_t)line * (size_t)n + (size_t)clampi(k * XW + e, 0, n - 1);
+    };
+    T rp[PER], ru[PER];
+    auto load_regs = [&](int k) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int l, e;
+            const size_t a = chunk_addr(k, i, l, e);
+            rp[i] = phi[a];
+            ru[i] = vel[a];
+        }
+    };
+    auto store_tile = [&]() {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int item = lane + 64 * i;
+            tp[wv][item / XW][item % XW] = rp[i];
+            tu[wv][item / XW][item % XW] = ru[i];
+        }
+    };
+
+    T bc0, bcN;
+    {
+        T v[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) v[t] = ld(-2 + t);
+        bc0 = isf ? wenojs_f(v[1], v[2], v[3], v[4], v[5]) : wenojs_g(v[0], v[1], v[2], v[3], v[4]);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) v[t] = ld(n - 2 + t);
+        bcN = isf ? wenojs_f(v[1], v[2], v[3], v[4], v[5]) : wenojs_g(v[0], v[1], v[2], v[3], v[4]);
+    }
+    T w[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) w[t] = ld(-1 + t);
+    auto coeffs = [&](T &A, T &B, T &C, T &S) {
+        const T x0 = isf ? w[5] : w[0], x1 = isf ? w[4] : w[1], x2 = isf ? w[3] : w[2];
+        const T x3 = isf ? w[2] : w[3], x4 = isf ? w[1] : w[4];
+        T Ag, Cg;
+        crweno_g_row_fast(x0, x1, x2, x3, x4, Ag, B, Cg, S);
+        A = isf ? Cg : Ag;
+        C = isf ? Ag : Cg;
+    };
+    auto shift = [&](T next) {
+#pragma unroll
+        for (int t = 0; t < 5; ++t) w[t] = w[t + 1];
+        w[5] = next;
+    };
+    T cp, dp;
+    size_t o = slot;
+    {
+        T A, B, C, S;
+        coeffs(A, B, C, S);
+        S = S - A * bc0;
+        const T rr = rcp_pos(B);
+        cp = C * rr;
+        dp = S * rr;
+        if (act) {
+            scr[o] = cp;
+            scr[o + nslot] = dp;
+        }
+        o += 2 * nslot;
+        shift(ld(5));
+    }
+    auto mid = [&](T next) {
+        T A, B, C, S;
+        coeffs(A, B, C, S);
+        const T rr = rcp_pos(B - A * cp);
+        const T c = C * rr;
+        const T d = (S - A * dp) * rr;
+        if (act) {
+            scr[o] = c;
+            scr[o + nslot] = d;
+        }
+        o += 2 * nslot;
+        cp = c;
+        dp = d;
+        shift(next);
+    };
+    // rows 2 .. n-2; row m consumes cell m + 4 = element m + 3 of chunk (m + 3) / XW
+    const int kfirst = 5 / XW, klast = (n + 1) / XW;
+    load_regs(kfirst);
+    for (int k = kfirst; k <= klast; ++k) {
+        __syncthreads();   // every wave is done with the previous chunk
+        store_tile();
+        __syncthreads();
+        if (k < klast) load_regs(k + 1);
+        const int mlo = max(2, k * XW - 3), mhi = min(n - 2, k * XW + XW - 4);
+        for (int m = mlo; m <= mhi; ++m) {
+            const int e = m + 3 - k * XW;
+            mid(flux(tp[wv][ln][e], tu[wv][ln][e]));
+        }
+    }
+    {
+        T A, B, C, S;
+        coeffs(A, B, C, S);
+        S = S - C * bcN;
+        dp = (S - A * dp) * rcp_pos(B - A * cp);
+    }
+
+    // backward sweep; cell c = m + 1 goes to element m of the output tile
+    T X = dp;
+    auto put = [&](int q, T t) {   // flux difference t of this lane's system at element q
+        const T t2 = __shfl_xor(t, 32, 64);
+        if (!isf) tp[wv][ln][q % XW] = t + t2;
+    };
+    auto flush = [&](int k) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int l, e;
+            chunk_addr(k, i, l, e);
+            const int q = k * XW + e;
+            if (wave * 32 + l < nlines && q < n) R[(size_t)(wave * 32 + l) * (size_t)n + q] = tp[wv][l][e];
+        }
+        __syncthreads();
+    };
+    __syncthreads();   // the forward sweep's last tile reads are done
+    int kout = (n - 1) / XW;
+    put(n - 1, (X - bcN) * inv_dd);
+    T pc[AXB], pd[AXB];
+    auto load_b = [&](int r, int m) {
+        const size_t ob = (size_t)(m - 1) * 2 * nslot + slot;
+        pc[r] = scr[ob];
+        pd[r] = scr[ob + nslot];
+    };
+    auto back = [&](int m, T c, T d) {
+        if (m / XW != kout) {   // element m starts a lower chunk: write the finished one
+            flush(kout);
+            kout = m / XW;
+        }
+        const T X1 = d - c * X;
+        put(m, (X1 - X) * inv_dd);
+        X = X1;
+    };
+    int m0 = n - 2;
+    if (m0 - AXB + 1 >= 1) {
+#pragma unroll
+        for (int r = 0; r < AXB; ++r) load_b(r, m0 - r);
+    }
+    for (; m0 - AXB + 1 >= 1; m0 -= AXB) {
+        const bool more = m0 - 2 * AXB + 1 >= 1;
+#pragma unroll
+        for (int r = 0; r < AXB; ++r) {
+            const T c = pc[r], d = pd[r];
+            if (more) load_b(r, m0 - AXB - r);
+            back(m0 - r, c, d);
+        }
+    }
+    for (int m = m0; m >= 1; --m) {
+        const size_t ob = (size_t)(m - 1) * 2 * nslot + slot;
+        back(m, scr[ob], scr[ob + nslot]);
+    }
+    if (kout != 0) {
+        flush(kout);
+        kout = 0;
+    }
+    put(0, (bc0 - X) * inv_dd);
+    flush(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -939,21 +1125,21 @@ template <typename T>
 void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const T *vel, T *R, T *L1, T *L2,
                      T *scr, int nx, int ny, int nz, T dd, T dt)
 {
-    if (dir == 0) {
-        const int nlines = ny * nz;
-        hipLaunchKernelGGL((k_adv_x<T>), dim3(nblk(nlines, 64)), dim3(64), 0, st, phi, vel, R, scr, nx, ny, nlines,
-                           dd);
-        return;
-    }
-    const int n = dir == 1 ? ny : nz;
-    const int nlines = dir == 1 ? nx * nz : nx * ny;
-    const long ostride = dir == 1 ? (long)nx * ny : (long)nx;
-    const long estride = dir == 1 ? (long)nx : (long)nx * ny;
-    const dim3 g(nblk(nlines, 256)), b(256);
-#define LSMP_ADV(M, S)                                                                                          \
-    hipLaunchKernelGGL((k_adv_strided<T, M, S>), g, b, 0, st, phi, vel, R, L1, L2, scr, nx, nlines, n, ostride, \
-                       estride, dd, dt)
-    if (mode == 1) LSMP_ADV(1, 1);
+    const int n = dir == 0 ? nx : dir == 1 ? ny : nz;
+    const int nlines = dir == 0 ? ny * nz : dir == 1 ? nx * nz : nx * ny;
+    // line L starts at (L / lw) * ostride + (L % lw) * istride
+    const int lw = dir == 0 ? nlines : nx;
+    const long ostride = dir == 0 ? 0 : dir == 1 ? (long)nx * ny : (long)nx;
+    const long istride = dir == 0 ? (long)nx : 1;
+    const long estride = dir == 0 ? 1 : dir == 1 ? (long)nx : (long)nx * ny;
+    const size_t threads = (size_t)((nlines + 31) / 32) * 64;
+    const dim3 g((unsigned)((threads + 255) / 256)), b(256);
+#define LSMP_ADV(M, S)                                                                                        \
+    hipLaunchKernelGGL((k_adv_pair<T, M, S>), g, b, 0, st, phi, vel, R, L1, L2, scr, nlines, n, lw, ostride, \
+                       istride, estride, dd, dt)
+    if (mode == 0) {
+        hipLaunchKernelGGL((k_adv_pair_x<T>), g, b, 0, st, phi, vel, R, scr, nlines, n, dd);
+    } else if (mode == 1) LSMP_ADV(1, 1);
     else if (stage == 1) LSMP_ADV(2, 1);
     else if (stage == 2) LSMP_ADV(2, 2);
     else LSMP_ADV(2, 3);
