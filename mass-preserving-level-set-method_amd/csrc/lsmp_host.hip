@@ -535,7 +535,9 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
     for (int d = 0; d < c->dim; ++d) {
         const size_t nlines = c->ncell / (size_t)ns[d];
         sc = std::max(sc, (size_t)2 * (size_t)(ns[d] - 2) * ((nlines + 31) / 32 * 64));
-        sc = std::max(sc, (size_t)2 * (size_t)(ns[d] + 5) * nlines);
+        // CCD checkpoints: 2 values per line and CSEG-row segment, lanes padded to 256
+        const size_t lanes = (nlines + 255) / 256 * 256, segs = (size_t)(ns[d] + 4) / CSEG + 1;
+        sc = std::max(sc, (size_t)2 * segs * lanes);
     }
     if (alloc(&c->scr, sc * c->esz)) return fail(LSMP_E_HIP);
     for (int d = 0; d < c->dim; ++d) {

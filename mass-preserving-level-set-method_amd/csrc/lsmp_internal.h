@@ -6,8 +6,7 @@
 
 namespace lsmp {
 
-// x-line kernels stage XCH rows of 64 lines through LDS
-constexpr int XCH = 16;
+constexpr int CSEG = 16;   // rows per CCD checkpoint segment
 
 // reinitialisation tile: RTX x RTY interior cells, one ring for the lambda
 // box, four for the WENO-HJ stencil; RKC planes per workgroup in z

@@ -768,117 +768,166 @@ __global__ void k_scale_by_max(T *__restrict__ phi, size_t n, const unsigned lon
 // asgn_src_ccd ccd_solvers.f90:291-310; TWIN_BKS matrix_solver.f90:47-78).
 // A box line has nb = n + 6 points; box point b holds interior cell
 // clamp(b - 3).  coef holds the factorised tables of this direction as
-// A, B, AA, BB, each (3, nb) column-major, then 1/Den of each row.  The forward sweep stores (S, SS)
-// of rows 0..nb-2 in scr[(b*2 + comp)*nlines + L].
+// A, B, AA, BB, each (3, nb) column-major, then 1/Den of each row.
+//
+// Checkpointed line solve (ccd_solvers.f90:412-427): the reference's
+// forward elimination and block substitution with its operation order per
+// row.  One lane per line.  The forward
+// elimination keeps its state (S, SS) in registers and stores it only at the
+// start of every CSEG-row segment; the backward substitution walks the
+// segments downwards, recomputes each segment's forward states from its
+// checkpoint into registers and substitutes through them.  Each segment's
+// loads are issued together, so memory latency is paid once per segment,
+// and the per-row scratch round trip of a sweep that stores every row (32 bytes per cell in
+// FP64) disappears.  The coefficient tables are staged in LDS when they fit
+// (TLDS), else read from global memory (uniform, scalar loads).
+//
+// Line L starts at (L / lw) * ostride + (L % lw) * istride, element stride
+// estride.  Checkpoint k of line L: ck[(2 k + comp) * NL + L], NL = lanes in
+// the grid.
 // ---------------------------------------------------------------------------
-template <typename T, class Loader, class Emit>
-LSMP_DEV void ccd_line(const Loader &ldq, const Emit &emit, const T *__restrict__ coef, T *__restrict__ scr,
-                       size_t nlines, size_t L, int n, T dx, bool store)
+template <typename T, int DIM, bool FINAL, bool TLDS>
+__global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, const T *__restrict__ coef,
+                                                  T *__restrict__ ck, T *__restrict__ out,
+                                                  const T *__restrict__ dxv, const T *__restrict__ dyv,
+                                                  T *__restrict__ gr, double *__restrict__ partial, int nlines,
+                                                  int n, int lw, long ostride, long istride, long estride,
+                                                  CcdArgs ca)
 {
+    extern __shared__ __attribute__((aligned(16))) unsigned char ccd_lds[];
     const int nb = n + 6;
-    const T *A = coef, *B = coef + 3 * nb, *AA = coef + 6 * nb, *BB = coef + 9 * nb, *IDEN = coef + 12 * nb;
-    const T idx = T(1) / dx, idx2 = T(1) / (dx * dx);
-    // sliding window f(b-1), f(b), f(b+1)
-    T fm1, f0, fp1;
-    T S, SS;
-    {
-        T a0 = ldq(-3), a1 = ldq(-2), a2 = ldq(-1), a3 = ldq(0);
-        S = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) * idx;
-        SS = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) * idx2;
-        fm1 = a0; f0 = a1; fp1 = a2;
+    const T *tab = coef;
+    if (TLDS) {
+        T *t = reinterpret_cast<T *>(ccd_lds);
+        for (int i = threadIdx.x; i < 13 * nb; i += blockDim.x) t[i] = coef[i];
+        __syncthreads();
+        tab = t;
     }
-    if (store) {
-        scr[L] = S;
-        scr[nlines + L] = SS;
-    }
-    for (int b = 1; b <= nb - 2; ++b) {
+    const T *A = tab, *B = tab + 3 * nb, *AA = tab + 6 * nb, *BB = tab + 9 * nb, *IDEN = tab + 12 * nb;
+    const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = L < nlines;
+    const int Lc = act ? L : nlines - 1;
+    const size_t base = (size_t)(Lc / lw) * (size_t)ostride + (size_t)(Lc % lw) * (size_t)istride;
+    const size_t NL = (size_t)gridDim.x * blockDim.x;
+    const T idx = T(1) / (T)ca.dx, idx2 = T(1) / ((T)ca.dx * (T)ca.dx);
+    auto ldq = [&](int q) -> T { return phi[base + (size_t)clampi(q, 0, n - 1) * (size_t)estride]; };
+    // forward row b from the window f(b-1), f(b), f(b+1) = phi(b-4), phi(b-3), phi(b-2)
+    auto frow = [&](int b, T fm1, T f0, T fp1, T &S, T &SS) {
         T s = T(15.0 / 16.0) * (-fm1 + fp1) * idx;
         T ss = (T(3) * fm1 - T(6) * f0 + T(3) * fp1) * idx2;
         s = s - (A[3 * b] * S + B[3 * b] * SS);
         ss = ss - (AA[3 * b] * S + BB[3 * b] * SS);
-        S = s; SS = ss;
-        if (store) {
-            size_t o = (size_t)b * 2 * nlines + L;
-            scr[o] = S;
-            scr[o + nlines] = SS;
+        S = s;
+        SS = ss;
+    };
+    double acc[3] = {0.0, 0.0, 0.0};
+    auto emit = [&](int q, T d, T dxa, T dya, T pa) {
+        const size_t a = base + (size_t)q * (size_t)estride;
+        if (!FINAL) {
+            out[a] = d;
+        } else {
+            T g;
+            if (DIM == 3) g = sqrt(sq(dxa) + sq(dya) + sq(d));
+            else g = sqrt(sq(dxa) + sq(d));
+            gr[a] = g;
+            // the mass integral is formed in FP64 in both forms
+            double h, dd;
+            ls_funs<double>((double)pa, ca.ls_wid, h, dd);
+            const double gg = (double)g;
+            const double r12 = ca.rho_12;
+            double rho, plam;
+            acc[0] += h * ca.dv;
+            if (!ca.inverse) rho = h + r12 * (1.0 - h);
+            else rho = r12 * h + (1.0 - h);
+            acc[1] += rho * h * ca.dv;
+            plam = dd * dd * gg;
+            if (ca.inverse) plam = plam * (2.0 * (r12 - 1.0) * h + 1.0) * ca.dx * ca.dy * ca.dz;
+            else plam = plam * (2.0 * (1.0 - r12) * h + r12) * ca.dx * ca.dy * ca.dz;
+            acc[2] += plam;
         }
-        fm1 = f0; f0 = fp1; fp1 = ldq(b - 3 + 2);
+    };
+
+    // row 0: one-sided closure (the state the reference's forward loop starts from)
+    T S0, SS0;
+    {
+        const T a0 = ldq(-3), a1 = ldq(-2), a2 = ldq(-1), a3 = ldq(0);
+        S0 = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) * idx;
+        SS0 = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) * idx2;
     }
+    // forward elimination over rows 1 .. nb-2, segment k = rows [k CSEG, k CSEG + CSEG - 1]
+    const int K = (nb - 2) / CSEG + 1;   // segments holding rows 0 .. nb-2
+    T S = S0, SS = SS0;
+    for (int k = 0; k < K; ++k) {
+        const int lo = k * CSEG;
+        if (k > 0 && act) {   // state before segment k
+            ck[(size_t)(2 * k) * NL + L] = S;
+            ck[(size_t)(2 * k + 1) * NL + L] = SS;
+        }
+        T f[CSEG + 2];   // phi(b-4) for b = lo .. lo + CSEG + 1
+#pragma unroll
+        for (int r = 0; r < CSEG + 2; ++r) f[r] = ldq(lo - 4 + r);
+#pragma unroll
+        for (int r = 0; r < CSEG; ++r) {
+            const int b = lo + r;
+            if (b >= 1 && b <= nb - 2) frow(b, f[r], f[r + 1], f[r + 2], S, SS);
+        }
+    }
+    // closure row nb-1 (reads box points nb-1 .. nb-4) and its 2x2 solve
     {
         const int b = nb - 1;
-        // one-sided closure of the last row reads box points nb-1 .. nb-4
-        T e0 = ldq(b - 3), e1 = ldq(b - 4), e2 = ldq(b - 5), e3 = ldq(b - 6);
+        const T e0 = ldq(b - 3), e1 = ldq(b - 4), e2 = ldq(b - 5), e3 = ldq(b - 6);
         T s = -(T(-3.5) * e0 + T(4) * e1 - T(0.5) * e2) * idx;
         T ss = (T(34.0 / 3.0) * e0 - T(83.0 / 4.0) * e1 + T(10) * e2 - T(7.0 / 12.0) * e3) * idx2;
         s = s - (A[3 * b] * S + B[3 * b] * SS);
         ss = ss - (AA[3 * b] * S + BB[3 * b] * SS);
         const T iden = IDEN[b];
-        T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
-        T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
-        S = solS; SS = solSS;
-        if (b - 3 >= 0 && b - 3 < n) emit(b - 3, S);
+        const T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
+        const T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
+        S = solS;
+        SS = solSS;
     }
-    for (int b = nb - 2; b >= 0; --b) {
-        T s = T(0), ss = T(0);
-        if (store) {
-            size_t o = (size_t)b * 2 * nlines + L;
-            s = scr[o];
-            ss = scr[o + nlines];
-        }
-        s = s - (A[3 * b + 2] * S + B[3 * b + 2] * SS);
-        ss = ss - (AA[3 * b + 2] * S + BB[3 * b + 2] * SS);
-        const T iden = IDEN[b];
-        T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
-        T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
-        S = solS; SS = solSS;
-        if (b - 3 >= 0 && b - 3 < n) emit(b - 3, S);
-    }
-}
-
-// Lines along y or z (lanes on consecutive i).  FINAL = true fuses the mass
-// correction's reductions (mass_preserving_level_set.f90:20-44 and ls_mv
-// ns_solver/others.f90:191-205): |grad| from (nx, ny[, nz]), H, delta, and
-// per-block partial sums of vol, mass and the lambda denominator.
-
-template <typename T, int DIM, bool FINAL>
-__global__ __launch_bounds__(256) void k_ccd_strided(const T *__restrict__ phi, const T *__restrict__ coef,
-                                                      T *__restrict__ scr, T *__restrict__ out,
-                                                      const T *__restrict__ dxv, const T *__restrict__ dyv,
-                                                      T *__restrict__ gr, double *__restrict__ partial, int nx,
-                                                      int nlines, int n, long ostride, long estride, CcdArgs ca)
-{
-    const int L = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = L < nlines;
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (act) {
-        const size_t base = (size_t)(L / nx) * (size_t)ostride + (size_t)(L % nx);
-        auto ldq = [&](int q) -> T { return phi[base + (size_t)clampi(q, 0, n - 1) * (size_t)estride]; };
-        auto emit = [&](int q, T d) {
-            const size_t a = base + (size_t)q * (size_t)estride;
-            if (!FINAL) {
-                out[a] = d;
-            } else {
-                T g;
-                if (DIM == 3) g = sqrt(sq(dxv[a]) + sq(dyv[a]) + sq(d));
-                else g = sqrt(sq(dxv[a]) + sq(d));
-                gr[a] = g;
-                // the mass integral is formed in FP64 in both forms
-                double h, dd;
-                ls_funs<double>((double)phi[a], ca.ls_wid, h, dd);
-                const double gg = (double)g;
-                const double r12 = ca.rho_12;
-                double rho, plam;
-                acc[0] += h * ca.dv;
-                if (!ca.inverse) rho = h + r12 * (1.0 - h);
-                else rho = r12 * h + (1.0 - h);
-                acc[1] += rho * h * ca.dv;
-                plam = dd * dd * gg;
-                if (ca.inverse) plam = plam * (2.0 * (r12 - 1.0) * h + 1.0) * ca.dx * ca.dy * ca.dz;
-                else plam = plam * (2.0 * (1.0 - r12) * h + r12) * ca.dx * ca.dy * ca.dz;
-                acc[2] += plam;
+    // backward substitution, segment by segment from the top
+    for (int k = K - 1; k >= 0; --k) {
+        const int lo = k * CSEG, hi = min(lo + CSEG - 1, nb - 2);
+        T f[CSEG + 2], dxa[CSEG], dya[CSEG], pa[CSEG];
+#pragma unroll
+        for (int r = 0; r < CSEG + 2; ++r) f[r] = ldq(lo - 4 + r);
+        if (FINAL) {
+#pragma unroll
+            for (int r = 0; r < CSEG; ++r) {   // operands of the emitted point q = b - 3
+                const size_t a = base + (size_t)clampi(lo + r - 3, 0, n - 1) * (size_t)estride;
+                dxa[r] = dxv[a];
+                dya[r] = DIM == 3 ? dyv[a] : T(0);
+                pa[r] = phi[a];
             }
-        };
-        ccd_line<T>(ldq, emit, coef, scr, (size_t)nlines, (size_t)L, n, (T)ca.dx, true);
+        }
+        T fs[CSEG], fss[CSEG];   // forward states of the segment's rows
+        T cs = S0, css = SS0;
+        if (k > 0) {
+            cs = ck[(size_t)(2 * k) * NL + Lc];
+            css = ck[(size_t)(2 * k + 1) * NL + Lc];
+        }
+#pragma unroll
+        for (int r = 0; r < CSEG; ++r) {
+            const int b = lo + r;
+            if (b >= 1 && b <= hi) frow(b, f[r], f[r + 1], f[r + 2], cs, css);
+            fs[r] = cs;
+            fss[r] = css;
+        }
+#pragma unroll
+        for (int r = CSEG - 1; r >= 0; --r) {
+            const int b = lo + r;
+            if (b <= hi) {
+                const T s = fs[r] - (A[3 * b + 2] * S + B[3 * b + 2] * SS);
+                const T ss = fss[r] - (AA[3 * b + 2] * S + BB[3 * b + 2] * SS);
+                const T iden = IDEN[b];
+                const T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
+                const T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
+                S = solS;
+                SS = solSS;
+                if (act && b - 3 >= 0 && b - 3 < n) emit(b - 3, S, dxa[r], dya[r], pa[r]);
+            }
+        }
     }
     if (FINAL) {
         block_sum<3>(acc);
@@ -888,111 +937,6 @@ __global__ __launch_bounds__(256) void k_ccd_strided(const T *__restrict__ phi, 
             partial[3 * blockIdx.x + 2] = acc[2];
         }
     }
-}
-
-// Lines along x through an LDS tile (first direction, writes d phi / dx).
-template <typename T>
-__global__ __launch_bounds__(64) void k_ccd_x(const T *__restrict__ phi, const T *__restrict__ coef,
-                                               T *__restrict__ scr, T *__restrict__ out, int nx, int ny,
-                                               int nlines, T dx)
-{
-    __shared__ T tp[64][XCH + 1];
-    __shared__ T to[64][XCH + 1];
-    const int t = threadIdx.x;
-    const int L0 = blockIdx.x * 64;
-    const int L = L0 + t;
-    const bool act = L < nlines;
-    const int n = nx;
-    auto base_of = [&](int l) -> size_t { return (size_t)(l / ny) * (size_t)nx * ny + (size_t)(l % ny) * nx; };
-    int cur = -1;
-    auto load_chunk = [&](int ch) {
-        __syncthreads();
-        for (int idx = t; idx < 64 * XCH; idx += 64) {
-            int b = idx / XCH, e = idx % XCH;
-            int lb = L0 + b, q = ch * XCH + e;
-            if (lb < nlines && q < n) tp[b][e] = phi[base_of(lb) + q];
-        }
-        __syncthreads();
-    };
-    // ldq is called with non-decreasing clamped q in the forward sweep; the
-    // closure of the last row re-reads the line end from global memory.
-    const size_t base = act ? base_of(L) : 0;
-    bool tail = false;
-    auto ldq = [&](int q) -> T {
-        q = clampi(q, 0, n - 1);
-        if (tail) return act ? phi[base + q] : T(0);
-        int ch = q / XCH;
-        if (ch != cur) { load_chunk(ch); cur = ch; }
-        return tp[t][q % XCH];
-    };
-    int och = (n - 1) / XCH;
-    auto flush = [&](int ch) {
-        __syncthreads();
-        for (int idx = t; idx < 64 * XCH; idx += 64) {
-            int b = idx / XCH, e = idx % XCH;
-            int lb = L0 + b, q = ch * XCH + e;
-            if (lb < nlines && q < n) out[base_of(lb) + q] = to[b][e];
-        }
-        __syncthreads();
-    };
-    auto emit = [&](int q, T d) {
-        int ch = q / XCH;
-        if (ch != och) { flush(och); och = ch; }
-        to[t][q % XCH] = d;
-    };
-    // the forward part of ccd_line, inlined here so that the tail reload can
-    // switch to global memory
-    const int nb = n + 6;
-    const T *A = coef, *B = coef + 3 * nb, *AA = coef + 6 * nb, *BB = coef + 9 * nb, *IDEN = coef + 12 * nb;
-    const T idx = T(1) / dx, idx2 = T(1) / (dx * dx);
-    const size_t nl = (size_t)nlines;
-    T fm1, f0, fp1, S, SS;
-    {
-        T a0 = ldq(-3), a1 = ldq(-2), a2 = ldq(-1), a3 = ldq(0);
-        S = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) * idx;
-        SS = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) * idx2;
-        fm1 = a0; f0 = a1; fp1 = a2;
-    }
-    if (act) { scr[L] = S; scr[nl + L] = SS; }
-    for (int b = 1; b <= nb - 2; ++b) {
-        T s = T(15.0 / 16.0) * (-fm1 + fp1) * idx;
-        T ss = (T(3) * fm1 - T(6) * f0 + T(3) * fp1) * idx2;
-        s = s - (A[3 * b] * S + B[3 * b] * SS);
-        ss = ss - (AA[3 * b] * S + BB[3 * b] * SS);
-        S = s; SS = ss;
-        if (act) {
-            size_t o = (size_t)b * 2 * nl + L;
-            scr[o] = S; scr[o + nl] = SS;
-        }
-        fm1 = f0; f0 = fp1; fp1 = ldq(b - 1);
-    }
-    tail = true;
-    {
-        const int b = nb - 1;
-        T e0 = ldq(b - 3), e1 = ldq(b - 4), e2 = ldq(b - 5), e3 = ldq(b - 6);
-        T s = -(T(-3.5) * e0 + T(4) * e1 - T(0.5) * e2) * idx;
-        T ss = (T(34.0 / 3.0) * e0 - T(83.0 / 4.0) * e1 + T(10) * e2 - T(7.0 / 12.0) * e3) * idx2;
-        s = s - (A[3 * b] * S + B[3 * b] * SS);
-        ss = ss - (AA[3 * b] * S + BB[3 * b] * SS);
-        const T iden = IDEN[b];
-        S = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
-        SS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
-    }
-    for (int b = nb - 2; b >= 0; --b) {
-        T s = T(0), ss = T(0);
-        if (act) {
-            size_t o = (size_t)b * 2 * nl + L;
-            s = scr[o]; ss = scr[o + nl];
-        }
-        s = s - (A[3 * b + 2] * S + B[3 * b + 2] * SS);
-        ss = ss - (AA[3 * b + 2] * S + BB[3 * b + 2] * SS);
-        const T iden = IDEN[b];
-        T solS = -(B[3 * b + 1] * ss - BB[3 * b + 1] * s) * iden;
-        T solSS = (A[3 * b + 1] * ss - AA[3 * b + 1] * s) * iden;
-        S = solS; SS = solSS;
-        if (b - 3 >= 0 && b - 3 < n) emit(b - 3, S);
-    }
-    flush(och);
 }
 
 // Deterministic final reduction of per-block partials (fixed order), then the
@@ -1193,28 +1137,30 @@ void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T 
                      const T *dxv, const T *dyv, T *gr, double *partial, int nx, int ny, int nz, const CcdArgs &ca,
                      int *nblocks_out)
 {
-    if (dir == 0) {
-        const int nlines = ny * nz;
-        hipLaunchKernelGGL((k_ccd_x<T>), dim3(nblk(nlines, 64)), dim3(64), 0, st, phi, coef, scr, out, nx, ny,
-                           nlines, (T)ca.dx);
-        if (nblocks_out) *nblocks_out = 0;
-        return;
-    }
-    const int n = dir == 1 ? ny : nz;
-    const int nlines = dir == 1 ? nx * nz : nx * ny;
-    const long ostride = dir == 1 ? (long)nx * ny : (long)nx;
-    const long estride = dir == 1 ? (long)nx : (long)nx * ny;
+    const int n = dir == 0 ? nx : dir == 1 ? ny : nz;
+    const int nlines = dir == 0 ? ny * nz : dir == 1 ? nx * nz : nx * ny;
+    const int lw = dir == 0 ? nlines : nx;
+    const long ostride = dir == 0 ? 0 : dir == 1 ? (long)nx * ny : (long)nx;
+    const long istride = dir == 0 ? (long)nx : 1;
+    const long estride = dir == 0 ? 1 : dir == 1 ? (long)nx : (long)nx * ny;
     const dim3 g(nblk(nlines, 256)), b(256);
-    if (!final_pass)
-        hipLaunchKernelGGL((k_ccd_strided<T, 3, false>), g, b, 0, st, phi, coef, scr, out, dxv, dyv, gr, partial, nx,
-                           nlines, n, ostride, estride, ca);
-    else if (dim == 3)
-        hipLaunchKernelGGL((k_ccd_strided<T, 3, true>), g, b, 0, st, phi, coef, scr, out, dxv, dyv, gr, partial, nx,
-                           nlines, n, ostride, estride, ca);
-    else
-        hipLaunchKernelGGL((k_ccd_strided<T, 2, true>), g, b, 0, st, phi, coef, scr, out, dxv, dyv, gr, partial, nx,
-                           nlines, n, ostride, estride, ca);
-    if (nblocks_out) *nblocks_out = (int)g.x;
+    const size_t tab = (size_t)13 * (n + 6) * sizeof(T);
+    const bool lds = tab <= 64 * 1024;
+    const size_t shm = lds ? tab : 0;
+#define LSMP_CCD(D, F)                                                                                         \
+    do {                                                                                                       \
+        if (lds)                                                                                               \
+            hipLaunchKernelGGL((k_ccd_seg<T, D, F, true>), g, b, shm, st, phi, coef, scr, out, dxv, dyv, gr,     \
+                               partial, nlines, n, lw, ostride, istride, estride, ca);                         \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_ccd_seg<T, D, F, false>), g, b, 0, st, phi, coef, scr, out, dxv, dyv, gr,      \
+                               partial, nlines, n, lw, ostride, istride, estride, ca);                         \
+    } while (0)
+    if (!final_pass) LSMP_CCD(3, false);
+    else if (dim == 3) LSMP_CCD(3, true);
+    else LSMP_CCD(2, true);
+#undef LSMP_CCD
+    if (nblocks_out) *nblocks_out = final_pass ? (int)g.x : 0;
 }
 
 void launch_finalize3(hipStream_t st, const double *partial, int nblk_, double imass, double *scal)
