@@ -30,6 +30,7 @@ struct lsmp_ctx {
     void *dxv = nullptr, *dyv = nullptr, *gr = nullptr, *scr = nullptr;
     void *coef[3] = {nullptr, nullptr, nullptr};
     double *partial = nullptr, *scal = nullptr;
+    double *vtab = nullptr;   // per-axis sine tables of the analytic velocity
     unsigned long long *u64 = nullptr;
     double *hbuf = nullptr;   // pinned readback
     int npartial = 0;

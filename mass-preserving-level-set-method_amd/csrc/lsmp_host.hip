@@ -457,13 +457,13 @@ template <typename T> int velocity_analytic(lsmp_ctx *c, int field, double t, do
     va.dx = c->p.dx; va.dy = c->p.dy; va.dz = c->p.dz;
     va.ct = std::cos(pi * t / t2s);
     va.omega = 2.0 * pi / t2s;
-    KT(c, "velocity", (launch_velocity<T>(c->st, (T *)c->u, (T *)c->v, (T *)c->w, va)));
+    KT(c, "velocity", (launch_velocity<T>(c->st, (T *)c->u, (T *)c->v, (T *)c->w, va, c->vtab)));
     return LSMP_OK;
 }
 
 void free_all(lsmp_ctx *c)
 {
-    void *bufs[] = {c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
+    void *bufs[] = {c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -540,6 +540,7 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
         sc = std::max(sc, (size_t)2 * segs * lanes);
     }
     if (alloc(&c->scr, sc * c->esz)) return fail(LSMP_E_HIP);
+    if (alloc((void **)&c->vtab, (size_t)2 * (c->nx + c->ny + c->nz) * sizeof(double))) return fail(LSMP_E_HIP);
     for (int d = 0; d < c->dim; ++d) {
         std::vector<double> tab;
         ccd_tables(ns[d] + 2 * GHC, params->dx, tab);   // all three solvers use dx (ccd_solvers.f90:406-408)

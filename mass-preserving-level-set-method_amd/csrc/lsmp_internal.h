@@ -54,7 +54,7 @@ template <typename T>
 void launch_mass_vol(hipStream_t st, const T *phi, size_t n, const CcdArgs &ca, double *partial, int nblocks);
 template <typename T>
 void launch_l1(hipStream_t st, const T *phi, const T *phi0, size_t n, double ls_wid, double *partial, int nblocks);
-template <typename T> void launch_velocity(hipStream_t st, T *u, T *v, T *w, const VelArgs &va);
+template <typename T> void launch_velocity(hipStream_t st, T *u, T *v, T *w, const VelArgs &va, double *tab);
 template <typename TO, typename TI> void launch_convert(hipStream_t st, TO *out, const TI *in, size_t n);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
 

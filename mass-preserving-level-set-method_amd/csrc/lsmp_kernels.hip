@@ -1026,31 +1026,52 @@ __global__ void k_l1(const T *__restrict__ phi, const T *__restrict__ phi0, size
 // (Vortex/lib/data/tree.f90:254-256).
 
 template <typename T>
-__global__ void k_velocity(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, VelArgs va)
+__global__ void k_velocity(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, VelArgs va,
+                           const double *__restrict__ tab)
 {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = (size_t)va.nx * va.ny * va.nz;
     if (q >= n) return;
     const int i = (int)(q % va.nx), j = (int)((q / va.nx) % va.ny), k = (int)(q / ((size_t)va.nx * va.ny));
-    const double pi = 3.141592653589793238462643383279502884;
-    const double x = va.xstart + (double)((float)(i + 1) - 0.5f) * va.dx;
-    const double y = va.ystart + (double)((float)(j + 1) - 0.5f) * va.dy;
-    const double z = va.zstart + (double)((float)(k + 1) - 0.5f) * va.dz;
     double uu, vv, ww = 0.0;
     if (va.field == 0) {
+        const double x = va.xstart + (double)((float)(i + 1) - 0.5f) * va.dx;
+        const double y = va.ystart + (double)((float)(j + 1) - 0.5f) * va.dy;
         uu = -va.omega * (y - 0.5);
         vv = va.omega * (x - 0.5);
-    } else if (va.field == 1) {
-        uu = -sq(sin(pi * x)) * sin(2.0 * pi * y) * va.ct;
-        vv = sq(sin(pi * y)) * sin(2.0 * pi * x) * va.ct;
     } else {
-        uu = 2.0 * sq(sin(pi * x)) * sin(2.0 * pi * y) * sin(2.0 * pi * z) * va.ct;
-        vv = -sin(2.0 * pi * x) * sq(sin(pi * y)) * sin(2.0 * pi * z) * va.ct;
-        ww = -sin(2.0 * pi * x) * sin(2.0 * pi * y) * sq(sin(pi * z)) * va.ct;
+        // sin(pi c) and sin(2 pi c) of each axis from the tables of k_vel_tables
+        const double *tx = tab, *ty = tab + 2 * va.nx, *tz = ty + 2 * va.ny;
+        const double sx = tx[2 * i], s2x = tx[2 * i + 1], sy = ty[2 * j], s2y = ty[2 * j + 1];
+        if (va.field == 1) {
+            uu = -sq(sx) * s2y * va.ct;
+            vv = sq(sy) * s2x * va.ct;
+        } else {
+            const double sz = tz[2 * k], s2z = tz[2 * k + 1];
+            uu = 2.0 * sq(sx) * s2y * s2z * va.ct;
+            vv = -s2x * sq(sy) * s2z * va.ct;
+            ww = -s2x * s2y * sq(sz) * va.ct;
+        }
     }
     u[q] = (T)uu;
     v[q] = (T)vv;
     w[q] = (T)ww;
+}
+
+// sin(pi c) and sin(2 pi c) at the cell centres c of each axis
+// (velocity.f90 evaluates them per cell; the products are formed in the same order).
+__global__ void k_vel_tables(VelArgs va, double *__restrict__ tab)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = va.nx + va.ny + (va.field == 2 ? va.nz : 0);
+    if (t >= nt) return;
+    const double pi = 3.141592653589793238462643383279502884;
+    double c;
+    if (t < va.nx) c = va.xstart + (double)((float)(t + 1) - 0.5f) * va.dx;
+    else if (t < va.nx + va.ny) c = va.ystart + (double)((float)(t - va.nx + 1) - 0.5f) * va.dy;
+    else c = va.zstart + (double)((float)(t - va.nx - va.ny + 1) - 0.5f) * va.dz;
+    tab[2 * t] = sin(pi * c);
+    tab[2 * t + 1] = sin(2.0 * pi * c);
 }
 
 template <typename TO, typename TI>
@@ -1187,10 +1208,12 @@ void launch_l1(hipStream_t st, const T *phi, const T *phi0, size_t n, double ls_
 }
 
 template <typename T>
-void launch_velocity(hipStream_t st, T *u, T *v, T *w, const VelArgs &va)
+void launch_velocity(hipStream_t st, T *u, T *v, T *w, const VelArgs &va, double *tab)
 {
     const size_t n = (size_t)va.nx * va.ny * va.nz;
-    hipLaunchKernelGGL((k_velocity<T>), dim3(nblk(n, 256)), dim3(256), 0, st, u, v, w, va);
+    if (va.field != 0)
+        hipLaunchKernelGGL(k_vel_tables, dim3(nblk((size_t)va.nx + va.ny + va.nz, 256)), dim3(256), 0, st, va, tab);
+    hipLaunchKernelGGL((k_velocity<T>), dim3(nblk(n, 256)), dim3(256), 0, st, u, v, w, va, tab);
 }
 
 template <typename TO, typename TI>
@@ -1218,7 +1241,7 @@ void launch_zero_u64(hipStream_t st, unsigned long long *p, int n)
     template void launch_mass_update<T>(hipStream_t, T *, const T *, size_t, T, const double *);               \
     template void launch_mass_vol<T>(hipStream_t, const T *, size_t, const CcdArgs &, double *, int);           \
     template void launch_l1<T>(hipStream_t, const T *, const T *, size_t, double, double *, int);              \
-    template void launch_velocity<T>(hipStream_t, T *, T *, T *, const VelArgs &);
+    template void launch_velocity<T>(hipStream_t, T *, T *, T *, const VelArgs &, double *);
 
 LSMP_INST(double)
 LSMP_INST(float)
