@@ -91,9 +91,17 @@ int lsmp_set_velocity(lsmp_handle h, const double *u_box, const double *v_box, c
  * time t (velocity.f90; main.f90:23-28 evaluates it at t^{n+1}). */
 int lsmp_set_velocity_analytic(lsmp_handle h, int field, double t, double t2s);
 
-/* One TVD-RK3 CRWENO5 advection step with step dt
- * (level_set_rk3_solver.f90:1-221). */
+/* One TVD-RK3 advection step with step dt (level_set_rk3_solver.f90:1-94),
+ * with the RHS of the selected scheme: CRWENO5 flux split (default,
+ * :104-221) or upwind CCD (:223-286). */
 int lsmp_rk3_advect(lsmp_handle h, double dt);
+/* The reference's level_set_rk3_source selector (:96-102), a source edit
+ * there: LSMP_SCHEME_CRWENO (source_weno) or LSMP_SCHEME_UCCD (source_ccd,
+ * -u . grad phi with the "uccd" solver of ccd_solvers.f90:165-214, 312-345,
+ * 428-438). */
+#define LSMP_SCHEME_CRWENO 0
+#define LSMP_SCHEME_UCCD 1
+int lsmp_set_advection_scheme(lsmp_handle h, int scheme);
 
 /* Reinitialisation toward |grad phi| = 1 (level_set_rk3_redis.f90:1-79).
  * btn = 1: sign from the smoothed Heaviside, constrained lambda; btn = 0:

@@ -29,6 +29,8 @@ struct lsmp_ctx {
     void *R = nullptr, *L1 = nullptr, *L2 = nullptr, *S0 = nullptr, *phi0 = nullptr;
     void *dxv = nullptr, *dyv = nullptr, *gr = nullptr, *scr = nullptr;
     void *coef[3] = {nullptr, nullptr, nullptr};
+    void *ucoef[3] = {nullptr, nullptr, nullptr};   // UCCD tables (upwind then downwind) per direction
+    int scheme = LSMP_SCHEME_CRWENO;
     double *partial = nullptr, *scal = nullptr;
     double *vtab = nullptr;   // per-axis sine tables of the analytic velocity
     unsigned long long *u64 = nullptr;

@@ -140,7 +140,10 @@ int check_params(const lsmp_params *p)
 // 2x2 block elimination of TWIN_DEC (matrix_solver.f90:6-45), for a box line
 // of nb points with spacing dx.  Output: A, B, AA, BB, each (3, nb), then the
 // nb reciprocal pivot determinants.
-void ccd_tables(int nb, double dx, std::vector<double> &out)
+// kind 0: CCD (init_ccd, ccd_solvers.f90:136-163); 1 / 2: the upwind / downwind
+// systems of UCCD (init_uccd, :165-214).  All share the boundary rows of bc
+// (:80-134); the tables are factorised by TWIN_DEC (matrix_solver.f90:6-45).
+void ccd_tables(int nb, double dx, std::vector<double> &out, int kind = 0)
 {
     out.assign((size_t)12 * nb, 0.0);
     double *A = out.data(), *B = A + 3 * nb, *AA = A + 6 * nb, *BB = A + 9 * nb;
@@ -150,9 +153,18 @@ void ccd_tables(int nb, double dx, std::vector<double> &out)
     M(A, 2, is) = 2.0;  M(B, 2, is) = -dx;  M(AA, 2, is) = -2.5 / dx;  M(BB, 2, is) = 8.5;
     M(A, 1, ie) = 1.0;  M(B, 1, ie) = 0.0;  M(AA, 1, ie) = 0.0;  M(BB, 1, ie) = 1.0;
     M(A, 0, ie) = 2.0;  M(B, 0, ie) = dx;   M(AA, 0, ie) = 2.5 / dx;  M(BB, 0, ie) = 8.5;
+    const double A1 = 0.875, B1 = 0.1251282341599089, B2 = -0.2487176584009104, B3 = 0.0001282341599089;
     for (int i = is + 1; i <= ie - 1; ++i) {
-        M(A, 0, i) = 7.0 / 16.0;  M(A, 1, i) = 1.0;  M(A, 2, i) = 7.0 / 16.0;
-        M(B, 0, i) = dx / 16.0;   M(B, 1, i) = 0.0;  M(B, 2, i) = -dx / 16.0;
+        if (kind == 0) {
+            M(A, 0, i) = 7.0 / 16.0;  M(A, 1, i) = 1.0;  M(A, 2, i) = 7.0 / 16.0;
+            M(B, 0, i) = dx / 16.0;   M(B, 1, i) = 0.0;  M(B, 2, i) = -dx / 16.0;
+        } else if (kind == 1) {
+            M(A, 0, i) = A1;       M(A, 1, i) = 1.0;      M(A, 2, i) = 0.0;
+            M(B, 0, i) = B1 * dx;  M(B, 1, i) = B2 * dx;  M(B, 2, i) = B3 * dx;
+        } else {
+            M(A, 0, i) = 0.0;       M(A, 1, i) = 1.0;       M(A, 2, i) = A1;
+            M(B, 0, i) = -B3 * dx;  M(B, 1, i) = -B2 * dx;  M(B, 2, i) = -B1 * dx;
+        }
         M(AA, 0, i) = -(9.0 / 8.0 / dx);  M(AA, 1, i) = 0.0;  M(AA, 2, i) = 9.0 / 8.0 / dx;
         M(BB, 0, i) = -1.0 / 8.0;  M(BB, 1, i) = 1.0;  M(BB, 2, i) = -1.0 / 8.0;
     }
@@ -259,10 +271,30 @@ int alloc(void **p, size_t bytes)
 // ------------------------------------------------------------------ phases
 template <typename T> int advect(lsmp_ctx *c, double dt)
 {
-    T *phi = (T *)c->phi, *R = (T *)c->R, *L1 = (T *)c->L1, *L2 = (T *)c->L2, *scr = (T *)c->scr;
+    T *R = (T *)c->R, *L1 = (T *)c->L1, *L2 = (T *)c->L2, *scr = (T *)c->scr;
     const T *u = (const T *)c->u, *v = (const T *)c->v, *w = (const T *)c->w;
     const T dx = (T)c->p.dx, dy = (T)c->p.dy, dz = (T)c->p.dz, tdt = (T)dt;
     for (int stage = 1; stage <= 3; ++stage) {
+        T *phi = (T *)c->phi;
+        if (c->scheme == LSMP_SCHEME_UCCD) {
+            // level_set_rk3_source_ccd: every solver uses dx (ccd_solvers.f90:406-408);
+            // the stage result goes to phiB, which then becomes phi
+            T *out = (T *)c->phiB;
+            const T *cu[3] = {(const T *)c->ucoef[0], (const T *)c->ucoef[1], (const T *)c->ucoef[2]};
+            KT(c, "adv_x", (launch_uccd_pass<T>(c->st, 0, 0, stage, phi, out, u, R, L1, L2, cu[0], scr, c->nx,
+                                                c->ny, c->nz, dx, tdt)));
+            if (c->dim == 3) {
+                KT(c, "adv_y", (launch_uccd_pass<T>(c->st, 1, 1, stage, phi, out, v, R, L1, L2, cu[1], scr, c->nx,
+                                                    c->ny, c->nz, dx, tdt)));
+                KT(c, "adv_z", (launch_uccd_pass<T>(c->st, 2, 2, stage, phi, out, w, R, L1, L2, cu[2], scr, c->nx,
+                                                    c->ny, c->nz, dx, tdt)));
+            } else {
+                KT(c, "adv_y", (launch_uccd_pass<T>(c->st, 1, 2, stage, phi, out, v, R, L1, L2, cu[1], scr, c->nx,
+                                                    c->ny, c->nz, dx, tdt)));
+            }
+            std::swap(c->phi, c->phiB);
+            continue;
+        }
         KT(c, "adv_x", (launch_adv_pass<T>(c->st, 0, 0, stage, phi, u, R, L1, L2, scr, c->nx, c->ny, c->nz, dx, tdt)));
         if (c->dim == 3) {
             KT(c, "adv_y", (launch_adv_pass<T>(c->st, 1, 1, stage, phi, v, R, L1, L2, scr, c->nx, c->ny, c->nz, dy, tdt)));
@@ -463,7 +495,7 @@ template <typename T> int velocity_analytic(lsmp_ctx *c, int field, double t, do
 
 void free_all(lsmp_ctx *c)
 {
-    void *bufs[] = {c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
+    void *bufs[] = {c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -535,9 +567,9 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
     for (int d = 0; d < c->dim; ++d) {
         const size_t nlines = c->ncell / (size_t)ns[d];
         sc = std::max(sc, (size_t)2 * (size_t)(ns[d] - 2) * ((nlines + 31) / 32 * 64));
-        // CCD checkpoints: 2 values per line and CSEG-row segment, lanes padded to 256
+        // CCD / UCCD checkpoints per line and CSEG-row segment, lanes padded to 256
         const size_t lanes = (nlines + 255) / 256 * 256, segs = (size_t)(ns[d] + 4) / CSEG + 1;
-        sc = std::max(sc, (size_t)2 * segs * lanes);
+        sc = std::max(sc, (size_t)4 * segs * lanes);   // UCCD keeps 4 (CCD 2)
     }
     if (alloc(&c->scr, sc * c->esz)) return fail(LSMP_E_HIP);
     if (alloc((void **)&c->vtab, (size_t)2 * (c->nx + c->ny + c->nz) * sizeof(double))) return fail(LSMP_E_HIP);
@@ -700,6 +732,32 @@ int lsmp_synchronize(lsmp_handle h)
 {
     if (!h) return LSMP_E_INVALID;
     LSMP_HIP(hipStreamSynchronize(h->st));
+    return LSMP_OK;
+}
+
+int lsmp_set_advection_scheme(lsmp_handle h, int scheme)
+{
+    if (!h || (scheme != LSMP_SCHEME_CRWENO && scheme != LSMP_SCHEME_UCCD)) return LSMP_E_INVALID;
+    if (scheme == LSMP_SCHEME_UCCD && !h->ucoef[0]) {
+        const int ns[3] = {h->nx, h->ny, h->nz};
+        for (int d = 0; d < h->dim; ++d) {
+            const int nb = ns[d] + 2 * GHC;
+            std::vector<double> all;
+            for (int kind = 1; kind <= 2; ++kind) {
+                std::vector<double> tab;
+                ccd_tables(nb, h->p.dx, tab, kind);   // every UCCD solver uses dx (ccd_solvers.f90:406-408)
+                all.insert(all.end(), tab.begin(), tab.end());
+            }
+            LSMP_HIP(hipMalloc(&h->ucoef[d], all.size() * h->esz));
+            if (h->fp32) {
+                std::vector<float> tf(all.begin(), all.end());
+                LSMP_HIP(hipMemcpy(h->ucoef[d], tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+            } else {
+                LSMP_HIP(hipMemcpy(h->ucoef[d], all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+            }
+        }
+    }
+    h->scheme = scheme;
     return LSMP_OK;
 }
 

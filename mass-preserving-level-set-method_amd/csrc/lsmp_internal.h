@@ -55,6 +55,9 @@ void launch_mass_vol(hipStream_t st, const T *phi, size_t n, const CcdArgs &ca, 
 template <typename T>
 void launch_l1(hipStream_t st, const T *phi, const T *phi0, size_t n, double ls_wid, double *partial, int nblocks);
 template <typename T> void launch_velocity(hipStream_t st, T *u, T *v, T *w, const VelArgs &va, double *tab);
+template <typename T>
+void launch_uccd_pass(hipStream_t st, int dir, int mode, int stage, const T *phi, T *phi_out, const T *vel, T *R,
+                      T *L1, T *L2, const T *coef, T *ck, int nx, int ny, int nz, T dx, T dt);
 template <typename TO, typename TI> void launch_convert(hipStream_t st, TO *out, const TI *in, size_t n);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
 

@@ -939,6 +939,173 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
     }
 }
 
+// ---------------------------------------------------------------------------
+// UCCD advection (level_set_rk3_source_ccd, level_set_rk3_solver.f90:223-286;
+// root_solve "uccd", ccd_solvers.f90:428-438): per line, the upwind (U) and
+// downwind (D) block systems of asgn_src_uccd (:312-345) with their TWIN_DEC
+// tables (coef: U tables, then D tables, 13 nb each), solved like
+// k_ccd_seg (checkpoints of both states every CSEG rows: ck[(4 k + comp) * NL
+// + L]); the emitted derivative at an interior point is U's for u > 0 and D's
+// otherwise.  The RHS accumulates -u phi_x (MODE 0), - v phi_y (MODE 1) and,
+// in the last direction (MODE 2), - w phi_z followed by the RK stage of
+// level_set_rk3_solver.f90:16-88, written to phi_out: the segment recompute
+// re-reads phi, so the stage cannot update it in place.
+// ---------------------------------------------------------------------------
+template <typename T, int MODE, int STAGE, bool TLDS>
+__global__ __launch_bounds__(256) void k_uccd_seg(const T *__restrict__ phi, T *__restrict__ phi_out,
+                                                   const T *__restrict__ vel, T *R, T *L1, T *L2,
+                                                   const T *__restrict__ coef, T *__restrict__ ck, int nlines,
+                                                   int n, int lw, long ostride, long istride, long estride, T dx,
+                                                   T dt)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char uccd_lds[];
+    const int nb = n + 6;
+    const T *tab = coef;
+    if (TLDS) {
+        T *t = reinterpret_cast<T *>(uccd_lds);
+        for (int i = threadIdx.x; i < 26 * nb; i += blockDim.x) t[i] = coef[i];
+        __syncthreads();
+        tab = t;
+    }
+    const T *AU = tab, *BU = tab + 3 * nb, *AAU = tab + 6 * nb, *BBU = tab + 9 * nb, *IU = tab + 12 * nb;
+    const T *AD = tab + 13 * nb, *BD = AD + 3 * nb, *AAD = AD + 6 * nb, *BBD = AD + 9 * nb, *ID = AD + 12 * nb;
+    const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = L < nlines;
+    const int Lc = act ? L : nlines - 1;
+    const size_t base = (size_t)(Lc / lw) * (size_t)ostride + (size_t)(Lc % lw) * (size_t)istride;
+    const size_t NL = (size_t)gridDim.x * blockDim.x;
+    const T idx = T(1) / dx, idx2 = T(1) / (dx * dx);
+    const T c3 = T(-0.06096119008109), c2 = T(1.99692238016218), c1 = T(-1.93596119008109);
+    auto ldq = [&](int q) -> T { return phi[base + (size_t)clampi(q, 0, n - 1) * (size_t)estride]; };
+    // forward row b of both systems from f(b-1), f(b), f(b+1) = phi(b-4), phi(b-3), phi(b-2)
+    auto frow = [&](int b, T fm1, T f0, T fp1, T &SU, T &SSU, T &SD, T &SSD) {
+        const T ss0 = (T(3) * fm1 - T(6) * f0 + T(3) * fp1) * idx2;
+        T su = (c1 * fm1 + c2 * f0 + c3 * fp1) * idx;
+        T sd = -(c3 * fm1 + c2 * f0 + c1 * fp1) * idx;
+        su = su - (AU[3 * b] * SU + BU[3 * b] * SSU);
+        const T ssu = ss0 - (AAU[3 * b] * SU + BBU[3 * b] * SSU);
+        sd = sd - (AD[3 * b] * SD + BD[3 * b] * SSD);
+        const T ssd = ss0 - (AAD[3 * b] * SD + BBD[3 * b] * SSD);
+        SU = su; SSU = ssu; SD = sd; SSD = ssd;
+    };
+    auto emit = [&](int q, T d1, T uq, T Rv, T Pv, T L1v, T L2v) {
+        const size_t a = base + (size_t)q * (size_t)estride;
+        if (MODE == 0) {
+            R[a] = -uq * d1;
+        } else if (MODE == 1) {
+            R[a] = Rv - uq * d1;
+        } else {
+            const T r = Rv - uq * d1;
+            if (STAGE == 1) {
+                L1[a] = r;
+                phi_out[a] = Pv + r * dt;
+            } else if (STAGE == 2) {
+                L2[a] = r;
+                phi_out[a] = Pv + (T(-3) * L1v + r) / T(4) * dt;
+            } else {
+                phi_out[a] = Pv + (-L1v - L2v + T(8) * r) / T(12) * dt;
+            }
+        }
+    };
+
+    // row 0: the one-sided closure, the same for both systems
+    T S0, SS0;
+    {
+        const T a0 = ldq(-3), a1 = ldq(-2), a2 = ldq(-1), a3 = ldq(0);
+        S0 = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) * idx;
+        SS0 = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) * idx2;
+    }
+    const int K = (nb - 2) / CSEG + 1;
+    T SU = S0, SSU = SS0, SD = S0, SSD = SS0;
+    for (int k = 0; k < K; ++k) {
+        const int lo = k * CSEG;
+        if (k > 0 && act) {
+            ck[(size_t)(4 * k) * NL + L] = SU;
+            ck[(size_t)(4 * k + 1) * NL + L] = SSU;
+            ck[(size_t)(4 * k + 2) * NL + L] = SD;
+            ck[(size_t)(4 * k + 3) * NL + L] = SSD;
+        }
+        T f[CSEG + 2];
+#pragma unroll
+        for (int r = 0; r < CSEG + 2; ++r) f[r] = ldq(lo - 4 + r);
+#pragma unroll
+        for (int r = 0; r < CSEG; ++r) {
+            const int b = lo + r;
+            if (b >= 1 && b <= nb - 2) frow(b, f[r], f[r + 1], f[r + 2], SU, SSU, SD, SSD);
+        }
+    }
+    // closure row nb-1 of both systems and their 2x2 solves
+    {
+        const int b = nb - 1;
+        const T e0 = ldq(b - 3), e1 = ldq(b - 4), e2 = ldq(b - 5), e3 = ldq(b - 6);
+        const T s0 = -(T(-3.5) * e0 + T(4) * e1 - T(0.5) * e2) * idx;
+        const T ss0 = (T(34.0 / 3.0) * e0 - T(83.0 / 4.0) * e1 + T(10) * e2 - T(7.0 / 12.0) * e3) * idx2;
+        {
+            const T s = s0 - (AU[3 * b] * SU + BU[3 * b] * SSU);
+            const T ss = ss0 - (AAU[3 * b] * SU + BBU[3 * b] * SSU);
+            SU = -(BU[3 * b + 1] * ss - BBU[3 * b + 1] * s) * IU[b];
+            SSU = (AU[3 * b + 1] * ss - AAU[3 * b + 1] * s) * IU[b];
+        }
+        {
+            const T s = s0 - (AD[3 * b] * SD + BD[3 * b] * SSD);
+            const T ss = ss0 - (AAD[3 * b] * SD + BBD[3 * b] * SSD);
+            SD = -(BD[3 * b + 1] * ss - BBD[3 * b + 1] * s) * ID[b];
+            SSD = (AD[3 * b + 1] * ss - AAD[3 * b + 1] * s) * ID[b];
+        }
+    }
+    for (int k = K - 1; k >= 0; --k) {
+        const int lo = k * CSEG, hi = min(lo + CSEG - 1, nb - 2);
+        T f[CSEG + 2], uq[CSEG], Rq[CSEG], Pq[CSEG], L1q[CSEG], L2q[CSEG];
+#pragma unroll
+        for (int r = 0; r < CSEG + 2; ++r) f[r] = ldq(lo - 4 + r);
+#pragma unroll
+        for (int r = 0; r < CSEG; ++r) {   // operands of the emitted point q = b - 3
+            const size_t a = base + (size_t)clampi(lo + r - 3, 0, n - 1) * (size_t)estride;
+            uq[r] = vel[a];
+            Rq[r] = MODE >= 1 ? R[a] : T(0);
+            Pq[r] = MODE == 2 ? phi[a] : T(0);
+            L1q[r] = (MODE == 2 && STAGE >= 2) ? L1[a] : T(0);
+            L2q[r] = (MODE == 2 && STAGE == 3) ? L2[a] : T(0);
+        }
+        T fu[CSEG], fuu[CSEG], fd[CSEG], fdd[CSEG];
+        T cu = S0, cuu = SS0, cd = S0, cdd = SS0;
+        if (k > 0) {
+            cu = ck[(size_t)(4 * k) * NL + Lc];
+            cuu = ck[(size_t)(4 * k + 1) * NL + Lc];
+            cd = ck[(size_t)(4 * k + 2) * NL + Lc];
+            cdd = ck[(size_t)(4 * k + 3) * NL + Lc];
+        }
+#pragma unroll
+        for (int r = 0; r < CSEG; ++r) {
+            const int b = lo + r;
+            if (b >= 1 && b <= hi) frow(b, f[r], f[r + 1], f[r + 2], cu, cuu, cd, cdd);
+            fu[r] = cu; fuu[r] = cuu; fd[r] = cd; fdd[r] = cdd;
+        }
+#pragma unroll
+        for (int r = CSEG - 1; r >= 0; --r) {
+            const int b = lo + r;
+            if (b <= hi) {
+                {
+                    const T s = fu[r] - (AU[3 * b + 2] * SU + BU[3 * b + 2] * SSU);
+                    const T ss = fuu[r] - (AAU[3 * b + 2] * SU + BBU[3 * b + 2] * SSU);
+                    const T sol = -(BU[3 * b + 1] * ss - BBU[3 * b + 1] * s) * IU[b];
+                    SSU = (AU[3 * b + 1] * ss - AAU[3 * b + 1] * s) * IU[b];
+                    SU = sol;
+                }
+                {
+                    const T s = fd[r] - (AD[3 * b + 2] * SD + BD[3 * b + 2] * SSD);
+                    const T ss = fdd[r] - (AAD[3 * b + 2] * SD + BBD[3 * b + 2] * SSD);
+                    const T sol = -(BD[3 * b + 1] * ss - BBD[3 * b + 1] * s) * ID[b];
+                    SSD = (AD[3 * b + 1] * ss - AAD[3 * b + 1] * s) * ID[b];
+                    SD = sol;
+                }
+                if (act && b - 3 >= 0 && b - 3 < n)
+                    emit(b - 3, uq[r] > T(0) ? SU : SD, uq[r], Rq[r], Pq[r], L1q[r], L2q[r]);
+            }
+        }
+    }
+}
+
 // Deterministic final reduction of per-block partials (fixed order), then the
 // mass-correction multiplier lambda = (imass - mass) / sum (mass_preserving_level_set.f90:46).
 // scal: [0] vol, [1] mass, [2] denominator, [3] lambda, [4] flag (1 = empty band).
@@ -1184,6 +1351,36 @@ void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T 
     if (nblocks_out) *nblocks_out = final_pass ? (int)g.x : 0;
 }
 
+template <typename T>
+void launch_uccd_pass(hipStream_t st, int dir, int mode, int stage, const T *phi, T *phi_out, const T *vel, T *R,
+                      T *L1, T *L2, const T *coef, T *ck, int nx, int ny, int nz, T dx, T dt)
+{
+    const int n = dir == 0 ? nx : dir == 1 ? ny : nz;
+    const int nlines = dir == 0 ? ny * nz : dir == 1 ? nx * nz : nx * ny;
+    const int lw = dir == 0 ? nlines : nx;
+    const long ostride = dir == 0 ? 0 : dir == 1 ? (long)nx * ny : (long)nx;
+    const long istride = dir == 0 ? (long)nx : 1;
+    const long estride = dir == 0 ? 1 : dir == 1 ? (long)nx : (long)nx * ny;
+    const dim3 g(nblk(nlines, 256)), b(256);
+    const size_t tab = (size_t)26 * (n + 6) * sizeof(T);
+    const bool lds = tab <= 64 * 1024;
+#define LSMP_UCCD(M, S)                                                                                        \
+    do {                                                                                                       \
+        if (lds)                                                                                               \
+            hipLaunchKernelGGL((k_uccd_seg<T, M, S, true>), g, b, tab, st, phi, phi_out, vel, R, L1, L2, coef,   \
+                               ck, nlines, n, lw, ostride, istride, estride, dx, dt);                          \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_uccd_seg<T, M, S, false>), g, b, 0, st, phi, phi_out, vel, R, L1, L2, coef,    \
+                               ck, nlines, n, lw, ostride, istride, estride, dx, dt);                          \
+    } while (0)
+    if (mode == 0) LSMP_UCCD(0, 1);
+    else if (mode == 1) LSMP_UCCD(1, 1);
+    else if (stage == 1) LSMP_UCCD(2, 1);
+    else if (stage == 2) LSMP_UCCD(2, 2);
+    else LSMP_UCCD(2, 3);
+#undef LSMP_UCCD
+}
+
 void launch_finalize3(hipStream_t st, const double *partial, int nblk_, double imass, double *scal)
 {
     hipLaunchKernelGGL(k_finalize3, dim3(1), dim3(1024), 0, st, partial, nblk_, imass, scal);
@@ -1241,7 +1438,9 @@ void launch_zero_u64(hipStream_t st, unsigned long long *p, int n)
     template void launch_mass_update<T>(hipStream_t, T *, const T *, size_t, T, const double *);               \
     template void launch_mass_vol<T>(hipStream_t, const T *, size_t, const CcdArgs &, double *, int);           \
     template void launch_l1<T>(hipStream_t, const T *, const T *, size_t, double, double *, int);              \
-    template void launch_velocity<T>(hipStream_t, T *, T *, T *, const VelArgs &, double *);
+    template void launch_velocity<T>(hipStream_t, T *, T *, T *, const VelArgs &, double *);                  \
+    template void launch_uccd_pass<T>(hipStream_t, int, int, int, const T *, T *, const T *, T *, T *, T *,     \
+                                      const T *, T *, int, int, int, T, T);
 
 LSMP_INST(double)
 LSMP_INST(float)

@@ -15,6 +15,7 @@ private
 
 integer(c_int), parameter, public :: LSMP_FP64 = 0, LSMP_FP32 = 1
 integer(c_int), parameter, public :: LSMP_VEL_ROTATION = 0, LSMP_VEL_VORTEX2D = 1, LSMP_VEL_LEVEQUE3D = 2
+integer(c_int), parameter, public :: LSMP_SCHEME_CRWENO = 0, LSMP_SCHEME_UCCD = 1
 
 type, bind(C), public :: lsmp_params
     integer(c_int) :: nx, ny, nz, ghc, dim
@@ -110,6 +111,12 @@ interface
         import :: c_int, c_ptr
         type(c_ptr), value :: h
     end function
+    ! level_set_rk3_source selector (level_set_rk3_solver.f90:96-102)
+    integer(c_int) function c_lsmp_set_advection_scheme(h, scheme) bind(C, name="lsmp_set_advection_scheme")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int), value :: scheme
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
         import :: c_size_t, c_ptr
         type(c_ptr), value :: s
@@ -119,6 +126,7 @@ end interface
 public :: c_lsmp_init, c_lsmp_finalize, c_lsmp_upload_phi, c_lsmp_download_phi, c_lsmp_store_phi0
 public :: c_lsmp_set_velocity, c_lsmp_set_velocity_analytic, c_lsmp_rk3_advect, c_lsmp_reinit
 public :: c_lsmp_mass_vol, c_lsmp_mass_fix, c_lsmp_interface_l1, c_lsmp_get_stats, c_lsmp_synchronize
+public :: c_lsmp_set_advection_scheme
 public :: lsmp_check, lsmp_error_message
 
 contains

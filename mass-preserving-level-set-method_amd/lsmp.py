@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "lib", "liblsmp.so")
 
 FP64, FP32 = 0, 1
 FALLBACK_RESTART, FALLBACK_CONTINUE = 0, 1
+SCHEME_CRWENO, SCHEME_UCCD = 0, 1
 VEL_ROTATION, VEL_VORTEX2D, VEL_LEVEQUE3D = 0, 1, 2
 
 ERRORS = {0: "ok", -1: "invalid parameter or size", -2: "HIP runtime error",
@@ -29,7 +30,7 @@ EXPORTS = [
     "lsmp_init", "lsmp_finalize", "lsmp_error_string", "lsmp_upload_phi", "lsmp_download_phi",
     "lsmp_store_phi0", "lsmp_set_velocity", "lsmp_set_velocity_analytic", "lsmp_rk3_advect",
     "lsmp_reinit", "lsmp_mass_vol", "lsmp_mass_fix", "lsmp_interface_l1", "lsmp_get_stats",
-    "lsmp_set_profiling", "lsmp_kernel_time", "lsmp_synchronize",
+    "lsmp_set_profiling", "lsmp_kernel_time", "lsmp_synchronize", "lsmp_set_advection_scheme",
     "lsmp_net_create", "lsmp_net_destroy", "lsmp_net_param_count", "lsmp_net_set_params",
     "lsmp_net_get_params", "lsmp_net_dataset_size", "lsmp_net_collect", "lsmp_net_train",
     "lsmp_net_eval_grad", "lsmp_reinit_learned", "lsmp_reinit_residual", "lsmp_net_add_samples",
@@ -90,6 +91,7 @@ def load_library(path=LIB_PATH):
     L.lsmp_set_profiling.argtypes = [H, C.c_int]
     L.lsmp_kernel_time.argtypes = [H, C.c_char_p, C.POINTER(C.c_longlong), _DP]
     L.lsmp_synchronize.argtypes = [H]
+    L.lsmp_set_advection_scheme.argtypes = [H, C.c_int]
     _FP = C.POINTER(C.c_float)
     _IP = C.POINTER(C.c_int)
     L.lsmp_net_create.argtypes = [C.c_int, C.c_uint, C.POINTER(H)]
@@ -193,6 +195,10 @@ class Solver:
         _check(self.lib.lsmp_set_velocity_analytic(self.h, field, t, t2s))
 
     # -- the hot path
+    def set_advection_scheme(self, scheme):
+        """SCHEME_CRWENO (level_set_rk3_source_weno) or SCHEME_UCCD (source_ccd)."""
+        _check(self.lib.lsmp_set_advection_scheme(self.h, int(scheme)))
+
     def rk3_advect(self, dt=None):
         """level_set_rk3_solver (level_set_rk3_solver.f90:1-94)."""
         _check(self.lib.lsmp_rk3_advect(self.h, self.params.dt if dt is None else dt))

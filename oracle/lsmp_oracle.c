@@ -29,6 +29,7 @@ struct orc_state {
     double time;
     int iter;
     int last_redis_iters, last_mass_iters;
+    int scheme;             /* advection RHS: 0 CRWENO flux split, 1 UCCD */
 };
 
 static inline double sq(double x) { return x * x; }
@@ -88,6 +89,7 @@ double orc_get(const orc_state *s, const char *n)
     if (!strcmp(n, "dt")) return s->p.dt;
     if (!strcmp(n, "redis_iters")) return (double)s->last_redis_iters;
     if (!strcmp(n, "mass_iters")) return (double)s->last_mass_iters;
+    if (!strcmp(n, "scheme")) return (double)s->scheme;
     return NAN;
 }
 
@@ -99,6 +101,7 @@ void orc_set(orc_state *s, const char *n, double v)
     else if (!strcmp(n, "iter")) s->iter = (int)v;
     else if (!strcmp(n, "dt")) s->p.dt = v;
     else if (!strcmp(n, "mass")) s->mass = v;
+    else if (!strcmp(n, "scheme")) s->scheme = (int)v;
 }
 
 /* ------------------------------------------------------------------------
@@ -423,6 +426,12 @@ void orc_rk3_source(orc_state *s)
 }
 
 /* TVD-RK3, incremental form (level_set_rk3_solver.f90:1-94). */
+static void rk3_rhs(orc_state *s)
+{
+    if (s->scheme == 1) orc_rk3_source_uccd(s);
+    else orc_rk3_source(s);
+}
+
 void orc_rk3_advect(orc_state *s)
 {
     const int nx = s->p.nx, ny = s->p.ny, ks = s->ks, ke = s->ke;
@@ -430,7 +439,7 @@ void orc_rk3_advect(orc_state *s)
     double *phi = s->f[ORC_PHI], *tmp = s->f[ORC_PHI_TMP];
     double *l1 = s->f[ORC_L1], *l2 = s->f[ORC_L2], *l3 = s->f[ORC_L3];
 
-    orc_rk3_source(s);
+    rk3_rhs(s);
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -440,7 +449,7 @@ void orc_rk3_advect(orc_state *s)
             }
     orc_bc(s, phi);
 
-    orc_rk3_source(s);
+    rk3_rhs(s);
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -450,7 +459,7 @@ void orc_rk3_advect(orc_state *s)
             }
     orc_bc(s, phi);
 
-    orc_rk3_source(s);
+    rk3_rhs(s);
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -818,6 +827,137 @@ void orc_ccd_line(int n, double dx, const double *f, double *fx, double *fxx)
     ccd_build(&m, n, dx);
     ccd_solve(&m, f, fx, fxx);
     ccd_free(&m);
+}
+
+/* Upwind CCD (ccd_solvers.f90:165-214 init_uccd, :80-134 bc, :312-345
+ * asgn_src_uccd, root_solve "uccd" :428-438): two block systems with the CCD
+ * boundary rows, each factorised once by TWIN_DEC; fx(i) takes the upwind
+ * system's value for u(i) > 0 and the downwind one's otherwise (fxx is not
+ * returned: nothing on the advection path reads it). */
+static void uccd_build(ccd_mat *mu, ccd_mat *md, int n, double dx)
+{
+    const double A1 = 0.875, B1 = 0.1251282341599089, B2 = -0.2487176584009104, B3 = 0.0001282341599089;
+    ccd_mat *ms[2] = {mu, md};
+    for (int w = 0; w < 2; ++w) {
+        ccd_mat *m = ms[w];
+        m->n = n; m->dx = dx;
+        m->A = (double *)calloc((size_t)(3 * n), sizeof(double));
+        m->B = (double *)calloc((size_t)(3 * n), sizeof(double));
+        m->AA = (double *)calloc((size_t)(3 * n), sizeof(double));
+        m->BB = (double *)calloc((size_t)(3 * n), sizeof(double));
+        const int is = 0, ie = n - 1;
+        M3(m->A, 1, is) = 1.0;  M3(m->B, 1, is) = 0.0;
+        M3(m->AA, 1, is) = 0.0; M3(m->BB, 1, is) = 1.0;
+        M3(m->A, 2, is) = 2.0;  M3(m->B, 2, is) = -dx;
+        M3(m->AA, 2, is) = -2.5 / dx; M3(m->BB, 2, is) = 8.5;
+        M3(m->A, 1, ie) = 1.0;  M3(m->B, 1, ie) = 0.0;
+        M3(m->AA, 1, ie) = 0.0; M3(m->BB, 1, ie) = 1.0;
+        M3(m->A, 0, ie) = 2.0;  M3(m->B, 0, ie) = dx;
+        M3(m->AA, 0, ie) = 2.5 / dx; M3(m->BB, 0, ie) = 8.5;
+        for (int i = is + 1; i <= ie - 1; ++i) {
+            if (w == 0) {
+                M3(m->A, 0, i) = A1;       M3(m->A, 1, i) = 1.0;      M3(m->A, 2, i) = 0.0;
+                M3(m->B, 0, i) = B1 * dx;  M3(m->B, 1, i) = B2 * dx;  M3(m->B, 2, i) = B3 * dx;
+            } else {
+                M3(m->A, 0, i) = 0.0;      M3(m->A, 1, i) = 1.0;      M3(m->A, 2, i) = A1;
+                M3(m->B, 0, i) = -B3 * dx; M3(m->B, 1, i) = -B2 * dx; M3(m->B, 2, i) = -B1 * dx;
+            }
+            M3(m->AA, 0, i) = -(9.0 / 8.0 / dx); M3(m->AA, 1, i) = 0.0; M3(m->AA, 2, i) = 9.0 / 8.0 / dx;
+            M3(m->BB, 0, i) = -1.0 / 8.0; M3(m->BB, 1, i) = 1.0; M3(m->BB, 2, i) = -1.0 / 8.0;
+        }
+        twin_dec(m->A, m->B, m->AA, m->BB, n);
+    }
+}
+
+static void uccd_solve(const ccd_mat *mu, const ccd_mat *md, const double *f, const double *u, double *fx,
+                       double *work)
+{
+    const int n = mu->n, is = 0, ie = n - 1;
+    const double dx = mu->dx;
+    const double c3 = -0.06096119008109, c2 = 1.99692238016218, c1 = -1.93596119008109;
+    double *SU = work, *SSU = work + n, *SD = work + 2 * n, *SSD = work + 3 * n;
+    SU[is] = (-3.5 * f[is] + 4.0 * f[is + 1] - 0.5 * f[is + 2]) / dx;
+    SSU[is] = (34.0 / 3.0 * f[is] - 83.0 / 4.0 * f[is + 1] + 10.0 * f[is + 2] - 7.0 / 12.0 * f[is + 3]) /
+              (dx * dx);
+    SD[is] = SU[is];
+    SSD[is] = SSU[is];
+    for (int i = is + 1; i <= ie - 1; ++i) {
+        SU[i] = (c1 * f[i - 1] + c2 * f[i] + c3 * f[i + 1]) / dx;
+        SSU[i] = (3.0 * f[i - 1] - 6.0 * f[i] + 3.0 * f[i + 1]) / (dx * dx);
+        SD[i] = -(c3 * f[i - 1] + c2 * f[i] + c1 * f[i + 1]) / dx;
+        SSD[i] = SSU[i];
+    }
+    SU[ie] = -(-3.5 * f[ie] + 4.0 * f[ie - 1] - 0.5 * f[ie - 2]) / dx;
+    SSU[ie] = (34.0 / 3.0 * f[ie] - 83.0 / 4.0 * f[ie - 1] + 10.0 * f[ie - 2] - 7.0 / 12.0 * f[ie - 3]) /
+              (dx * dx);
+    SD[ie] = SU[ie];
+    SSD[ie] = SSU[ie];
+    twin_bks(mu->A, mu->B, mu->AA, mu->BB, SU, SSU, n);
+    twin_bks(md->A, md->B, md->AA, md->BB, SD, SSD, n);
+    for (int i = is; i <= ie; ++i) fx[i] = u[i] > 0.0 ? SU[i] : SD[i];
+}
+
+void orc_uccd_line(int n, double dx, const double *f, const double *u, double *fx)
+{
+    ccd_mat mu, md;
+    uccd_build(&mu, &md, n, dx);
+    double *work = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+    uccd_solve(&mu, &md, f, u, fx, work);
+    free(work);
+    ccd_free(&mu);
+    ccd_free(&md);
+}
+
+/* level_set_rk3_source_ccd (level_set_rk3_solver.f90:223-286): UCCD first
+ * derivatives along every interior x, y (and, in 3D, z) line of the box, then
+ * tmp = -u phi_x - v phi_y - w phi_z.  In 2D the reference also solves the
+ * 7-point z lines, whose term is multiplied by w = 0; it is omitted here. */
+void orc_rk3_source_uccd(orc_state *s)
+{
+    const int nx = s->p.nx, ny = s->p.ny, ks = s->ks, ke = s->ke;
+    const int three_d = (s->p.dim == 3);
+    double *phi = s->f[ORC_PHI], *tmp = s->f[ORC_PHI_TMP];
+    double *U = s->f[ORC_U], *V = s->f[ORC_V], *W = s->f[ORC_W];
+    double *sx = s->f[ORC_SX1], *sy = s->f[ORC_SY1], *sz = s->f[ORC_SZ1];
+    const int nmax = (s->nxb > s->nyb ? (s->nxb > s->nzb ? s->nxb : s->nzb) : (s->nyb > s->nzb ? s->nyb : s->nzb));
+    double *f = (double *)malloc(sizeof(double) * nmax), *u = (double *)malloc(sizeof(double) * nmax);
+    double *fx = (double *)malloc(sizeof(double) * nmax), *work = (double *)malloc(sizeof(double) * 4 * nmax);
+    ccd_mat mu, md;
+    /* all three solvers are built with dx (ccd_solvers.f90:406-408) */
+    uccd_build(&mu, &md, s->nxb, s->p.dx);
+    for (int k = ks; k <= ke; ++k)
+        for (int j = 1; j <= ny; ++j) {
+            for (int i = 1 - GHC; i <= nx + GHC; ++i) { f[i - 1 + GHC] = AT(phi, i, j, k); u[i - 1 + GHC] = AT(U, i, j, k); }
+            uccd_solve(&mu, &md, f, u, fx, work);
+            for (int i = 1 - GHC; i <= nx + GHC; ++i) AT(sx, i, j, k) = fx[i - 1 + GHC];
+        }
+    ccd_free(&mu); ccd_free(&md);
+    uccd_build(&mu, &md, s->nyb, s->p.dx);
+    for (int k = ks; k <= ke; ++k)
+        for (int i = 1; i <= nx; ++i) {
+            for (int j = 1 - GHC; j <= ny + GHC; ++j) { f[j - 1 + GHC] = AT(phi, i, j, k); u[j - 1 + GHC] = AT(V, i, j, k); }
+            uccd_solve(&mu, &md, f, u, fx, work);
+            for (int j = 1 - GHC; j <= ny + GHC; ++j) AT(sy, i, j, k) = fx[j - 1 + GHC];
+        }
+    ccd_free(&mu); ccd_free(&md);
+    if (three_d) {
+        uccd_build(&mu, &md, s->nzb, s->p.dx);
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                for (int k = 1 - GHC; k <= s->p.nz + GHC; ++k) { f[k - 1 + GHC] = AT(phi, i, j, k); u[k - 1 + GHC] = AT(W, i, j, k); }
+                uccd_solve(&mu, &md, f, u, fx, work);
+                for (int k = 1 - GHC; k <= s->p.nz + GHC; ++k) AT(sz, i, j, k) = fx[k - 1 + GHC];
+            }
+        ccd_free(&mu); ccd_free(&md);
+    }
+    for (int k = ks; k <= ke; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                double r = -AT(U, i, j, k) * AT(sx, i, j, k) - AT(V, i, j, k) * AT(sy, i, j, k);
+                if (three_d) r = r - AT(W, i, j, k) * AT(sz, i, j, k);
+                AT(tmp, i, j, k) = r;
+            }
+    free(f); free(u); free(fx); free(work);
 }
 
 /* surface_norms: first and second CCD derivatives along every x, y and z line

@@ -69,6 +69,8 @@ double orc_phyn(double a, double b, double c, double d);
 /* ccd_solvers.f90:80-163, 291-310, 412-427; matrix_solver.f90:6-78.
  * f, fx, fxx have length n (the full box line). */
 void orc_ccd_line(int n, double dx, const double *f, double *fx, double *fxx);
+/* upwind CCD first derivative, ccd_solvers.f90:165-214, 312-345, 428-438 */
+void orc_uccd_line(int n, double dx, const double *f, const double *u, double *fx);
 
 /* ---- field-level routines, on the state ---- */
 void orc_bc(orc_state *s, double *dat);                   /* branches.f90:275-320 */
@@ -76,6 +78,9 @@ void orc_ls_funs(orc_state *s);                           /* ns_solver/others.f9
 void orc_ls_mv(orc_state *s);                             /* ns_solver/others.f90:172-213 */
 void orc_rk3_source(orc_state *s);                        /* level_set_rk3_solver.f90:104-221 */
 void orc_rk3_advect(orc_state *s);                        /* level_set_rk3_solver.f90:1-94 */
+void orc_rk3_source_uccd(orc_state *s);                   /* level_set_rk3_solver.f90:223-286 */
+/* orc_set(s, "scheme", 1) makes orc_rk3_advect use the UCCD RHS (the
+ * reference's level_set_rk3_source selector, :96-102). */
 void orc_redis_gradient(orc_state *s);                    /* level_set_rk3_redis.f90:266-369 */
 void orc_redis_lambda(orc_state *s, int btn);             /* level_set_rk3_redis.f90:371-454 */
 /* level_set_rk3_redis.f90:1-79; timestop < 0 selects the reference default.

@@ -1,8 +1,9 @@
 """Write golden vectors from the reference's own Fortran units.
 
-Runs the reference's solve_tridiagonal (Vortex/src/others/TDMA.f90:1-36) and
-its CCD line solver (Vortex/lib/solver/ccd_solvers.f90:382-427 with
-matrix_solver.f90:6-78), compiled from the reference sources by
+Runs the reference's solve_tridiagonal (Vortex/src/others/TDMA.f90:1-36), its
+CCD line solver (Vortex/lib/solver/ccd_solvers.f90:382-427 with
+matrix_solver.f90:6-78) and its upwind CCD solver ("uccd", ccd_solvers.f90:
+165-214, 312-345, 428-438), compiled from the reference sources by
 `make -C oracle ref`, on seeded inputs, and stores inputs and outputs in
 ref_vectors.npz next to this script.
 
@@ -49,6 +50,16 @@ def main():
         tab = np.zeros(12 * n)
         R.ref_ccd_tables(n, dx, O._dp(tab))
         out[f"ccd{n}_tables"] = tab
+    # UCCD: velocity of both signs with exact zeros (the downwind branch)
+    for n in (7, 38, 262):
+        dx = 1.0 / (n - 6)
+        x = (np.arange(n) - 2.5) * dx
+        f = np.cos(2 * np.pi * x) + 0.1 * rng.normal(size=n)
+        u = rng.normal(size=n)
+        u[::5] = 0.0
+        fx = np.zeros(n)
+        R.ref_uccd_line(n, dx, O._dp(f), O._dp(u), O._dp(fx))
+        out[f"uccd{n}_f"], out[f"uccd{n}_u"], out[f"uccd{n}_dx"], out[f"uccd{n}_fx"] = f, u, dx, fx
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
     print("wrote", os.path.join(HERE, "ref_vectors.npz"), len(out), "arrays")
 

@@ -69,7 +69,8 @@ def lib():
         L.orc_phyn.restype = C.c_double
         L.orc_phyn.argtypes = [C.c_double] * 4
         L.orc_ccd_line.argtypes = [C.c_int, C.c_double, DP, DP, DP]
-        for name in ["orc_ls_funs", "orc_ls_mv", "orc_rk3_source", "orc_rk3_advect",
+        L.orc_uccd_line.argtypes = [C.c_int, C.c_double, DP, DP, DP]
+        for name in ["orc_ls_funs", "orc_ls_mv", "orc_rk3_source", "orc_rk3_advect", "orc_rk3_source_uccd",
                      "orc_redis_gradient", "orc_surface_norms", "orc_store_initial"]:
             getattr(L, name).argtypes = [C.c_void_p]
         L.orc_bc.argtypes = [C.c_void_p, DP]

@@ -91,6 +91,52 @@ def test_advect_parity(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=ids)
+def test_advect_uccd_parity(case):
+    """The UCCD advection variant (level_set_rk3_source_ccd) against the
+    oracle's UCCD RHS, two RK3 steps, 2D and 3D, non-cubic grids."""
+    o, s = pair(**case)
+    lsmp = load_lsmp()
+    o.set("scheme", 1)
+    s.set_advection_scheme(lsmp.SCHEME_UCCD)
+    say("advect uccd", case)
+    load_state(o, s, shape=2 if case["dim"] == 3 else 1, field=2 if case["dim"] == 3 else 1, perturb=True)
+    for _ in range(2):
+        o.rk3_advect()
+        s.rk3_advect()
+        e = rel_err(s.download_phi(), o.field("PHI"))
+        say("  rel err", e)
+        assert e <= TOL
+    # switching back restores the default scheme on the same handle
+    s.set_advection_scheme(lsmp.SCHEME_CRWENO)
+    o.set("scheme", 0)
+    o.rk3_advect()
+    s.rk3_advect()
+    assert rel_err(s.download_phi(), o.field("PHI")) <= TOL
+    s.close()
+
+
+def test_uccd_fp32_close_to_fp64():
+    lsmp = load_lsmp()
+    n = 48
+    outs = []
+    for prec in (lsmp.FP64, lsmp.FP32):
+        o = O.Oracle(O.make_params(n, dim=3))
+        o.initial_phi(2, 0)
+        o.velocity(2, 0.3, 3.0)
+        s = lsmp.Solver(lsmp.make_params(n, dim=3, precision=prec))
+        s.set_advection_scheme(lsmp.SCHEME_UCCD)
+        s.upload_phi(o.field("PHI"))
+        s.set_velocity(o.field("U"), o.field("V"), o.field("W"))
+        for _ in range(5):
+            s.rk3_advect()
+        outs.append(s.download_phi())
+        s.close()
+    d = np.max(np.abs(outs[0] - outs[1]))
+    say("uccd fp32 vs fp64 max diff", d)
+    assert np.isfinite(outs[1]).all() and d < 1e-5
+
+
+@pytest.mark.parametrize("case", CASES, ids=ids)
 def test_analytic_velocity_matches_uploaded(case):
     """Device-evaluated velocity (velocity.f90) gives the same step as the
     oracle's host velocity uploaded as boxes."""

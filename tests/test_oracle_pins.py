@@ -118,6 +118,92 @@ def test_ccd_not_exact_for_degree5():
     assert np.max(np.abs(fx - 5 * x ** 4)) > 1e-9
 
 
+# ---------------------------------------------------------------- UCCD (ccd_solvers.f90:165-214, 312-345, 428-438)
+@pytest.mark.parametrize("n", [7, 38, 262])
+def test_uccd_bitexact_vs_reference_vectors(gold, n):
+    f, u = gold[f"uccd{n}_f"], gold[f"uccd{n}_u"]
+    dx = float(gold[f"uccd{n}_dx"])
+    fx = np.zeros(n)
+    O.lib().orc_uccd_line(n, dx, dp(f.copy()), dp(u.copy()), dp(fx))
+    assert np.array_equal(fx, gold[f"uccd{n}_fx"])
+
+
+def test_uccd_live_reference_unit():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+    rng = np.random.default_rng(5)
+    for n in (7, 10, 134, 518):
+        dx = 1.0 / (n - 6)
+        f = rng.normal(size=n)
+        u = rng.normal(size=n)
+        u[::3] = 0.0
+        a, b = np.zeros(n), np.zeros(n)
+        O.lib().orc_uccd_line(n, dx, dp(f.copy()), dp(u.copy()), dp(a))
+        R.ref_uccd_line(n, dx, dp(f.copy()), dp(u.copy()), dp(b))
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_uccd_polynomial_exactness(deg, sign):
+    """Both the upwind (u > 0) and the downwind system return the exact first
+    derivative of a quartic; degree 5 is not reproduced (control below)."""
+    rng = np.random.default_rng(10 * deg + (sign > 0))
+    n, dx = 40, 0.03
+    x = np.arange(n) * dx - 0.4
+    c = rng.normal(size=deg + 1)
+    f = np.polyval(c, x)
+    want = np.polyval(np.polyder(c), x)
+    fx = np.zeros(n)
+    O.lib().orc_uccd_line(n, dx, dp(f), dp(np.full(n, sign)), dp(fx))
+    assert np.max(np.abs(fx - want)) < 1e-11 * (np.max(np.abs(want)) + 1.0)
+
+
+def test_uccd_upwind_branches_differ_and_degree5_not_exact():
+    n, dx = 40, 0.03
+    x = np.arange(n) * dx - 0.4
+    f = x ** 5
+    up, down = np.zeros(n), np.zeros(n)
+    O.lib().orc_uccd_line(n, dx, dp(f), dp(np.ones(n)), dp(up))
+    O.lib().orc_uccd_line(n, dx, dp(f), dp(-np.ones(n)), dp(down))
+    assert np.max(np.abs(up - 5 * x ** 4)) > 1e-7
+    assert np.max(np.abs(up[3:-3] - down[3:-3])) > 1e-7
+
+
+def test_uccd_rhs_of_linear_field():
+    """The UCCD RHS -u phi_x - v phi_y - w phi_z of a linear field (linear in
+    the ghosts too) is the constant -(u a + v b + w c), with either sign of
+    the velocity per direction."""
+    o = O.Oracle(O.make_params(16, dim=3))
+    nzb, nyb, nxb = o.field("PHI").shape
+    x = ((np.arange(nxb) - 2.5) / 16)[None, None, :]
+    y = ((np.arange(nyb) - 2.5) / 16)[None, :, None]
+    z = ((np.arange(nzb) - 2.5) / 16)[:, None, None]
+    o.field("PHI")[:] = 0.7 * x - 0.4 * y + 0.25 * z + 0.2
+    o.field("U")[:] = 0.3
+    o.field("V")[:] = -0.1
+    o.field("W")[:] = 0.05
+    o.rk3_source_uccd()
+    want = -(0.3 * 0.7 + 0.1 * 0.4 + 0.05 * 0.25)
+    assert np.max(np.abs(o.interior("PHI_TMP") - want)) < 1e-12
+
+
+def test_uccd_scheme_switch_changes_advection():
+    """orc_set("scheme", 1) routes orc_rk3_advect through the UCCD RHS."""
+    res = []
+    for scheme in (0, 1):
+        o = O.Oracle(O.make_params(24, dim=3))
+        o.set("scheme", scheme)
+        assert o.get("scheme") == scheme
+        o.initial_phi(2, 0)
+        o.velocity(2, 0.3, 3.0)
+        o.rk3_advect()
+        res.append(o.interior("PHI").copy())
+    d = np.max(np.abs(res[0] - res[1]))
+    assert 0 < d < 1e-2   # two consistent schemes, one step apart on a coarse grid
+
+
 # ---------------------------------------------------------------- CRWENO5 / WENO-JS
 def _line(vals):
     return np.ascontiguousarray(vals, dtype=np.float64)
