@@ -34,6 +34,7 @@ struct lsmp_ctx {
     double *partial = nullptr, *scal = nullptr;
     double *vtab = nullptr;   // per-axis sine tables of the analytic velocity
     unsigned long long *u64 = nullptr;
+    unsigned long long *rctl = nullptr;   // pseudo-time loop control block (k_redis_ctl)
     double *hbuf = nullptr;   // pinned readback
     int npartial = 0;
     int *band_list = nullptr;        // narrow-band cell indices (learned reinitialisation)

@@ -350,28 +350,46 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
             timestop = 2.0 * c->p.ls_wid;
         }
     }
-    int iter = 0;
-    double time = 0.0, error = 0.0;
-    for (;;) {
-        iter = iter + 1;
+    // pseudo-steps until time > timestop, accumulated as the reference does
+    int budget = 0;
+    for (double time = 0.0; budget < 1000000;) {
+        ++budget;
         time = time + c->p.rdt;
-        T *cur = (T *)c->phi, *b = (T *)c->phiB, *cc = (T *)c->phiC;
-        launch_zero_u64(c->st, c->u64, 1);
-        LSMP_LAUNCHED();
-        KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 1, btn, cur, b, cur, S0, ra, c->u64)));
-        KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 2, btn, b, cc, cur, S0, ra, c->u64)));
-        KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 3, btn, cc, b, cur, S0, ra, c->u64)));
-        LSMP_HIP(hipMemcpyAsync(c->hbuf, c->u64, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
-        LSMP_HIP(hipStreamSynchronize(c->st));
-        unsigned long long bits;
-        std::memcpy(&bits, c->hbuf, sizeof bits);
-        std::memcpy(&error, &bits, sizeof error);
-        // the new phi is in b; the old one becomes a spare
-        c->phiB = c->phi;
-        c->phi = b;
-        if (time > timestop || error <= 1.0e-6) break;
-        if (max_iter > 0 && iter >= max_iter) break;
+        if (time > timestop) break;
     }
+    if (max_iter > 0 && budget > max_iter) budget = max_iter;
+    // the max |dphi| <= 1e-6 stop is taken on the device (k_redis_ctl); steps
+    // are queued CHUNK at a time and the host reads the control block per chunk
+    constexpr int CHUNK = 16;
+    unsigned long long *ctl = c->rctl;
+    launch_redis_ctl(c->st, ctl, 0);
+    int launched = 0;
+    unsigned long long hc[4] = {0, 0, 0, 0};
+    while (launched < budget) {
+        const int k = std::min(CHUNK, budget - launched);
+        for (int q = 0; q < k; ++q) {
+            T *cur = (T *)c->phi, *b = (T *)c->phiB, *cc = (T *)c->phiC;
+            launch_redis_ctl(c->st, ctl, 1);
+            LSMP_LAUNCHED();
+            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 1, btn, cur, b, cur, S0, ra, ctl)));
+            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 2, btn, b, cc, cur, S0, ra, ctl)));
+            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 3, btn, cc, b, cur, S0, ra, ctl)));
+            launch_redis_ctl(c->st, ctl, 2);
+            // the new phi is in b; the old one becomes a spare
+            c->phiB = c->phi;
+            c->phi = b;
+        }
+        launched += k;
+        LSMP_HIP(hipMemcpyAsync(c->hbuf, ctl, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+        std::memcpy(hc, c->hbuf, sizeof hc);
+        if (hc[1]) break;
+    }
+    // each queued step that the stop flag skipped still exchanged phi and phiB
+    const int iter = (int)hc[2];
+    if ((launched - iter) % 2) std::swap(c->phi, c->phiB);
+    double error;
+    std::memcpy(&error, &hc[3], sizeof error);
     if (std::isnan(error)) return LSMP_E_NAN;
     c->stats.last_redis_iters = iter;
     c->stats.last_red_error = error;
@@ -495,7 +513,7 @@ template <typename T> int velocity_analytic(lsmp_ctx *c, int field, double t, do
 
 void free_all(lsmp_ctx *c)
 {
-    void *bufs[] = {c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
+    void *bufs[] = {c->rctl, c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -573,6 +591,7 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
     }
     if (alloc(&c->scr, sc * c->esz)) return fail(LSMP_E_HIP);
     if (alloc((void **)&c->vtab, (size_t)2 * (c->nx + c->ny + c->nz) * sizeof(double))) return fail(LSMP_E_HIP);
+    if (alloc((void **)&c->rctl, 4 * sizeof(unsigned long long))) return fail(LSMP_E_HIP);
     for (int d = 0; d < c->dim; ++d) {
         std::vector<double> tab;
         ccd_tables(ns[d] + 2 * GHC, params->dx, tab);   // all three solvers use dx (ccd_solvers.f90:406-408)

@@ -37,7 +37,7 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
                      T *scr, int nx, int ny, int nz, T dd, T dt);
 template <typename T>
 void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *phi, T *phi_out, const T *phi0,
-                        const T *S0, const RedisArgs &ra, unsigned long long *errmax);
+                        const T *S0, const RedisArgs &ra, unsigned long long *ctl);
 template <typename T> void launch_sign_init(hipStream_t st, const T *phi, T *S0, size_t n, T ls_wid, int raw);
 template <typename T>
 void launch_grad_max(hipStream_t st, int dim, const T *phi, const T *S0, const RedisArgs &ra,
@@ -60,5 +60,6 @@ void launch_uccd_pass(hipStream_t st, int dir, int mode, int stage, const T *phi
                       T *L1, T *L2, const T *coef, T *ck, int nx, int ny, int nz, T dx, T dt);
 template <typename TO, typename TI> void launch_convert(hipStream_t st, TO *out, const TI *in, size_t n);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
+void launch_redis_ctl(hipStream_t st, unsigned long long *ctl, int op);
 
 }  // namespace lsmp

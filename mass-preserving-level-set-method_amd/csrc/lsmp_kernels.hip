@@ -505,11 +505,12 @@ LSMP_DEV void ring_pos(int t, int &r, int &c)
 template <typename T, int STAGE, int BTN>
 __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                        const T *__restrict__ phi0, const T *__restrict__ S0,
-                                                       RedisArgs ra, unsigned long long *errmax)
+                                                       RedisArgs ra, unsigned long long *ctl)
 {
     __shared__ T sphi[RPY][RPX];
     __shared__ T ss2[REY][REX];
     __shared__ T ss3[REY][REX];
+    if (ctl[1]) return;   // the pseudo-time loop has stopped (k_redis_ctl)
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy, inv12dz = (T)ra.inv12dz;
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
@@ -624,7 +625,7 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
     }
     if (STAGE == 3) {
         double m = block_max(lerr);
-        if (t == 0) atomic_max_nonneg(errmax, m);
+        if (t == 0) atomic_max_nonneg(ctl, m);
     }
 }
 
@@ -633,11 +634,12 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
 template <typename T, int STAGE, int BTN>
 __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                        const T *__restrict__ phi0, const T *__restrict__ S0,
-                                                       RedisArgs ra, unsigned long long *errmax)
+                                                       RedisArgs ra, unsigned long long *ctl)
 {
     __shared__ T sphi[RPY][RPX];
     __shared__ T ss2[REY][REX];
     __shared__ T ss3[REY][REX];
+    if (ctl[1]) return;   // the pseudo-time loop has stopped (k_redis_ctl)
     const int nx = ra.nx, ny = ra.ny;
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy;
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy;
@@ -706,7 +708,7 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
     }
     if (STAGE == 3) {
         double m = block_max(lerr);
-        if (t == 0) atomic_max_nonneg(errmax, m);
+        if (t == 0) atomic_max_nonneg(ctl, m);
     }
 }
 
@@ -722,6 +724,29 @@ __global__ void k_sign_init(const T *__restrict__ phi, T *__restrict__ S0, size_
     T H, D;
     ls_funs(p, ls_wid, H, D);
     S0[q] = T(2) * H - T(1);
+}
+
+// Device-side control of the pseudo-time loop (level_set_rk3_redis.f90:17-23, 68):
+// ctl[0] max |dphi| of the running pseudo-step (bits of a non-negative double),
+// ctl[1] stop flag, ctl[2] pseudo-steps run, ctl[3] max |dphi| of the last one.
+// op 0 resets, op 1 opens a pseudo-step, op 2 closes it and stops the loop once
+// the step's max |dphi| <= 1e-6 (a NaN never stops it, as in the reference);
+// the time limit is the host's count of launched steps.  Stage kernels return
+// at once when the flag is set, so steps can be queued ahead without host
+// round trips and the step count stays the reference's.
+__global__ void k_redis_ctl(unsigned long long *ctl, int op)
+{
+    if (threadIdx.x != 0) return;
+    if (op == 0) {
+        ctl[0] = 0; ctl[1] = 0; ctl[2] = 0; ctl[3] = 0;
+    } else if (op == 1) {
+        if (!ctl[1]) ctl[0] = 0;
+    } else if (!ctl[1]) {
+        ctl[2] += 1;
+        ctl[3] = ctl[0];
+        const double e = __longlong_as_double((long long)ctl[0]);
+        if (e <= 1.0e-6) ctl[1] = 1;
+    }
 }
 
 // Godunov |grad phi| of every interior cell from global loads, reduced to a
@@ -1280,17 +1305,17 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
 
 template <typename T>
 void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *phi, T *phi_out, const T *phi0,
-                        const T *S0, const RedisArgs &ra, unsigned long long *errmax)
+                        const T *S0, const RedisArgs &ra, unsigned long long *ctl)
 {
     if (dim == 3) {
         const dim3 g(nblk(ra.nx, RTX), nblk(ra.ny, RTY), nblk(ra.nz, RKC));
-#define LSMP_R3(S, B) hipLaunchKernelGGL((k_redis_stage3d<T, S, B>), g, dim3(RNT), 0, st, phi, phi_out, phi0, S0, ra, errmax)
+#define LSMP_R3(S, B) hipLaunchKernelGGL((k_redis_stage3d<T, S, B>), g, dim3(RNT), 0, st, phi, phi_out, phi0, S0, ra, ctl)
         if (btn) { if (stage == 1) LSMP_R3(1, 1); else if (stage == 2) LSMP_R3(2, 1); else LSMP_R3(3, 1); }
         else { if (stage == 1) LSMP_R3(1, 0); else if (stage == 2) LSMP_R3(2, 0); else LSMP_R3(3, 0); }
 #undef LSMP_R3
     } else {
         const dim3 g(nblk(ra.nx, RTX), nblk(ra.ny, RTY), 1);
-#define LSMP_R2(S, B) hipLaunchKernelGGL((k_redis_stage2d<T, S, B>), g, dim3(RNT), 0, st, phi, phi_out, phi0, S0, ra, errmax)
+#define LSMP_R2(S, B) hipLaunchKernelGGL((k_redis_stage2d<T, S, B>), g, dim3(RNT), 0, st, phi, phi_out, phi0, S0, ra, ctl)
         if (btn) { if (stage == 1) LSMP_R2(1, 1); else if (stage == 2) LSMP_R2(2, 1); else LSMP_R2(3, 1); }
         else { if (stage == 1) LSMP_R2(1, 0); else if (stage == 2) LSMP_R2(2, 0); else LSMP_R2(3, 0); }
 #undef LSMP_R2
@@ -1417,6 +1442,11 @@ template <typename TO, typename TI>
 void launch_convert(hipStream_t st, TO *out, const TI *in, size_t n)
 {
     hipLaunchKernelGGL((k_convert<TO, TI>), dim3(nblk(n, 256)), dim3(256), 0, st, out, in, n);
+}
+
+void launch_redis_ctl(hipStream_t st, unsigned long long *ctl, int op)
+{
+    hipLaunchKernelGGL(k_redis_ctl, dim3(1), dim3(64), 0, st, ctl, op);
 }
 
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n)

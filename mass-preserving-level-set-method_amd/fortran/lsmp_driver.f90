@@ -10,6 +10,9 @@
 ! metric -- runs on the GPU through lsmp_mod's ISO_C_BINDING interface.
 !
 ! usage:  lsmp_driver <disk|vortex2d|vortex> [deck=./input/default.txt] [max_steps]
+!                     [fp64|fp32] [crweno|uccd]
+! (max_steps 0 = no limit; the last two choose the bulk precision and the
+! level_set_rk3_source variant, level_set_rk3_solver.f90:96-102)
 !
 ! The deck has the reference's format (odd lines are comments).  Like the
 ! reference's shipped cases it needs threads grid 1 1 1 and ghc = 3; method 1
@@ -23,6 +26,8 @@ character(len=256) :: casename, deck, arg
 character(len=30) :: name
 integer :: method, gx, gy, gz, nthreads, nplots, level, ug, ghc, how_to_paras, inverse
 integer :: t2red, iter, max_steps, dim, field, iostat, ubc(2), vbc(2), wbc(2), per(3)
+integer(c_int) :: precision, scheme
+integer(8) :: clk0, clk1, clkrate
 real(8) :: xstart, xend, ystart, yend, zstart, zend, t2s, t2p, dt, rdt, ls_wid
 real(8) :: p_tol, p_w1, p_w2, p_b, t_tol, t_w, rho_1, mu_1, rho_2, mu_2, sigma, g, ggx, ggy, ggz
 real(8) :: cl, cu, ct, re, fr, we, rho_12, mu_12
@@ -40,7 +45,7 @@ integer :: fmass, fvol, fen, fmom, fpos
 ! ------------------------------------------------------------ arguments
 call get_command_argument(1, casename)
 if (len_trim(casename) == 0) then
-    write(*,*) "usage: lsmp_driver <disk|vortex2d|vortex> [deck] [max_steps]"
+    write(*,*) "usage: lsmp_driver <disk|vortex2d|vortex> [deck] [max_steps] [fp64|fp32] [crweno|uccd]"
     stop 1
 endif
 deck = "./input/default.txt"
@@ -49,6 +54,13 @@ if (len_trim(arg) > 0) deck = arg
 max_steps = huge(1)
 call get_command_argument(3, arg)
 if (len_trim(arg) > 0) read(arg, *) max_steps
+if (max_steps <= 0) max_steps = huge(1)
+precision = LSMP_FP64
+call get_command_argument(4, arg)
+if (trim(arg) == "fp32") precision = LSMP_FP32
+scheme = LSMP_SCHEME_CRWENO
+call get_command_argument(5, arg)
+if (trim(arg) == "uccd") scheme = LSMP_SCHEME_UCCD
 
 select case (trim(casename))
 case ("disk")
@@ -149,10 +161,11 @@ write(*,'(A20,I5,A3,I5,A3,I5)') "Grids:", nx, "x", ny, "x", nz
 prm%nx = nx; prm%ny = ny; prm%nz = nz; prm%ghc = ghc; prm%dim = dim
 prm%dx = dx; prm%dy = dy; prm%dz = dz
 prm%dt = dt; prm%rdt = rdt; prm%ls_wid = ls_wid
-prm%rho_12 = rho_12; prm%inverse = inverse; prm%precision = LSMP_FP64
+prm%rho_12 = rho_12; prm%inverse = inverse; prm%precision = precision
 prm%xstart = xstart; prm%xend = xend; prm%ystart = ystart; prm%yend = yend
 prm%zstart = zstart; prm%zend = zend
 call lsmp_check(c_lsmp_init(prm, h), "lsmp_init")
+call lsmp_check(c_lsmp_set_advection_scheme(h, scheme), "lsmp_set_advection_scheme")
 
 ! ------------------------------------------------------------ initial condition (problem_init.f90:32-103)
 allocate(phi(1-ghc:nx+ghc, 1-ghc:ny+ghc, kb0:kb1))
@@ -184,6 +197,8 @@ time = 0.0d0
 iter = 0
 loss_mass_avg = 0.0d0; loss_mass_max = 0.0d0
 loss_vol_avg = 0.0d0; loss_vol_max = 0.0d0
+call lsmp_check(c_lsmp_synchronize(h), "lsmp_synchronize")
+call system_clock(clk0, clkrate)
 do
     if (time + dt > t2s) dt = t2s - time + 1.0d-10
     time = time + dt
@@ -226,6 +241,8 @@ do
     if (time + 1.0d-10 > t2s) exit
     if (iter >= max_steps) exit
 enddo
+call lsmp_check(c_lsmp_synchronize(h), "lsmp_synchronize")
+call system_clock(clk1)
 
 ! ------------------------------------------------------------ calculate_interface_loss (output.f90:169-231)
 call lsmp_check(c_lsmp_interface_l1(h, l1), "lsmp_interface_l1")
@@ -242,6 +259,8 @@ write(456,'(A10, 2ES15.4)') "LS Vol:", loss_vol_avg / iter, loss_vol_max
 write(456,'(A10, 2ES15.4)') "LS Int:", l1
 close(456)
 write(*,'(A,I8,A,ES24.16,A,ES24.16)') " finished: steps", iter, "  L1", l1, "  max |dM|/M", loss_mass_max
+write(*,'(A,F12.4,A,ES12.4)') " wall ms/step", 1.0d3 * real(clk1 - clk0, 8) / real(clkrate, 8) / iter, &
+    "  cell-updates/s", real(nx, 8) * ny * nz * iter / (real(clk1 - clk0, 8) / real(clkrate, 8))
 
 close(fmass); close(fvol); close(fen); close(fmom); close(fpos)
 call c_lsmp_finalize(h)
