@@ -133,6 +133,16 @@ int lsmp_kernel_time(lsmp_handle h, const char *name, long long *launches, doubl
 /* Block until all work queued by the handle has finished. */
 int lsmp_synchronize(lsmp_handle h);
 
+/* Per-interface mass (no counterpart in the reference, which conserves and
+ * reports the total only): the 6-connected components of the interior cells
+ * with phi > phi_min, numbered by their smallest cell index, and the FP64
+ * level-set mass sum rho(H) H dV of each (the integrand of ls_mv,
+ * ns_solver/others.f90:172-213).  query_cells (interior flat indices
+ * (k ny + j) nx + i) receive their component number or -1; comp_mass gets the
+ * first maxcomp masses, *ncomp the number of components. */
+int lsmp_component_masses(lsmp_handle h, double phi_min, int nquery, const long long *query_cells, int *query_comp,
+                          int maxcomp, double *comp_mass, int *ncomp);
+
 /* ------------------------------------------------------------------------
  * Learned reinitialisation.  No counterpart in the reference: it replaces
  * most pseudo-steps of level_set_rk3_redis (level_set_rk3_redis.f90:25-74)

@@ -31,6 +31,7 @@ EXPORTS = [
     "lsmp_store_phi0", "lsmp_set_velocity", "lsmp_set_velocity_analytic", "lsmp_rk3_advect",
     "lsmp_reinit", "lsmp_mass_vol", "lsmp_mass_fix", "lsmp_interface_l1", "lsmp_get_stats",
     "lsmp_set_profiling", "lsmp_kernel_time", "lsmp_synchronize", "lsmp_set_advection_scheme",
+    "lsmp_component_masses",
     "lsmp_net_create", "lsmp_net_destroy", "lsmp_net_param_count", "lsmp_net_set_params",
     "lsmp_net_get_params", "lsmp_net_dataset_size", "lsmp_net_collect", "lsmp_net_train",
     "lsmp_net_eval_grad", "lsmp_reinit_learned", "lsmp_reinit_residual", "lsmp_net_add_samples",
@@ -92,6 +93,8 @@ def load_library(path=LIB_PATH):
     L.lsmp_kernel_time.argtypes = [H, C.c_char_p, C.POINTER(C.c_longlong), _DP]
     L.lsmp_synchronize.argtypes = [H]
     L.lsmp_set_advection_scheme.argtypes = [H, C.c_int]
+    L.lsmp_component_masses.argtypes = [H, C.c_double, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_int),
+                                        C.c_int, _DP, C.POINTER(C.c_int)]
     _FP = C.POINTER(C.c_float)
     _IP = C.POINTER(C.c_int)
     L.lsmp_net_create.argtypes = [C.c_int, C.c_uint, C.POINTER(H)]
@@ -198,6 +201,18 @@ class Solver:
     def set_advection_scheme(self, scheme):
         """SCHEME_CRWENO (level_set_rk3_source_weno) or SCHEME_UCCD (source_ccd)."""
         _check(self.lib.lsmp_set_advection_scheme(self.h, int(scheme)))
+
+    def component_masses(self, phi_min, query_cells=(), maxcomp=4096):
+        """Connected components of phi > phi_min and their level-set masses.
+        Returns (masses (ncomp,), component of each query cell or -1)."""
+        q = np.ascontiguousarray(query_cells, dtype=np.int64)
+        qc = np.empty(q.size, dtype=np.int32)
+        m = np.empty(maxcomp, dtype=np.float64)
+        nc = C.c_int()
+        _check(self.lib.lsmp_component_masses(self.h, phi_min, q.size, q.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                              qc.ctypes.data_as(C.POINTER(C.c_int)), maxcomp,
+                                              m.ctypes.data_as(_DP), C.byref(nc)))
+        return m[:min(nc.value, maxcomp)].copy(), qc
 
     def rk3_advect(self, dt=None):
         """level_set_rk3_solver (level_set_rk3_solver.f90:1-94)."""

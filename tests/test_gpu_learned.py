@@ -334,3 +334,47 @@ def test_ablation_network_gains_over_sweeps_alone(seed):
                 assert e_n < 0.97 * e_s
                 s.reinit(1)
                 s.mass_fix(imass)
+
+
+def test_component_masses_against_scipy_labels():
+    """Per-interface masses: the GPU's 6-connected components of phi > phi_min
+    against scipy.ndimage.label on the same field (partition equality through
+    the component of every cell's representative), and the component masses
+    summing to the solver's total level-set mass."""
+    ndimage = pytest.importorskip("scipy.ndimage")
+    lsmp = load_lsmp()
+    n = 40
+    x = (np.arange(-3, n + 3) + 0.5) / n
+    z, y, xx = np.meshgrid(x, x, x, indexing="ij", sparse=True)
+    spheres = [((0.25, 0.3, 0.3), 0.12), ((0.7, 0.7, 0.6), 0.15), ((0.3, 0.75, 0.7), 0.1)]
+    box = np.full((n + 6,) * 3, -1.0)
+    for (cx, cy, cz), r in spheres:
+        box = np.maximum(box, r - np.sqrt((xx - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2))
+    box = np.ascontiguousarray(box)
+    h = 1.0 / n
+    with lsmp.Solver(lsmp.make_params(n, dim=3)) as s:
+        s.upload_phi(box)
+        total, _ = s.mass_vol()
+        phi_min = -1.5 * h
+        inner = interior(s.download_phi())
+        lab, nlab = ndimage.label(inner > phi_min)   # 6-connectivity by default
+        cells = np.flatnonzero(lab.ravel())
+        masses, comp = s.component_masses(phi_min, cells)
+        say(f"components gpu {masses.size} scipy {nlab}; masses {masses}")
+        assert masses.size == nlab == 3
+        # same partition: scipy label -> gpu component is a bijection
+        pairs = set(zip(lab.ravel()[cells].tolist(), comp.tolist()))
+        assert len(pairs) == nlab and len({p[1] for p in pairs}) == nlab and min(comp) >= 0
+        assert abs(masses.sum() - total) <= 1e-12 * total
+        # each component's mass equals the mass of its sphere alone
+        centre_cells = [int(((round(cz * n - 0.5)) * n + round(cy * n - 0.5)) * n + round(cx * n - 0.5))
+                        for (cx, cy, cz), _ in spheres]
+        _, cc = s.component_masses(phi_min, centre_cells)
+        assert sorted(cc.tolist()) == [0, 1, 2]
+        for (c3, r), k in zip(spheres, cc):
+            cx, cy, cz = c3
+            one = np.ascontiguousarray(np.broadcast_to(r - np.sqrt((xx - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2),
+                                                       box.shape))
+            s.upload_phi(np.maximum(one, -1.0))
+            m1, _ = s.mass_vol()
+            assert abs(masses[k] - m1) <= 1e-12 * m1

@@ -59,6 +59,9 @@ CASES = [
     dict(nx=40, ny=20, nz=36, dim=3),
     dict(nx=48, dim=2),
     dict(nx=70, ny=50, dim=2),
+    # x lines of 640 cells: the CCD and UCCD coefficient tables exceed 64 KB and
+    # are read from global memory instead of LDS
+    dict(nx=640, ny=40, dim=2),
 ]
 
 
