@@ -476,69 +476,113 @@ __global__ __launch_bounds__(256) void k_adv_pair_x(const T *__restrict__ phi, c
 // Reinitialisation stage, 3D (level_set_rk3_redis.f90:176-264 with the
 // gradient :266-369, lambda :371-454 and ls_funs ns_solver/others.f90:1-45).
 //
-// A workgroup owns an interior tile RTX x RTY in (i, j) and marches over RKC
-// planes in k.  Each thread of the tile + 1 ring keeps its own column
-// phi(k-3..k+3) in registers (the z stencil); the current plane, with a
-// 4-cell halo for the x and y stencils, is published to LDS.  grad,
-// s2 = delta S0 (|grad| - 1) and s3 = |grad| delta^2 are formed on the tile +
-// 1 ring and kept for three planes in LDS, so the 3x3x3 lambda quadrature
-// (centre weight 51) reads LDS only.  The next plane's global loads are issued
-// before the current plane is computed, hiding their latency.
+// A workgroup of R3X*R3Y threads owns an interior tile R3X x R3Y in (i, j),
+// one cell per thread, and marches over RKC planes in k.  Each thread keeps
+// its column phi(k-3..k+3) in registers (the z stencil); the current plane,
+// with a 4-cell halo for the x and y stencils, is published to LDS (the
+// halo by threads 0..RHALO-1, one cell each, loaded a plane ahead).  grad,
+// s2 = delta S0 (|grad| - 1) and s3 = |grad| delta^2 are needed on the
+// tile + 1 ring for the 3x3x3 lambda quadrature (centre weight 51); the
+// in-plane 3x3 sums go through LDS and each thread keeps three planes of them
+// in registers, so the quadrature never re-reads a plane.  s2 and s3 vanish
+// where delta = 0, so a cell that does not update (the ring, and the two
+// extra planes k0-1 and k1) needs its gradient only inside the band: the
+// ring cells are a second pass of threads 0..RRING3-1 that does work only
+// for in-band cells, reading their z column from the cache.  With every
+// thread on a tile cell a 32 x 8 tile is a 4-wave workgroup; at about 160
+// VGPRs (FP64) three fit on a CU.
 // The RK stage is written in Shu-Osher form, algebraically identical to the
 // reference's increments: phi1 = phi0 - dtau l(phi0);
 // phi2 = 3/4 phi0 + 1/4 (phi1 - dtau l(phi1)); phi3 = 1/3 phi0 + 2/3 (phi2 - dtau l(phi2)).
 // Stage 3 also reduces max |phi3 - phi0| (:46-62).
 // ---------------------------------------------------------------------------
-constexpr int RRING = RPX * RPY - REX * REY;   // halo-only cells of the LDS plane
-static_assert(RRING <= RNT, "one thread per halo cell");
+constexpr int RNT3 = R3X * R3Y;                   // one thread per tile cell
+constexpr int RHALO = P3X * P3Y - R3X * R3Y;      // LDS plane cells outside the tile
+constexpr int RRING3 = 2 * E3X + 2 * R3Y;         // ring of the tile + 1 ring
+static_assert(RHALO <= 2 * RNT3 && RRING3 <= RNT3, "at most two halo cells and one ring cell per thread");
 
-// LDS position (row, col) of halo cell number t: three rows on top, three at
-// the bottom, three columns left and right of the tile + 1 ring.
-LSMP_DEV void ring_pos(int t, int &r, int &c)
+// LDS position (row, col) of halo cell number t of a (TX + 8) x (TY + 8)
+// plane: four rows on top, four at the bottom, four columns left and right
+// of the TX x TY tile.
+template <int TX, int TY> LSMP_DEV void halo_pos(int t, int &r, int &c)
 {
-    if (t < 3 * RPX) { r = t / RPX; c = t % RPX; }
-    else if (t < 6 * RPX) { r = RPY - 3 + (t - 3 * RPX) / RPX; c = (t - 3 * RPX) % RPX; }
-    else if (t < 6 * RPX + 3 * REY) { const int u = t - 6 * RPX; r = 3 + u / 3; c = u % 3; }
-    else { const int u = t - 6 * RPX - 3 * REY; r = 3 + u / 3; c = RPX - 3 + u % 3; }
+    constexpr int PX = TX + 8, PY = TY + 8;
+    if (t < 4 * PX) { r = t / PX; c = t % PX; }
+    else if (t < 8 * PX) { r = PY - 4 + (t - 4 * PX) / PX; c = (t - 4 * PX) % PX; }
+    else { const int u = t - 8 * PX; r = 4 + u / 8; c = u % 8 < 4 ? u % 8 : PX - 8 + u % 8; }
+}
+
+// Extended-tile position (row ey, column ex) of ring cell number u of a
+// TX x TY tile + 1 ring: the top and bottom rows, then the left and right
+// columns.
+template <int TX, int TY> LSMP_DEV void ext_ring_pos(int u, int &ey, int &ex)
+{
+    constexpr int EX = TX + 2, EY = TY + 2;
+    if (u < EX) { ey = 0; ex = u; }
+    else if (u < 2 * EX) { ey = EY - 1; ex = u - EX; }
+    else if (u < 2 * EX + TY) { ey = 1 + u - 2 * EX; ex = 0; }
+    else { ey = 1 + u - 2 * EX - TY; ex = EX - 1; }
+}
+
+// Thread t's cell of the extended 2D tile: tile cells first, ring after them.
+LSMP_DEV void ext_pos(int t, int &ey, int &ex)
+{
+    if (t < RTX * RTY) { ex = 1 + t % RTX; ey = 1 + t / RTX; }
+    else if (t < REX * REY) ext_ring_pos<RTX, RTY>(t - RTX * RTY, ey, ex);
+    else { ex = 0; ey = 0; }
+}
+
+// delta(phi) != 0 exactly where ls_funs_fast takes its middle branch
+template <typename T> LSMP_DEV bool in_band(T x)
+{
+    return !(x > T(1) - T(1.0e-12)) && !(x < T(-1) + T(1.0e-12));
 }
 
 template <typename T, int STAGE, int BTN>
-__global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi, T *__restrict__ phi_out,
-                                                       const T *__restrict__ phi0, const T *__restrict__ S0,
-                                                       RedisArgs ra, unsigned long long *ctl)
+__global__ __launch_bounds__(RNT3) void k_redis_stage3d(const T *__restrict__ phi, T *__restrict__ phi_out,
+                                                        const T *__restrict__ phi0, const T *__restrict__ S0,
+                                                        RedisArgs ra, unsigned long long *ctl)
 {
-    __shared__ T sphi[RPY][RPX];
-    __shared__ T ss2[REY][REX];
-    __shared__ T ss3[REY][REX];
+    __shared__ T sphi[P3Y][P3X];
+    __shared__ T ss2[E3Y][E3X];
+    __shared__ T ss3[E3Y][E3X];
     if (ctl[1]) return;   // the pseudo-time loop has stopped (k_redis_ctl)
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy, inv12dz = (T)ra.inv12dz;
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
     const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
     const int t = threadIdx.x;
-    const int i0 = blockIdx.x * RTX, j0 = blockIdx.y * RTY, k0 = blockIdx.z * RKC;
+    const int i0 = blockIdx.x * R3X, j0 = blockIdx.y * R3Y, k0 = blockIdx.z * RKC;
     const int k1 = min(k0 + RKC, nz);
-    const bool ext = t < REX * REY;
-    const int ex = t % REX, ey = t / REX;
-    const int gi = clampi(i0 - 1 + ex, 0, nx - 1), gj = clampi(j0 - 1 + ey, 0, ny - 1);
-    const bool inner = ext && ex >= 1 && ex <= RTX && ey >= 1 && ey <= RTY && (i0 - 1 + ex) < nx &&
-                       (j0 - 1 + ey) < ny;
-    // stencil centre in LDS: the clamped cell, so out-of-domain ring cells
-    // take the values of the boundary cell (zero-gradient ghosts)
+    const int tx = t % R3X, ty = t / R3X;
+    const int ex = tx + 1, ey = ty + 1;
+    // cells past the domain edge hold the boundary cell (zero-gradient ghosts)
+    const int gi = min(i0 + tx, nx - 1), gj = min(j0 + ty, ny - 1);
+    const bool inner = i0 + tx < nx && j0 + ty < ny;
     const int pc = gi - (i0 - 4), pr = gj - (j0 - 4);
-    const bool ring = t < RRING;
-    int rr = 0, rc = 0;
-    if (ring) ring_pos(t, rr, rc);
+    // halo cells t and t + RNT3 of the LDS plane are loaded by this thread
+    const bool halo = t < RHALO, halo2 = t + RNT3 < RHALO;
+    int hr = 0, hc = 0, hr2 = 0, hc2 = 0;
+    if (halo) halo_pos<R3X, R3Y>(t, hr, hc);
+    if (halo2) halo_pos<R3X, R3Y>(t + RNT3, hr2, hc2);
+    const bool rg = BTN == 1 && t < RRING3;
+    int qy = 0, qx = 0;
+    if (rg) ext_ring_pos<R3X, R3Y>(t, qy, qx);
+    const int qi = clampi(i0 - 1 + qx, 0, nx - 1), qj = clampi(j0 - 1 + qy, 0, ny - 1);
+    const int qc = qi - (i0 - 4), qr = qj - (j0 - 4);
     const size_t plane = (size_t)nx * ny;
     const size_t col = (size_t)gj * nx + gi;
-    const size_t rcol = (size_t)clampi(j0 - 4 + rr, 0, ny - 1) * nx + clampi(i0 - 4 + rc, 0, nx - 1);
+    const size_t hcol = (size_t)clampi(j0 - 4 + hr, 0, ny - 1) * nx + clampi(i0 - 4 + hc, 0, nx - 1);
+    const size_t hcol2 = (size_t)clampi(j0 - 4 + hr2, 0, ny - 1) * nx + clampi(i0 - 4 + hc2, 0, nx - 1);
+    const size_t qcol = (size_t)qj * nx + qi;
     auto P = [&](int p) -> size_t { return (size_t)clampi(p, 0, nz - 1) * plane; };
 
     T w[7];
 #pragma unroll
-    for (int d = 0; d < 7; ++d) w[d] = ext ? phi[P(k0 - 4 + d) + col] : T(0);
-    T ring_cur = ring ? phi[P(k0 - 1) + rcol] : T(0);
-    T s0_cur = ext ? S0[P(k0 - 1) + col] : T(0);
+    for (int d = 0; d < 7; ++d) w[d] = phi[P(k0 - 4 + d) + col];
+    T halo_cur = halo ? phi[P(k0 - 1) + hcol] : T(0);
+    T halo2_cur = halo2 ? phi[P(k0 - 1) + hcol2] : T(0);
+    T s0_cur = S0[P(k0 - 1) + col];
     T p0_cur = T(0);
     T grad_q = T(0), del_q = T(0), s0_q = T(0), phi_q = T(0), s2_q = T(0), s3_q = T(0);
     // in-plane 3x3 sums of s2 and s3 around this cell for planes p-2, p-1, p
@@ -546,18 +590,21 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
     double lerr = 0.0;
     for (int p = k0 - 1; p <= k1; ++p) {
         // next iteration's inputs, in flight while this plane is computed
-        const T w_next = ext ? phi[P(p + 4) + col] : T(0);
-        const T ring_next = ring ? phi[P(p + 1) + rcol] : T(0);
-        const T s0_next = ext ? S0[P(p + 1) + col] : T(0);
+        const T w_next = phi[P(p + 4) + col];
+        const T halo_next = halo ? phi[P(p + 1) + hcol] : T(0);
+        const T halo2_next = halo2 ? phi[P(p + 1) + hcol2] : T(0);
+        const T s0_next = S0[P(p + 1) + col];
         const T p0_next = (STAGE > 1 && inner && p >= k0 && p < k1) ? phi0[(size_t)p * plane + col] : T(0);
 
-        if (ext) sphi[ey + 3][ex + 3] = w[3];
-        if (ring) sphi[rr][rc] = ring_cur;
+        sphi[ty + 4][tx + 4] = w[3];
+        if (halo) sphi[hr][hc] = halo_cur;
+        if (halo2) sphi[hr2][hc2] = halo2_cur;
         __syncthreads();
         const bool live = p >= 0 && p < nz;
         T grad_p = T(0), del_p = T(0), s2_p = T(0), s3_p = T(0);
         const T s0_p = s0_cur, phi_p = w[3];
-        if (ext && live) {
+        const bool centre = inner && p >= k0 && p < k1;
+        if (live && (centre || (BTN == 1 && in_band(w[3] * inv_w)))) {
             T xs1, xs2, ys1, ys2, zs1, zs2;
             weno_hj_pm_fast(sphi[pr][pc - 3], sphi[pr][pc - 2], sphi[pr][pc - 1], w[3], sphi[pr][pc + 1],
                             sphi[pr][pc + 2], sphi[pr][pc + 3], inv12dx, invdx, xs1, xs2);
@@ -570,9 +617,35 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
                 ls_funs_fast(w[3], inv_w, H, del_p);
                 s2_p = del_p * s0_p * (grad_p - T(1));
                 s3_p = grad_p * del_p * del_p;
-                ss2[ey][ex] = s2_p;
-                ss3[ey][ex] = s3_p;
             }
+        }
+        if (BTN == 1 && live) {
+            ss2[ey][ex] = s2_p;
+            ss3[ey][ex] = s3_p;
+        }
+        if (rg && live) {
+            // ring cell of the extended tile: s2, s3 only, and only in the band
+            const T rphi = sphi[qr][qc];
+            T r2 = T(0), r3 = T(0);
+            if (in_band(rphi * inv_w)) {
+                T z[7];
+#pragma unroll
+                for (int d = 0; d < 7; ++d) z[d] = phi[P(p - 3 + d) + qcol];
+                const T rs0 = S0[(size_t)p * plane + qcol];
+                T xs1, xs2, ys1, ys2, zs1, zs2;
+                weno_hj_pm_fast(sphi[qr][qc - 3], sphi[qr][qc - 2], sphi[qr][qc - 1], rphi, sphi[qr][qc + 1],
+                                sphi[qr][qc + 2], sphi[qr][qc + 3], inv12dx, invdx, xs1, xs2);
+                weno_hj_pm_fast(sphi[qr - 3][qc], sphi[qr - 2][qc], sphi[qr - 1][qc], rphi, sphi[qr + 1][qc],
+                                sphi[qr + 2][qc], sphi[qr + 3][qc], inv12dy, invdy, ys1, ys2);
+                weno_hj_pm_fast(z[0], z[1], z[2], z[3], z[4], z[5], z[6], inv12dz, invdz, zs1, zs2);
+                const T g = godunov<T, 3>(xs1, xs2, ys1, ys2, zs1, zs2, rs0);
+                T H, del;
+                ls_funs_fast(rphi, inv_w, H, del);
+                r2 = del * rs0 * (g - T(1));
+                r3 = g * del * del;
+            }
+            ss2[qy][qx] = r2;
+            ss3[qy][qx] = r3;
         }
         __syncthreads();
         if (BTN == 1 && inner && live) {
@@ -618,7 +691,8 @@ __global__ __launch_bounds__(RNT) void k_redis_stage3d(const T *__restrict__ phi
 #pragma unroll
         for (int d = 0; d < 6; ++d) w[d] = w[d + 1];
         w[6] = w_next;
-        ring_cur = ring_next;
+        halo_cur = halo_next;
+        halo2_cur = halo2_next;
         s0_cur = s0_next;
         p0_cur = p0_next;
         grad_q = grad_p; del_q = del_p; s0_q = s0_p; phi_q = phi_p; s2_q = s2_p; s3_q = s3_p;
@@ -647,10 +721,10 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
     const int t = threadIdx.x;
     const int i0 = blockIdx.x * RTX, j0 = blockIdx.y * RTY;
     const bool ext = t < REX * REY;
-    const int ex = t % REX, ey = t / REX;
+    int ex, ey;
+    ext_pos(t, ey, ex);   // tile cells on whole waves, the ring after them
     const int gi = clampi(i0 - 1 + ex, 0, nx - 1), gj = clampi(j0 - 1 + ey, 0, ny - 1);
-    const bool inner = ext && ex >= 1 && ex <= RTX && ey >= 1 && ey <= RTY && (i0 - 1 + ex) < nx &&
-                       (j0 - 1 + ey) < ny;
+    const bool inner = t < RTX * RTY && (i0 - 1 + ex) < nx && (j0 - 1 + ey) < ny;
     const int pc = gi - (i0 - 4), pr = gj - (j0 - 4);
     for (int idx = t; idx < RPX * RPY; idx += RNT) {
         int c = idx % RPX, r = idx / RPX;
@@ -659,7 +733,13 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
     }
     __syncthreads();
     T grad_p = T(0), del_p = T(0), s0_p = T(0), phi_p = T(0);
-    if (ext) {
+    // a ring cell's gradient only enters lambda through s2, s3 = 0 where delta = 0
+    if (ext && !inner && !(BTN == 1 && in_band(sphi[pr][pc] * inv_w))) {
+        if (BTN == 1) {
+            ss2[ey][ex] = T(0);
+            ss3[ey][ex] = T(0);
+        }
+    } else if (ext) {
         const T c0 = sphi[pr][pc];
         T xs1, xs2, ys1, ys2;
         weno_hj_pm_fast(sphi[pr][pc - 3], sphi[pr][pc - 2], sphi[pr][pc - 1], c0, sphi[pr][pc + 1],
@@ -1308,8 +1388,8 @@ void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *ph
                         const T *S0, const RedisArgs &ra, unsigned long long *ctl)
 {
     if (dim == 3) {
-        const dim3 g(nblk(ra.nx, RTX), nblk(ra.ny, RTY), nblk(ra.nz, RKC));
-#define LSMP_R3(S, B) hipLaunchKernelGGL((k_redis_stage3d<T, S, B>), g, dim3(RNT), 0, st, phi, phi_out, phi0, S0, ra, ctl)
+        const dim3 g(nblk(ra.nx, R3X), nblk(ra.ny, R3Y), nblk(ra.nz, RKC));
+#define LSMP_R3(S, B) hipLaunchKernelGGL((k_redis_stage3d<T, S, B>), g, dim3(RNT3), 0, st, phi, phi_out, phi0, S0, ra, ctl)
         if (btn) { if (stage == 1) LSMP_R3(1, 1); else if (stage == 2) LSMP_R3(2, 1); else LSMP_R3(3, 1); }
         else { if (stage == 1) LSMP_R3(1, 0); else if (stage == 2) LSMP_R3(2, 0); else LSMP_R3(3, 0); }
 #undef LSMP_R3

@@ -65,10 +65,14 @@ def interface_masses(s, centre_cells, ls_wid):
     return np.array([masses[c] if c >= 0 else np.nan for c in comp]), comp
 
 
-def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, weights=None, log=print):
+def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.01, weights=None, log=print):
+    """The case at n^3.  Two interfaces are told apart only while their
+    surfaces stay more than 2 ls_wid + dx = 3.5 dx apart (the regions
+    phi > -ls_wid then share no face); the default gap 0.01 clears that at
+    n >= 350."""
     prec = lsmp.FP64 if form.startswith("fp64") else lsmp.FP32
     learned = form.endswith("learned")
-    centres, radii = seeded_spheres(count, seed=seed)
+    centres, radii = seeded_spheres(count, gap=gap, seed=seed)
     t0 = time.time()
     box = spheres_box(n, centres, radii)
     log(f"  {count} spheres at {n}^3 built in {time.time() - t0:.1f} s")
@@ -102,7 +106,8 @@ def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, weight
         pseudo += k
         s.mass_fix(imass)
         if (it + 1) % max(1, nsteps // 20) == 0:
-            log(f"  {form} {n}^3 step {it + 1}/{nsteps} t={t:.4f}")
+            log(f"  {form} {n}^3 step {it + 1}/{nsteps} t={t:.4f} elapsed {time.perf_counter() - w0:.1f} s "
+                f"pseudo-steps {pseudo} fallbacks {fallbacks}")
     s.synchronize()
     el = time.perf_counter() - w0
     mass, _ = s.mass_vol()
