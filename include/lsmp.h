@@ -135,13 +135,19 @@ int lsmp_synchronize(lsmp_handle h);
 
 /* Per-interface mass (no counterpart in the reference, which conserves and
  * reports the total only): the 6-connected components of the interior cells
- * with phi > phi_min, numbered by their smallest cell index, and the FP64
- * level-set mass sum rho(H) H dV of each (the integrand of ls_mv,
- * ns_solver/others.f90:172-213).  query_cells (interior flat indices
- * (k ny + j) nx + i) receive their component number or -1; comp_mass gets the
- * first maxcomp masses, *ncomp the number of components. */
-int lsmp_component_masses(lsmp_handle h, double phi_min, int nquery, const long long *query_cells, int *query_comp,
-                          int maxcomp, double *comp_mass, int *ncomp);
+ * with phi > phi_core, numbered by their smallest cell index; each cell with
+ * phi_min < phi <= phi_core joins the component that reaches it in the fewest
+ * face steps through such cells (the lower number on ties; a cell no
+ * component reaches joins none).  phi_core = 0, phi_min = -ls_wid keeps two
+ * interfaces apart as long as their zero level sets do not touch;
+ * phi_core = phi_min labels the region phi > phi_min as such.  Each
+ * component gets the FP64 level-set mass sum rho(H) H dV of its cells (the
+ * integrand of ls_mv, ns_solver/others.f90:172-213).  query_cells (interior
+ * flat indices (k ny + j) nx + i) receive their component number or -1;
+ * comp_mass gets the first maxcomp masses, *ncomp the number of components.
+ * LSMP_E_INVALID if phi_min > phi_core. */
+int lsmp_component_masses(lsmp_handle h, double phi_core, double phi_min, int nquery, const long long *query_cells,
+                          int *query_comp, int maxcomp, double *comp_mass, int *ncomp);
 
 /* ------------------------------------------------------------------------
  * Learned reinitialisation.  No counterpart in the reference: it replaces

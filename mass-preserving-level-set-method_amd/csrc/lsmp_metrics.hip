@@ -1,5 +1,7 @@
 // lsmp_metrics.hip -- per-interface mass: connected components of the region
-// phi > phi_min (6-connected, interior cells) and the level-set mass of each.
+// phi > phi_core (6-connected, interior cells), each grown through the cells
+// with phi_min < phi <= phi_core (the smeared half of the interface band),
+// and the level-set mass of each.
 //
 // No counterpart in the reference, which conserves and reports the total mass
 // only (ls_mv, ns_solver/others.f90:172-213).  The per-component integrand is
@@ -9,10 +11,14 @@
 // min-propagation over the 6 neighbours (each sweep also follows the label
 // pointer once, which halves the remaining distance) until a sweep changes
 // nothing.  The component's label is then the smallest cell index in it.
+// Growth: one face-neighbour layer per sweep (read one label array, write the
+// other), so a band cell joins the component it is fewest steps from, the
+// lower label on ties; band cells no component reaches belong to none.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "lsmp_ctx.h"
@@ -57,6 +63,34 @@ __global__ void k_cc_sweep(int *__restrict__ lab, int nx, int ny, int nz, unsign
     }
 }
 
+template <typename T>
+__global__ void k_cc_grow(const T *__restrict__ phi, const int *__restrict__ lab, int *__restrict__ lab_out,
+                          int nx, int ny, int nz, T phi_min, unsigned *changed)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int l = lab[q];
+    if (l != NOLAB || !(phi[q] > phi_min)) {
+        lab_out[q] = l;
+        return;
+    }
+    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+    int m = NOLAB;
+    auto nb = [&](int a, int b, int c) {
+        const int v = lab[((size_t)c * ny + b) * nx + a];
+        if (v < m) m = v;
+    };
+    if (i > 0) nb(i - 1, j, k);
+    if (i < nx - 1) nb(i + 1, j, k);
+    if (j > 0) nb(i, j - 1, k);
+    if (j < ny - 1) nb(i, j + 1, k);
+    if (k > 0) nb(i, j, k - 1);
+    if (k < nz - 1) nb(i, j, k + 1);
+    lab_out[q] = m;
+    if (m != NOLAB) *changed = 1u;
+}
+
 // roots (lab[q] == q) get consecutive component ids in index order
 __global__ void k_cc_roots(const int *__restrict__ lab, size_t n, int *__restrict__ rootflag)
 {
@@ -99,8 +133,8 @@ thread_local char g_merr[256];
 unsigned cdiv(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
 
 template <typename T>
-int components(lsmp_ctx *c, double phi_min, int nquery, const long long *query, int *query_comp, int maxcomp,
-               double *comp_mass, int *ncomp)
+int components(lsmp_ctx *c, double phi_core, double phi_min, int nquery, const long long *query, int *query_comp,
+               int maxcomp, double *comp_mass, int *ncomp)
 {
     const size_t n = c->ncell;
     int *lab = nullptr, *cid = nullptr;
@@ -126,8 +160,8 @@ int components(lsmp_ctx *c, double phi_min, int nquery, const long long *query, 
     MET_TRY(hipMalloc(&lab, n * sizeof(int)));
     MET_TRY(hipMalloc(&cid, n * sizeof(int)));
     MET_TRY(hipMalloc(&flag, sizeof(unsigned)));
-    hipLaunchKernelGGL((k_cc_init<T>), dim3(cdiv(n, 256)), dim3(256), 0, c->st, (const T *)c->phi, n, (T)phi_min,
-                       lab);
+    hipLaunchKernelGGL((k_cc_init<T>), dim3(cdiv(n, 256)), dim3(256), 0, c->st, (const T *)c->phi, n,
+                       (T)phi_core, lab);
     MET_TRY(hipGetLastError());
     // propagate until a sweep changes nothing (checked every 8 sweeps)
     unsigned hflag = 1;
@@ -138,6 +172,22 @@ int components(lsmp_ctx *c, double phi_min, int nquery, const long long *query, 
         MET_TRY(hipGetLastError());
         MET_TRY(hipMemcpyAsync(&hflag, flag, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
         MET_TRY(hipStreamSynchronize(c->st));
+    }
+    if (phi_min < phi_core) {
+        // grow the components through the band, one layer per sweep; cid is
+        // the second label array until the ids are numbered
+        hflag = 1;
+        int *a = lab, *b = cid;
+        for (int sweep = 0; hflag && sweep < 1000000; ++sweep) {
+            MET_TRY(hipMemsetAsync(flag, 0, sizeof(unsigned), c->st));
+            hipLaunchKernelGGL((k_cc_grow<T>), dim3(cdiv(n, 256)), dim3(256), 0, c->st, (const T *)c->phi, a, b,
+                               c->nx, c->ny, c->nz, (T)phi_min, flag);
+            MET_TRY(hipGetLastError());
+            MET_TRY(hipMemcpyAsync(&hflag, flag, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+            MET_TRY(hipStreamSynchronize(c->st));
+            std::swap(a, b);
+        }
+        if (a != lab) MET_TRY(hipMemcpyAsync(lab, a, n * sizeof(int), hipMemcpyDeviceToDevice, c->st));
     }
     // component ids: prefix numbering of the roots, on the host (labels are
     // cell indices, so a host pass over the root flags is exact and ordered)
@@ -178,12 +228,15 @@ int components(lsmp_ctx *c, double phi_min, int nquery, const long long *query, 
 
 }  // namespace
 
-extern "C" int lsmp_component_masses(lsmp_handle h, double phi_min, int nquery, const long long *query_cells,
-                                     int *query_comp, int maxcomp, double *comp_mass, int *ncomp)
+extern "C" int lsmp_component_masses(lsmp_handle h, double phi_core, double phi_min, int nquery,
+                                     const long long *query_cells, int *query_comp, int maxcomp, double *comp_mass,
+                                     int *ncomp)
 {
     if (!h || nquery < 0 || maxcomp < 0 || (nquery > 0 && (!query_cells || !query_comp)) ||
-        (maxcomp > 0 && !comp_mass))
+        (maxcomp > 0 && !comp_mass) || !(phi_min <= phi_core))
         return LSMP_E_INVALID;
-    return h->fp32 ? components<float>(h, phi_min, nquery, query_cells, query_comp, maxcomp, comp_mass, ncomp)
-                   : components<double>(h, phi_min, nquery, query_cells, query_comp, maxcomp, comp_mass, ncomp);
+    return h->fp32 ? components<float>(h, phi_core, phi_min, nquery, query_cells, query_comp, maxcomp, comp_mass,
+                                       ncomp)
+                   : components<double>(h, phi_core, phi_min, nquery, query_cells, query_comp, maxcomp, comp_mass,
+                                        ncomp);
 }

@@ -93,8 +93,8 @@ def load_library(path=LIB_PATH):
     L.lsmp_kernel_time.argtypes = [H, C.c_char_p, C.POINTER(C.c_longlong), _DP]
     L.lsmp_synchronize.argtypes = [H]
     L.lsmp_set_advection_scheme.argtypes = [H, C.c_int]
-    L.lsmp_component_masses.argtypes = [H, C.c_double, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_int),
-                                        C.c_int, _DP, C.POINTER(C.c_int)]
+    L.lsmp_component_masses.argtypes = [H, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_longlong),
+                                        C.POINTER(C.c_int), C.c_int, _DP, C.POINTER(C.c_int)]
     _FP = C.POINTER(C.c_float)
     _IP = C.POINTER(C.c_int)
     L.lsmp_net_create.argtypes = [C.c_int, C.c_uint, C.POINTER(H)]
@@ -202,14 +202,17 @@ class Solver:
         """SCHEME_CRWENO (level_set_rk3_source_weno) or SCHEME_UCCD (source_ccd)."""
         _check(self.lib.lsmp_set_advection_scheme(self.h, int(scheme)))
 
-    def component_masses(self, phi_min, query_cells=(), maxcomp=4096):
-        """Connected components of phi > phi_min and their level-set masses.
-        Returns (masses (ncomp,), component of each query cell or -1)."""
+    def component_masses(self, phi_min, query_cells=(), maxcomp=4096, phi_core=None):
+        """Connected components of phi > phi_core (default phi_min), grown
+        through the cells with phi_min < phi <= phi_core, and their level-set
+        masses.  Returns (masses (ncomp,), component of each query cell or -1)."""
+        phi_core = phi_min if phi_core is None else phi_core
         q = np.ascontiguousarray(query_cells, dtype=np.int64)
         qc = np.empty(q.size, dtype=np.int32)
         m = np.empty(maxcomp, dtype=np.float64)
         nc = C.c_int()
-        _check(self.lib.lsmp_component_masses(self.h, phi_min, q.size, q.ctypes.data_as(C.POINTER(C.c_longlong)),
+        _check(self.lib.lsmp_component_masses(self.h, phi_core, phi_min, q.size,
+                                              q.ctypes.data_as(C.POINTER(C.c_longlong)),
                                               qc.ctypes.data_as(C.POINTER(C.c_int)), maxcomp,
                                               m.ctypes.data_as(_DP), C.byref(nc)))
         return m[:min(nc.value, maxcomp)].copy(), qc

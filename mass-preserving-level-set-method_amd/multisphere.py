@@ -2,9 +2,13 @@
 in [0.02, 0.06]) in the unit cube, advected by the 3D LeVeque deformation
 field (Vortex/src/setup/velocity.f90, t2s = T = 3), MPLS every step, on one
 GPU at 512^3.  Reports cell-updates/s, the L1 shape error against the initial
-field, the total |dM|/M and the per-interface |dM_i|/M_i: every sphere's
-level-set mass is that of the connected component of phi > -ls_wid holding
-its centre cell (lsmp_component_masses), at t = 0 and at the end.
+field, the total |dM|/M and the per-interface |dM_i|/M_i.  An interface is a
+connected component of phi > 0 together with the band cells
+-ls_wid < phi <= 0 nearest to it (lsmp_component_masses with phi_core = 0,
+phi_min = -ls_wid); sphere i's mass is that of the component holding its
+centre, carried as a material point by the same velocity and RK3 steps as the
+field.  Spheres whose components have merged by the end are counted, not
+compared.
 
     python mass-preserving-level-set-method_amd/multisphere.py --n 512 --form fp32-learned
     python mass-preserving-level-set-method_amd/multisphere.py --n 512 --form fp64-plain --frac 0.125
@@ -60,9 +64,37 @@ def spheres_box(n, centres, radii, floor=-0.25):
     return box
 
 
-def interface_masses(s, centre_cells, ls_wid):
-    masses, comp = s.component_masses(-ls_wid, centre_cells)
-    return np.array([masses[c] if c >= 0 else np.nan for c in comp]), comp
+def leveque_velocity(p, t, T):
+    """The 3D LeVeque field at points p (m, 3) and time t
+    (Vortex/src/setup/velocity.f90), as lsmp_set_velocity_analytic sets it."""
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    ct = np.cos(np.pi * t / T)
+    s1 = lambda c: np.sin(np.pi * c)
+    s2 = lambda c: np.sin(2 * np.pi * c)
+    return np.stack([2 * s1(x) ** 2 * s2(y) * s2(z) * ct, -s2(x) * s1(y) ** 2 * s2(z) * ct,
+                     -s2(x) * s2(y) * s1(z) ** 2 * ct], axis=1)
+
+
+def track_points(p, t, dt, T):
+    """One step of the solver's TVD-RK3 (velocity frozen at t + dt) applied
+    to material points."""
+    u = lambda q: leveque_velocity(q, t + dt, T)
+    p1 = p + dt * u(p)
+    p2 = 0.75 * p + 0.25 * (p1 + dt * u(p1))
+    return p / 3 + 2 / 3 * (p2 + dt * u(p2))
+
+
+def point_cells(p, n):
+    ijk = np.clip(np.floor(p * n).astype(np.int64), 0, n - 1)
+    return (ijk[:, 2] * n + ijk[:, 1]) * n + ijk[:, 0]
+
+
+def interface_masses(s, cells, ls_wid):
+    """Per-interface masses of the spheres whose centres lie in `cells`:
+    (mass of each sphere's component or nan, component of each, all
+    component masses)."""
+    masses, comp = s.component_masses(-ls_wid, cells, phi_core=0.0)
+    return np.array([masses[c] if c >= 0 else np.nan for c in comp]), comp, masses
 
 
 def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.01, weights=None, log=print):
@@ -85,9 +117,8 @@ def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.
     del box
     s.store_phi0()
     imass, _ = s.mass_vol()
-    ijk = np.clip(np.floor(centres * n).astype(np.int64), 0, n - 1)
-    centre_cells = (ijk[:, 2] * n + ijk[:, 1]) * n + ijk[:, 0]
-    m0, comp0 = interface_masses(s, centre_cells, s.params.ls_wid)
+    m0, comp0, _ = interface_masses(s, point_cells(centres, n), s.params.ls_wid)
+    pts = centres.copy()
     dt = s.params.dt
     nsteps = int(round(T / dt * frac))
     t = 0.0
@@ -105,20 +136,33 @@ def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.
             _, k = s.reinit(1)
         pseudo += k
         s.mass_fix(imass)
+        pts = track_points(pts, t - dt, dt, T)
         if (it + 1) % max(1, nsteps // 20) == 0:
             log(f"  {form} {n}^3 step {it + 1}/{nsteps} t={t:.4f} elapsed {time.perf_counter() - w0:.1f} s "
                 f"pseudo-steps {pseudo} fallbacks {fallbacks}")
     s.synchronize()
     el = time.perf_counter() - w0
     mass, _ = s.mass_vol()
-    m1, comp1 = interface_masses(s, centre_cells, s.params.ls_wid)
+    m1, comp1, all1 = interface_masses(s, point_cells(pts, n), s.params.ls_wid)
+    # a sphere is compared when its component holds no other sphere's centre,
+    # at the start and at the end
+    alone = np.array([c0 >= 0 and c1 >= 0 and np.sum(comp0 == c0) == 1 and np.sum(comp1 == c1) == 1
+                      for c0, c1 in zip(comp0, comp1)])
     rel = np.abs(m1 - m0) / m0
+    # components holding no sphere centre: pieces broken off, or islands
+    frag = np.setdiff1d(np.arange(all1.size), comp1[comp1 >= 0])
     out = dict(form=form, n=n, spheres=count, T=T, t_end=t, steps=nsteps, ms_per_step=el / nsteps * 1e3,
                cell_updates_per_s=n ** 3 * nsteps / el, l1=s.interface_l1(), dM_over_M=abs(mass - imass) / imass,
                components_start=int(len(set(comp0.tolist()) - {-1})),
-               components_end=int(len(set(comp1.tolist()) - {-1})),
-               interface_dM_over_M_max=float(np.nanmax(rel)), interface_dM_over_M_mean=float(np.nanmean(rel)),
-               pseudo_per_step=pseudo / max(nsteps, 1), fallbacks=fallbacks)
+               components_end=int(len(set(comp1.tolist()) - {-1})), components_all_end=int(all1.size),
+               spheres_compared=int(alone.sum()),
+               interface_dM_over_M_max=float(np.max(rel[alone])) if alone.any() else None,
+               interface_dM_over_M_mean=float(np.mean(rel[alone])) if alone.any() else None,
+               mass_in_components_end=float(all1.sum() / mass),
+               fragments_end=int(frag.size), fragment_mass_end=float(all1[frag].sum() / mass),
+               fragment_mass_max_end=float(all1[frag].max() / mass) if frag.size else 0.0,
+               pseudo_per_step=pseudo / max(nsteps, 1), fallbacks=fallbacks,
+               interface_mass_start=m0.tolist(), interface_mass_end=m1.tolist())
     s.close()
     if net is not None:
         net.close()

@@ -44,3 +44,37 @@ def test_spheres_box_matches_brute_force():
     # each sphere's centre cell is inside (phi > 0)
     ijk = np.clip(np.floor(c * n).astype(int), 0, n - 1)
     assert np.all(box[ijk[:, 2] + 3, ijk[:, 1] + 3, ijk[:, 0] + 3] > 0)
+
+
+def test_tracked_centres_return_after_one_period():
+    """The LeVeque field reverses at T/2, so its flow map over [0, T] is the
+    identity: tracked centres move far and come back, with the first-order
+    error of a velocity frozen over each step (halving with dt)."""
+    M = _ms()
+    c, _ = M.seeded_spheres(64, seed=2026)
+    errs = []
+    for n in (96, 192):
+        dt, T = 0.1 / n, 3.0
+        p, t, far = c.copy(), 0.0, 0.0
+        nsteps = int(round(T / dt))
+        for it in range(nsteps):
+            p = M.track_points(p, t, dt, T)
+            t += dt
+            if it == nsteps // 2:
+                far = np.max(np.linalg.norm(p - c, axis=1))
+        assert far > 0.5
+        errs.append(np.max(np.linalg.norm(p - c, axis=1)))
+    assert errs[0] < 3e-3 and 1.8 < errs[0] / errs[1] < 2.2, errs
+
+
+def test_velocity_is_divergence_free():
+    M = _ms()
+    rng = np.random.default_rng(3)
+    p = rng.uniform(0.05, 0.95, size=(50, 3))
+    h = 1e-5
+    div = np.zeros(50)
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = h
+        div += (M.leveque_velocity(p + e, 0.4, 3.0)[:, a] - M.leveque_velocity(p - e, 0.4, 3.0)[:, a]) / (2 * h)
+    assert np.max(np.abs(div)) < 1e-8
