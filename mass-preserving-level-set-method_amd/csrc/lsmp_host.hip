@@ -577,14 +577,16 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
                        &c->S0, &c->phi0, &c->dxv, &c->dyv, &c->gr};
     for (void **f : fields)
         if (alloc(f, fb)) return fail(LSMP_E_HIP);
-    // scratch: CRWENO (c', d') for n-2 rows of 2 systems per line (64 slots
-    // per 32 lines), or CCD (S, SS) for n+5 rows per line, whichever
-    // direction needs most
+    // scratch: CRWENO (c', d') for n-2 rows of 2 systems per line (x: 64
+    // slots per 32 lines), or for n-1-n/2 rows of 4 half systems per line
+    // (y, z: 64 slots per 16 lines), or CCD (S, SS) for n+5 rows per line,
+    // whichever direction needs most
     size_t sc = 0;
     const int ns[3] = {c->nx, c->ny, c->nz};
     for (int d = 0; d < c->dim; ++d) {
         const size_t nlines = c->ncell / (size_t)ns[d];
         sc = std::max(sc, (size_t)2 * (size_t)(ns[d] - 2) * ((nlines + 31) / 32 * 64));
+        sc = std::max(sc, (size_t)2 * (size_t)(ns[d] - 1 - ns[d] / 2) * ((nlines + 15) / 16 * 64));
         // CCD / UCCD checkpoints per line and CSEG-row segment, lanes padded to 256
         const size_t lanes = (nlines + 255) / 256 * 256, segs = (size_t)(ns[d] + 4) / CSEG + 1;
         sc = std::max(sc, (size_t)4 * segs * lanes);   // UCCD keeps 4 (CCD 2)

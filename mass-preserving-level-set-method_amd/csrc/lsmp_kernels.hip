@@ -38,46 +38,72 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
 // from WENO-JS, faces 1..n-1 from the two tridiagonal systems.
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
-// Paired line solve: two lanes per line.  Lane l < 32 of a wave solves the
-// positive-flux (g) system of line 32 w + l, lane l + 32 the negative-flux (f)
-// system of the same line.  An f row is the g row of the mirrored stencil
-// f(i+3..i-1) with A and C exchanged (crweno_flux_split :388-407 against
-// :362-381), so both halves run one instruction stream; the two boundary
-// faces come from WENO-JS per system (:383-384, 409-410).  Each lane runs the
-// reference's Thomas recurrence (TDMA.f90:1-36) on its system: the forward
-// sweep keeps (c', d') of rows 1..n-2 in scr[((m-1)*2 + comp)*nslot + slot],
-// slot = 64 w + lane (coalesced), and the backward sweep forms the flux
-// difference of each cell; the f lane hands its half to the g lane by a
-// cross-half shuffle and the g lane applies the RHS or the RK stage.
-//
-// Memory latency is hidden by prefetching into registers: the forward sweep
-// loads the AXC cells it will need a chunk ahead, the backward sweep the
-// scratch rows and RHS operands of the next AXB rows.
+// Shared by both line-solve layouts: an f row is the g row of the mirrored
+// stencil f(i+3..i-1) with A and C exchanged (crweno_flux_split :388-407
+// against :362-381), so lanes solving g and f systems run one instruction
+// stream; the two boundary faces come from WENO-JS per system (:383-384,
+// 409-410).  Each lane runs the reference's Thomas recurrence (TDMA.f90:1-36)
+// on its rows: the forward sweep keeps (c', d') in
+// scr[((row-1)*2 + comp)*nslot + slot], slot = 64 w + lane (coalesced), and
+// the backward sweep forms the flux difference of each cell; the f lane hands
+// its half to the g lane by a cross-half shuffle (lane ^ 32) and the g lane
+// applies the RHS or the RK stage.  Loads are prefetched into registers a
+// chunk of rows ahead (AXB rows of scratch in the x kernel's backward sweep).
 //
 // Line L starts at (L / lw) * ostride + (L % lw) * istride with element
 // stride estride (x lines: lw = nlines, istride = nx, estride = 1).
 // ---------------------------------------------------------------------------
-constexpr int AXC = 16;
 constexpr int AXB = 8;
 
+// ---------------------------------------------------------------------------
+// Twisted line solve: four lanes per line.  The rows 1..n-1 of each system
+// are eliminated from both ends at once and meet at the twist face k = n/2:
+// lanes 0-15 of a wave take the g system of lines 16 w + 0..15 from face 1
+// upward, lanes 16-31 the same g systems from face n-1 downward, lanes 32-47
+// and 48-63 the f systems likewise.  A lane that starts at the far end runs
+// the same code on the reversed line (primed cell c' = n + 1 - c): reversing
+// a line mirrors the stencil and exchanges A and C, which is what the f rows
+// already do, so every lane computes its rows as crweno_g_row of the window
+// in its own order, mirrored when isf != bot.  Each lane runs the Thomas
+// forward recurrence (TDMA.f90:1-36) over its len rows into scratch; the two
+// halves then solve the twist row together,
+//   (B - A c'_top - C c'_bot) X_k = S - A d'_top - C d'_bot,
+// and each back-substitutes over its own rows from X_k, emitting the flux
+// difference of every cell (the far-end lanes with the sign of the reversed
+// orientation).  Same arithmetic per row as one Thomas sweep, half the
+// sequential length, twice the lanes.  With twice the waves resident, the
+// rolling prefetch is shallower than the two-lane kernel's (TXC rows ahead in
+// the forward sweep, TXB in the backward one), which keeps the VGPRs low
+// enough for more waves per SIMD.
+// ---------------------------------------------------------------------------
+// rows of forward / backward prefetch: FP64 4 / 4 (about 100 VGPRs, four
+// waves per SIMD), FP32 8 / 4 (five waves); measured best at 256^3
+template <typename T> struct TwistDepth { static constexpr int C = 8, B = 4; };
+template <> struct TwistDepth<double> { static constexpr int C = 4, B = 4; };
+
 template <typename T, int MODE, int STAGE>
-__global__ __launch_bounds__(256) void k_adv_pair(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
-                                                   T *__restrict__ scr, int nlines, int n, int lw, long ostride,
-                                                   long istride, long estride, T dd, T dt)
+__global__ __launch_bounds__(256) void k_adv_twist(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
+                                                    T *__restrict__ scr, int nlines, int n, int lw, long ostride,
+                                                    long istride, long estride, T dd, T dt)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int nwaves = (nlines + 31) >> 5;
+    const int nwaves = (nlines + 15) >> 4;
     if (wave >= nwaves) return;   // whole waves only: the shuffles below need every lane
-    const bool isf = lane >= 32;
-    const int L = wave * 32 + (lane & 31);
+    constexpr int TXC = TwistDepth<T>::C, TXB = TwistDepth<T>::B;
+    const bool isf = lane >= 32, bot = (lane & 16) != 0, mir = isf != bot;
+    const int L = wave * 16 + (lane & 15);
     const bool act = L < nlines;
     const int Lc = act ? L : nlines - 1;   // idle lanes shadow the last line and store nothing
     const size_t base = (size_t)(Lc / lw) * (size_t)ostride + (size_t)(Lc % lw) * (size_t)istride;
     const size_t nslot = (size_t)nwaves * 64, slot = (size_t)wave * 64 + lane;
+    const int k = n / 2;                         // twist face
+    const int len = bot ? n - 1 - k : k - 1;     // rows this lane eliminates (>= 1 for n >= 4)
     const T inv_dd = T(1) / dd;
-    // Fortran cell c = 1..n -> element c - 1, clamped (zero-gradient ghosts)
-    auto at = [&](int c) -> size_t { return base + (size_t)clampi(c - 1, 0, n - 1) * (size_t)estride; };
+    const T sdd = bot ? -inv_dd : inv_dd;        // flux difference in this lane's orientation
+    // this lane's cell c' = 1..n -> element of the line, clamped (zero-gradient ghosts)
+    auto el = [&](int c) -> int { return clampi(bot ? n - c : c - 1, 0, n - 1); };
+    auto at = [&](int c) -> size_t { return base + (size_t)el(c) * (size_t)estride; };
     auto flux = [&](T p, T u) {
         T g, f;
         flux_split(p, u, g, f);
@@ -88,9 +114,7 @@ __global__ __launch_bounds__(256) void k_adv_pair(T *phi, const T *__restrict__ 
         return flux(phi[a], vel[a]);
     };
     auto emit = [&](size_t a, T t1, T t2, T Rv, T Pv, T L1v, T L2v) {
-        if (MODE == 0) {
-            R[a] = t1 + t2;
-        } else if (MODE == 1) {
+        if (MODE == 1) {
             R[a] = (Rv + t1) + t2;
         } else {
             const T r = (Rv + t1) + t2;
@@ -105,43 +129,30 @@ __global__ __launch_bounds__(256) void k_adv_pair(T *phi, const T *__restrict__ 
             }
         }
     };
-    auto emit_direct = [&](int c, T t) {   // boundary cells: operands loaded here
-        const T t2 = __shfl_xor(t, 32, 64);
-        if (isf || !act) return;
-        const size_t a = at(c);
-        T Rv = T(0), Pv = T(0), L1v = T(0), L2v = T(0);
-        if (MODE >= 1) Rv = R[a];
-        if (MODE == 2) Pv = phi[a];
-        if (MODE == 2 && STAGE >= 2) L1v = L1[a];
-        if (MODE == 2 && STAGE == 3) L2v = L2[a];
-        emit(a, t, t2, Rv, Pv, L1v, L2v);
-    };
 
-    // boundary faces of this lane's system
-    T bc0, bcN;
+    // boundary face where this lane starts: face 0 (top) or face n (bottom),
+    // from the WENO-JS closure of its system on the cells in natural order
+    T bc;
     {
-        T v[6];
+        T v[6], o[6];
 #pragma unroll
         for (int t = 0; t < 6; ++t) v[t] = ld(-2 + t);
-        bc0 = isf ? wenojs_f(v[1], v[2], v[3], v[4], v[5]) : wenojs_g(v[0], v[1], v[2], v[3], v[4]);
 #pragma unroll
-        for (int t = 0; t < 6; ++t) v[t] = ld(n - 2 + t);
-        bcN = isf ? wenojs_f(v[1], v[2], v[3], v[4], v[5]) : wenojs_g(v[0], v[1], v[2], v[3], v[4]);
+        for (int t = 0; t < 6; ++t) o[t] = bot ? v[5 - t] : v[t];
+        bc = isf ? wenojs_f(o[1], o[2], o[3], o[4], o[5]) : wenojs_g(o[0], o[1], o[2], o[3], o[4]);
     }
 
-    // forward sweep; w[t] holds this system's flux at cell m - 2 + t.  Row 1
-    // and row n - 1 carry the boundary faces; rows 2 .. n - 2 run in unrolled
-    // chunks (n >= 4, so there is at least one).
+    // forward sweep over rows j = 1..len; w[t] holds the flux at cell j - 2 + t
     T w[6];
 #pragma unroll
     for (int t = 0; t < 6; ++t) w[t] = ld(-1 + t);
     auto coeffs = [&](T &A, T &B, T &C, T &S) {
-        const T x0 = isf ? w[5] : w[0], x1 = isf ? w[4] : w[1], x2 = isf ? w[3] : w[2];
-        const T x3 = isf ? w[2] : w[3], x4 = isf ? w[1] : w[4];
+        const T x0 = mir ? w[5] : w[0], x1 = mir ? w[4] : w[1], x2 = mir ? w[3] : w[2];
+        const T x3 = mir ? w[2] : w[3], x4 = mir ? w[1] : w[4];
         T Ag, Cg;
         crweno_g_row_fast(x0, x1, x2, x3, x4, Ag, B, Cg, S);
-        A = isf ? Cg : Ag;
-        C = isf ? Ag : Cg;
+        A = mir ? Cg : Ag;
+        C = mir ? Ag : Cg;
     };
     auto shift = [&](T next) {
 #pragma unroll
@@ -149,11 +160,11 @@ __global__ __launch_bounds__(256) void k_adv_pair(T *phi, const T *__restrict__ 
         w[5] = next;
     };
     T cp, dp;
-    size_t o = slot;   // scratch offset of the current row: (m - 1) * 2 * nslot + slot
+    size_t o = slot;   // scratch offset of the current row: (j - 1) * 2 * nslot + slot
     {
         T A, B, C, S;
         coeffs(A, B, C, S);
-        S = S - A * bc0;
+        S = S - A * bc;
         const T rr = rcp_pos(B);
         cp = C * rr;
         dp = S * rr;
@@ -179,93 +190,112 @@ __global__ __launch_bounds__(256) void k_adv_pair(T *phi, const T *__restrict__ 
         dp = d;
         shift(next);
     };
-    // row m consumes cell m + 4; slot r of the rolling buffer serves rows
-    // m = 2 + r (mod AXC) and is reloaded AXC rows ahead as soon as it is used
-    T pp[AXC], pu[AXC];
+    // row j consumes cell j + 4; slot r of the rolling buffer serves rows
+    // j = 2 + r (mod TXC) and is reloaded TXC rows ahead as soon as it is used
+    T pp[TXC], pu[TXC];
     int m0 = 2;
-    if (m0 + AXC - 1 <= n - 2) {
+    if (m0 + TXC - 1 <= len) {
 #pragma unroll
-        for (int r = 0; r < AXC; ++r) {
+        for (int r = 0; r < TXC; ++r) {
             const size_t ad = at(m0 + 4 + r);
             pp[r] = phi[ad];
             pu[r] = vel[ad];
         }
     }
-    for (; m0 + AXC - 1 <= n - 2; m0 += AXC) {
+    for (; m0 + TXC - 1 <= len; m0 += TXC) {
 #pragma unroll
-        for (int r = 0; r < AXC; ++r) {
+        for (int r = 0; r < TXC; ++r) {
             const T next = flux(pp[r], pu[r]);
-            const size_t ad = at(m0 + AXC + 4 + r);   // clamped: reads past the line end are harmless
+            const size_t ad = at(m0 + TXC + 4 + r);   // clamped: reads past the line end are harmless
             pp[r] = phi[ad];
             pu[r] = vel[ad];
             mid(next);
         }
     }
-    for (int m = m0; m <= n - 2; ++m) mid(ld(m + 4));
+    for (int j = m0; j <= len; ++j) mid(ld(j + 4));
+
+    // twist row: face len + 1 of this lane = face k of the line
+    T X;
     {
         T A, B, C, S;
         coeffs(A, B, C, S);
-        S = S - C * bcN;
-        dp = (S - A * dp) * rcp_pos(B - A * cp);
+        const T oc = __shfl_xor(cp, 16, 64), od = __shfl_xor(dp, 16, 64);
+        const T Xk = (S - A * dp - C * od) * rcp_pos(B - A * cp - C * oc);
+        X = __shfl(Xk, lane & ~16, 64);   // both halves continue from the top lane's value
     }
 
-    // backward sweep: X(n-1) = d'(n-1), then X(m) = d'(m) - c'(m) X(m+1) for
-    // m = n - 2 .. 1, emitting cell m + 1 (element m of the line)
-    T X = dp;
-    emit_direct(n, (X - bcN) * inv_dd);
-    size_t ae = base + (size_t)(n - 2) * (size_t)estride;   // element of the row being emitted
+    // back-substitution: X(j) = d'(j) - c'(j) X(j+1) for j = len .. 1,
+    // emitting cell j + 1 of this lane (between its faces j and j + 1)
+    long ae = (long)(base + (size_t)el(len + 1) * (size_t)estride);
+    const long astep = bot ? estride : -estride;
     auto back = [&](T c, T d, T Rv, T Pv, T L1v, T L2v) {
         const T X1 = d - c * X;
-        const T t = (X1 - X) * inv_dd;
+        const T t = (X1 - X) * sdd;
         const T t2 = __shfl_xor(t, 32, 64);
-        if (!isf && act) emit(ae, t, t2, Rv, Pv, L1v, L2v);
-        ae -= (size_t)estride;
+        if (!isf && act) emit((size_t)ae, t, t2, Rv, Pv, L1v, L2v);
+        ae += astep;
         X = X1;
     };
-    // rolling buffer of the scratch rows and RHS operands, AXB rows ahead
-    T pc[AXB], pd[AXB], pR[AXB], pP[AXB], pL1[AXB], pL2[AXB];
-    auto load_b = [&](int r, int m) {   // row m >= 1 into slot r
-        const size_t ob = (size_t)(m - 1) * 2 * nslot + slot;
+    // rolling buffer of the scratch rows and RHS operands, TXB rows ahead
+    T pc[TXB], pd[TXB], pR[TXB], pP[TXB], pL1[TXB], pL2[TXB];
+    auto load_b = [&](int r, int j) {   // row j >= 1 into slot r
+        const size_t ob = (size_t)(j - 1) * 2 * nslot + slot;
         pc[r] = scr[ob];
         pd[r] = scr[ob + nslot];
-        if (MODE >= 1 && !isf) {
-            const size_t ad = base + (size_t)m * (size_t)estride;
+        if (!isf) {
+            const size_t ad = at(j + 1);
             pR[r] = R[ad];
             if (MODE == 2) pP[r] = phi[ad];
             if (MODE == 2 && STAGE >= 2) pL1[r] = L1[ad];
             if (MODE == 2 && STAGE == 3) pL2[r] = L2[ad];
         }
     };
-    m0 = n - 2;
-    if (m0 - AXB + 1 >= 1) {
+    m0 = len;
+    if (m0 - TXB + 1 >= 1) {
 #pragma unroll
-        for (int r = 0; r < AXB; ++r) load_b(r, m0 - r);
+        for (int r = 0; r < TXB; ++r) load_b(r, m0 - r);
     }
-    for (; m0 - AXB + 1 >= 1; m0 -= AXB) {
-        const bool more = m0 - 2 * AXB + 1 >= 1;   // the next chunk is full: reload
+    for (; m0 - TXB + 1 >= 1; m0 -= TXB) {
+        const bool more = m0 - 2 * TXB + 1 >= 1;   // the next chunk is full: reload
 #pragma unroll
-        for (int r = 0; r < AXB; ++r) {
+        for (int r = 0; r < TXB; ++r) {
             const T c = pc[r], d = pd[r], Rv = pR[r], Pv = pP[r], L1v = pL1[r], L2v = pL2[r];
-            if (more) load_b(r, m0 - AXB - r);
+            if (more) load_b(r, m0 - TXB - r);
             back(c, d, Rv, Pv, L1v, L2v);
         }
     }
-    for (int m = m0; m >= 1; --m) {   // tail
-        const size_t ob = (size_t)(m - 1) * 2 * nslot + slot;
+    for (int j = m0; j >= 1; --j) {   // tail
+        const size_t ob = (size_t)(j - 1) * 2 * nslot + slot;
         T Rv = T(0), Pv = T(0), L1v = T(0), L2v = T(0);
-        if (MODE >= 1 && !isf) {
-            Rv = R[ae];
-            if (MODE == 2) Pv = phi[ae];
-            if (MODE == 2 && STAGE >= 2) L1v = L1[ae];
-            if (MODE == 2 && STAGE == 3) L2v = L2[ae];
+        if (!isf) {
+            Rv = R[(size_t)ae];
+            if (MODE == 2) Pv = phi[(size_t)ae];
+            if (MODE == 2 && STAGE >= 2) L1v = L1[(size_t)ae];
+            if (MODE == 2 && STAGE == 3) L2v = L2[(size_t)ae];
         }
         back(scr[ob], scr[ob + nslot], Rv, Pv, L1v, L2v);
     }
-    emit_direct(1, (bc0 - X) * inv_dd);
+    // the boundary cell: between the boundary face and face 1 of this lane
+    {
+        const T t = (bc - X) * sdd;
+        const T t2 = __shfl_xor(t, 32, 64);
+        if (!isf && act) {
+            const size_t a = at(1);
+            T L1v = T(0), L2v = T(0);
+            const T Rv = R[a];
+            const T Pv = MODE == 2 ? phi[a] : T(0);
+            if (MODE == 2 && STAGE >= 2) L1v = L1[a];
+            if (MODE == 2 && STAGE == 3) L2v = L2[a];
+            emit(a, t, t2, Rv, Pv, L1v, L2v);
+        }
+    }
 }
 
-// The same paired solve for lines along x (contiguous in memory), first
-// direction only (R = t1 + t2).  Per wave, the 32 lines' cells arrive in
+// Paired solve for lines along x (contiguous in memory), first direction
+// only (R = t1 + t2): two lanes per line, lane l < 32 of a wave on the g
+// system of line 32 w + l, lane l + 32 on its f system, each over all rows
+// 1..n-1.  (A twisted x kernel measured no faster: these lines are already
+// read coalesced through LDS.)  Per wave, the 32 lines' cells arrive in
 // chunks of XW through an LDS tile that the wave loads cooperatively (16
 // consecutive elements per 128-byte segment, coalesced); the next chunk is
 // loaded into registers while the current one is consumed.  The backward
@@ -1369,10 +1399,11 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
     const long ostride = dir == 0 ? 0 : dir == 1 ? (long)nx * ny : (long)nx;
     const long istride = dir == 0 ? (long)nx : 1;
     const long estride = dir == 0 ? 1 : dir == 1 ? (long)nx : (long)nx * ny;
-    const size_t threads = (size_t)((nlines + 31) / 32) * 64;
-    const dim3 g((unsigned)((threads + 255) / 256)), b(256);
-#define LSMP_ADV(M, S)                                                                                        \
-    hipLaunchKernelGGL((k_adv_pair<T, M, S>), g, b, 0, st, phi, vel, R, L1, L2, scr, nlines, n, lw, ostride, \
+    const size_t threads = (size_t)((nlines + 31) / 32) * 64;       // two lanes per line (x)
+    const size_t threads4 = (size_t)((nlines + 15) / 16) * 64;      // four lanes per line (y, z)
+    const dim3 g((unsigned)((threads + 255) / 256)), g4((unsigned)((threads4 + 255) / 256)), b(256);
+#define LSMP_ADV(M, S)                                                                                          \
+    hipLaunchKernelGGL((k_adv_twist<T, M, S>), g4, b, 0, st, phi, vel, R, L1, L2, scr, nlines, n, lw, ostride, \
                        istride, estride, dd, dt)
     if (mode == 0) {
         hipLaunchKernelGGL((k_adv_pair_x<T>), g, b, 0, st, phi, vel, R, scr, nlines, n, dd);

@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "liblsmp.so")
+# another build of the library can be named by LSMP_LIBRARY
+LIB_PATH = os.environ.get("LSMP_LIBRARY") or os.path.join(HERE, "lib", "liblsmp.so")
 
 FP64, FP32 = 0, 1
 FALLBACK_RESTART, FALLBACK_CONTINUE = 0, 1

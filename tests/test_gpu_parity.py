@@ -59,6 +59,9 @@ CASES = [
     dict(nx=40, ny=20, nz=36, dim=3),
     dict(nx=48, dim=2),
     dict(nx=70, ny=50, dim=2),
+    # odd line lengths: the two halves of a twisted line solve differ by a row
+    dict(nx=23, ny=17, nz=19, dim=3),
+    dict(nx=45, ny=31, dim=2),
     # x lines of 640 cells: the CCD and UCCD coefficient tables exceed 64 KB and
     # are read from global memory instead of LDS
     dict(nx=640, ny=40, dim=2),
