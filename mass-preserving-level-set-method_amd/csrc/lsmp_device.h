@@ -206,12 +206,14 @@ template <typename T> LSMP_DEV T phyn(T a, T b, T c, T d)
 // alp2 = 3/q2 normalise to w0 = q1 q2 / D and w2 = 3 q0 q1 / D with
 // D = q1 q2 + 6 q0 q2 + 3 q0 q1.  Equal to phyn() up to rounding of the weights
 // (q_k lies in [1e-12, 3e6], so D neither under- nor overflows in FP32).
-template <typename T> LSMP_DEV T phyn_fast(T a, T b, T c, T d)
+// sbc = (b - c)^2 and scd = (c - d)^2 come from the caller: the two terms of
+// one direction share them (squares of opposite differences are equal).
+template <typename T> LSMP_DEV T phyn_fast(T a, T b, T c, T d, T sbc, T scd)
 {
     const T eps = T(1.0e-6);
     T is0 = T(13) * sq(a - b) + T(3) * sq(a - T(3) * b);
-    T is1 = T(13) * sq(b - c) + T(3) * sq(b + c);
-    T is2 = T(13) * sq(c - d) + T(3) * sq(T(3) * c - d);
+    T is1 = T(13) * sbc + T(3) * sq(b + c);
+    T is2 = T(13) * scd + T(3) * sq(T(3) * c - d);
     T q0 = sq(eps + is0), q1 = sq(eps + is1), q2 = sq(eps + is2);
     T q12 = q1 * q2, q01 = q0 * q1;
     T r = rcp_pos(q12 + T(6) * (q0 * q2) + T(3) * q01);
@@ -229,8 +231,10 @@ LSMP_DEV void weno_hj_pm_fast(T m3, T m2, T m1, T c, T p1, T p2, T p3, T inv12d,
     const T a = inv12d * (-(m1 - m2) + T(7) * (c - m1) + T(7) * (p1 - c) - (p2 - p1));
     const T sm2 = m3 - T(2) * m2 + m1, sm1 = m2 - T(2) * m1 + c, s0 = m1 - T(2) * c + p1;
     const T sp1 = c - T(2) * p1 + p2, sp2 = p1 - T(2) * p2 + p3;
-    dp = a + invd * phyn_fast(sp2, sp1, s0, sm1);
-    dm = a - invd * phyn_fast(sm2, sm1, s0, sp1);
+    // (sp1 - s0)^2 is is1 of D+ and is2 of D-; (s0 - sm1)^2 is2 of D+ and is1 of D-
+    const T qp = sq(sp1 - s0), qm = sq(s0 - sm1);
+    dp = a + invd * phyn_fast(sp2, sp1, s0, sm1, qp, qm);
+    dm = a - invd * phyn_fast(sm2, sm1, s0, sp1, qm, qp);
 }
 
 // ls_funs with the divisions by ls_wid and pi replaced by products with
