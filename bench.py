@@ -64,17 +64,74 @@ BYTES_V = {
 ADV_Z_V = (5 + 6 + 6) / 3.0            # last pass: R, w, phi (+ l1, l2) in, phi (+ l1/l2) out
 
 
-def measured_traffic(kernel, n, fp64):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    passes (FETCH_SIZE, WRITE_SIZE, calibrated on a copy of known size), or
-    None when no pass covers this kernel, size and precision."""
+# kernel names of the committed counter passes for each timed kernel
+PMC_NAMES = {
+    "redis_stage": "k_redis_stage3d", "adv_x": "k_adv_pair_x", "adv_y": "k_adv_twist<double, 1,",
+    "adv_z": "k_adv_twist<double, 2,", "ccd_x": "k_ccd_seg<double, 3, false", "ccd_y": "k_ccd_seg<double, 3, false",
+    "ccd_z": "k_ccd_seg<double, 3, true", "mass_update": "k_mass_update",
+}
+FP64_PEAK_TFS = 78.6
+
+
+def pmc_figures(kernel, n, fp64):
+    """(HBM bytes, FP64 flop) per launch of `kernel` from the committed
+    rocprofv3 counter passes (FETCH_SIZE and WRITE_SIZE calibrated on a copy of
+    known size; FP64 VALU instruction counts, FMA as two flop), or
+    (None, None) when no pass covers this kernel, size and precision."""
     path = os.path.join(ROOT, "profiles", f"pmc_traffic_{n}cube_{'fp64' if fp64 else 'fp32'}.json")
-    if not os.path.exists(path):
-        return None
+    if not os.path.exists(path) or kernel not in PMC_NAMES:
+        return None, None
     data = json.load(open(path))["kernels"]
-    names = {"redis_stage": "k_redis_stage3d"}.get(kernel, kernel)
-    hits = [v["traffic_bytes"] for k, v in data.items() if names in k]
-    return round(sum(hits) / len(hits)) if hits else None
+    hits = [v for k, v in data.items() if PMC_NAMES[kernel] in k]
+    if not hits:
+        return None, None
+    traffic = sum(v["traffic_bytes"] for v in hits) / len(hits)
+    flop = sum(v.get("fp64_flop_per_cell", 0.0) for v in hits) / len(hits) * n ** 3
+    return round(traffic), flop
+
+
+def measured_traffic(kernel, n, fp64):
+    return pmc_figures(kernel, n, fp64)[0]
+
+
+def stream_copy_gbs(nbytes=2 << 30, reps=10):
+    """HBM bandwidth of a device-to-device copy (bytes read + written per
+    second) through the HIP runtime: the peak measured on this box, reported
+    beside the published one.  None if the buffers cannot be allocated."""
+    import ctypes as C
+    hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    hip.hipEventDestroy.argtypes = [C.c_void_p]
+    a, b, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), nbytes) != 0:
+        return None
+    if hip.hipMalloc(C.byref(b), nbytes) != 0:
+        hip.hipFree(a)
+        return None
+    try:
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        D2D = 3   # hipMemcpyDeviceToDevice
+        hip.hipMemcpyAsync(b, a, nbytes, D2D, None)   # warm-up
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, nbytes, D2D, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        return round(2 * nbytes * reps / (ms.value * 1e-3) / 1e9, 1) if ms.value > 0 else None
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
 
 
 def run_steps(s, nsteps, t, imass, field, net=None):
@@ -188,6 +245,20 @@ def main():
     for k, v in kernels.items():
         b = (ADV_Z_V if k == "adv_z" else BYTES_V.get(k, 0.0)) * V * n ** 3
         v["algo_GBs"] = round(b / (v["avg_us"] * 1e-6) / 1e9, 1) if b else None
+        tb, fl = pmc_figures(k, n, prec == lsmp.FP64)
+        if tb:
+            v["traffic_GBs"] = round(tb / (v["avg_us"] * 1e-6) / 1e9, 1)
+        if fl:
+            v["fp64_TFs"] = round(fl / (v["avg_us"] * 1e-6) / 1e12, 2)
+    # on-box streaming ceiling: the runtime's device-to-device copy, or the
+    # library's own streaming kernel (mass update: 2 fields read, 1 written)
+    # at its counter-measured bytes, whichever moves more
+    copy_gbs = stream_copy_gbs()
+    mu = kernels.get("mass_update", {}).get("traffic_GBs")
+    stream, stream_src = max([(copy_gbs or 0.0, "hipMemcpy device-to-device"),
+                              (mu or 0.0, "k_mass_update, counter-measured bytes")])
+    stream = stream or None
+    _, dom_flop = pmc_figures(dom, n, prec == lsmp.FP64)
 
     out = {
         "metric": METRIC,
@@ -221,6 +292,15 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom, n, prec == lsmp.FP64),
+            # measured on this box: a device-to-device copy, bytes read + written
+            "stream_peak": stream,
+            "stream_source": stream_src if stream else None,
+            "copy_GBs": copy_gbs,
+            "frac_of_stream": round(achieved / stream, 4) if stream else None,
+            # the FP64 co-bound of the same kernel (FP64 VALU flop per launch
+            # from the counter passes over its average launch time)
+            "fp64": ({"achieved": round(dom_flop / avg_s / 1e12, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                      "frac": round(dom_flop / avg_s / 1e12 / FP64_PEAK_TFS, 4)} if dom_flop else None),
         },
         "kernels": kernels,
     }

@@ -150,6 +150,28 @@ int lsmp_component_masses(lsmp_handle h, double phi_core, double phi_min, int nq
                           int *query_comp, int maxcomp, double *comp_mass, int *ncomp);
 
 /* ------------------------------------------------------------------------
+ * Two-marker MPLS branch (mpls, interface_solver.f90:34-121): while the case
+ * is not merged, each marker (marker(mid)%lsf) is transported on its own --
+ * lsmp_marker_load(mid), lsmp_rk3_advect, lsmp_reinit(1), lsmp_mass_fix
+ * against that marker's initial mass, lsmp_marker_store(mid) -- then
+ * lsmp_combine_markers forms phi from the two and reports whether some cell is
+ * inside both, in which case the caller reinitialises phi once and continues
+ * with phi alone (glb%merged).
+ * ------------------------------------------------------------------------ */
+/* upload / download marker mid (1 or 2), a box like lsmp_upload_phi's
+ * (problem_init.f90:83-84 set them); LSMP_E_STATE for a marker never set */
+int lsmp_set_marker(lsmp_handle h, int mid, const double *box);
+int lsmp_get_marker(lsmp_handle h, int mid, double *box);
+/* phi <- marker mid (interface_solver.f90:44-56); marker mid <- phi (:64-76) */
+int lsmp_marker_load(lsmp_handle h, int mid);
+int lsmp_marker_store(lsmp_handle h, int mid);
+/* phi <- merge of the two markers (:85-111): where both have the same sign,
+ * the smaller inside both and the larger outside both, elsewhere the larger;
+ * *reconstruct = 1 when some cell is inside both (:96).  LSMP_E_STATE unless
+ * both markers are set. */
+int lsmp_combine_markers(lsmp_handle h, int *reconstruct);
+
+/* ------------------------------------------------------------------------
  * Learned reinitialisation.  No counterpart in the reference: it replaces
  * most pseudo-steps of level_set_rk3_redis (level_set_rk3_redis.f90:25-74)
  * by one pass of a small network on the narrow band, followed by a few plain

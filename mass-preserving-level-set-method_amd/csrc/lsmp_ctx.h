@@ -39,6 +39,9 @@ struct lsmp_ctx {
     int npartial = 0;
     int *band_list = nullptr;        // narrow-band cell indices (learned reinitialisation)
     unsigned *band_count = nullptr;
+    void *mk[2] = {nullptr, nullptr};   // the two markers' fields (two-marker MPLS branch)
+    unsigned *mflag = nullptr;          // "inside both markers" flag of the merge
+    bool mk_set[2] = {false, false};
     bool vel_set = false, phi0_set = false;
     lsmp_stats stats{};
     bool profiling = false;

@@ -514,7 +514,8 @@ template <typename T> int velocity_analytic(lsmp_ctx *c, int field, double t, do
 void free_all(lsmp_ctx *c)
 {
     void *bufs[] = {c->rctl, c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
-                    c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count};
+                    c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count,
+                    c->mk[0], c->mk[1], c->mflag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
@@ -546,7 +547,7 @@ const char *lsmp_error_string(int code)
     case LSMP_E_HIP: return g_errbuf[0] ? g_errbuf : "HIP runtime error";
     case LSMP_E_NAN: return "non-finite value in the level-set field";
     case LSMP_E_EMPTY_BAND: return "mass correction: no cell inside the interface band";
-    case LSMP_E_STATE: return "call out of order (velocity or phi0 not set)";
+    case LSMP_E_STATE: return "call out of order (velocity, phi0 or marker not set)";
     default: return "unknown error";
     }
 }
@@ -650,6 +651,71 @@ int lsmp_store_phi0(lsmp_handle h)
     LSMP_HIP(hipMemcpyAsync(h->phi0, h->phi, h->ncell * h->esz, hipMemcpyDeviceToDevice, h->st));
     LSMP_HIP(hipStreamSynchronize(h->st));
     h->phi0_set = true;
+    return LSMP_OK;
+}
+
+// ---- two-marker MPLS branch (interface_solver.f90:34-121) ----
+
+int lsmp_set_marker(lsmp_handle h, int mid, const double *box)
+{
+    if (!h || !box || (mid != 1 && mid != 2)) return LSMP_E_INVALID;
+    if (!h->mk[0]) {   // both markers and the merge flag, all or nothing, on first use
+        void *a = nullptr, *b = nullptr, *f = nullptr;
+        if (alloc(&a, h->ncell * h->esz) || alloc(&b, h->ncell * h->esz) || alloc(&f, sizeof(unsigned))) {
+            for (void *q : {a, b, f})
+                if (q) (void)hipFree(q);
+            return LSMP_E_HIP;
+        }
+        h->mk[0] = a;
+        h->mk[1] = b;
+        h->mflag = (unsigned *)f;
+    }
+    const int rc = upload_field(h, h->mk[mid - 1], box);
+    if (rc == LSMP_OK) h->mk_set[mid - 1] = true;
+    return rc;
+}
+
+int lsmp_get_marker(lsmp_handle h, int mid, double *box)
+{
+    if (!h || !box || (mid != 1 && mid != 2)) return LSMP_E_INVALID;
+    if (!h->mk_set[mid - 1]) return LSMP_E_STATE;
+    return download_field(h, h->mk[mid - 1], box);
+}
+
+int lsmp_marker_load(lsmp_handle h, int mid)
+{
+    if (!h || (mid != 1 && mid != 2)) return LSMP_E_INVALID;
+    if (!h->mk_set[mid - 1]) return LSMP_E_STATE;
+    LSMP_HIP(hipMemcpyAsync(h->phi, h->mk[mid - 1], h->ncell * h->esz, hipMemcpyDeviceToDevice, h->st));
+    return LSMP_OK;
+}
+
+int lsmp_marker_store(lsmp_handle h, int mid)
+{
+    if (!h || (mid != 1 && mid != 2)) return LSMP_E_INVALID;
+    if (!h->mk[0]) return LSMP_E_STATE;
+    LSMP_HIP(hipMemcpyAsync(h->mk[mid - 1], h->phi, h->ncell * h->esz, hipMemcpyDeviceToDevice, h->st));
+    h->mk_set[mid - 1] = true;
+    return LSMP_OK;
+}
+
+int lsmp_combine_markers(lsmp_handle h, int *reconstruct)
+{
+    if (!h || !reconstruct) return LSMP_E_INVALID;
+    if (!h->mk_set[0] || !h->mk_set[1]) return LSMP_E_STATE;
+    LSMP_HIP(hipMemsetAsync(h->mflag, 0, sizeof(unsigned), h->st));
+    if (h->fp32)
+        launch_combine_markers<float>(h->st, (const float *)h->mk[0], (const float *)h->mk[1], (float *)h->phi,
+                                      h->ncell, h->mflag);
+    else
+        launch_combine_markers<double>(h->st, (const double *)h->mk[0], (const double *)h->mk[1], (double *)h->phi,
+                                       h->ncell, h->mflag);
+    LSMP_HIP(hipGetLastError());
+    LSMP_HIP(hipMemcpyAsync(h->hbuf, h->mflag, sizeof(unsigned), hipMemcpyDeviceToHost, h->st));
+    LSMP_HIP(hipStreamSynchronize(h->st));
+    unsigned f;
+    memcpy(&f, h->hbuf, sizeof f);
+    *reconstruct = f != 0u;
     return LSMP_OK;
 }
 

@@ -46,6 +46,8 @@ template <typename T>
 void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const T *vel, T *R, T *L1, T *L2,
                      T *scr, int nx, int ny, int nz, T dd, T dt);
 template <typename T>
+void launch_combine_markers(hipStream_t st, const T *m1, const T *m2, T *phi, size_t n, unsigned *flag);
+template <typename T>
 void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *phi, T *phi_out, const T *phi0,
                         const T *S0, const RedisArgs &ra, unsigned long long *ctl);
 template <typename T> void launch_sign_init(hipStream_t st, const T *phi, T *S0, size_t n, T ls_wid, int raw);

@@ -1549,6 +1549,36 @@ void launch_velocity(hipStream_t st, T *u, T *v, T *w, const VelArgs &va, double
     hipLaunchKernelGGL((k_velocity<T>), dim3(nblk(n, 256)), dim3(256), 0, st, u, v, w, va, tab);
 }
 
+// Merge of the two markers of the MPLS two-marker branch
+// (interface_solver.f90:85-111): where both have the same sign the merged phi
+// is the smaller inside both and the larger outside both, elsewhere the
+// larger; flag gets 1 when some cell is inside both.  The tests run in FP64
+// as the reference's do (a product of two FP32 values is exact in FP64).
+template <typename T>
+__global__ void k_combine_markers(const T *__restrict__ m1, const T *__restrict__ m2, T *__restrict__ phi,
+                                  size_t n, unsigned *__restrict__ flag)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;   // whole waves return together past the end (q grows with the lane)
+    const double a = (double)m1[q], b = (double)m2[q];
+    bool both = false;
+    double out;
+    if (a * b > 0.0) {
+        both = a + b > 0.0;
+        out = both ? fmin(a, b) : fmax(a, b);
+    } else {
+        out = fmax(a, b);
+    }
+    phi[q] = (T)out;
+    if (__any(both) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+template <typename T>
+void launch_combine_markers(hipStream_t st, const T *m1, const T *m2, T *phi, size_t n, unsigned *flag)
+{
+    hipLaunchKernelGGL((k_combine_markers<T>), dim3(nblk(n, 256)), dim3(256), 0, st, m1, m2, phi, n, flag);
+}
+
 template <typename TO, typename TI>
 void launch_convert(hipStream_t st, TO *out, const TI *in, size_t n)
 {
@@ -1568,6 +1598,7 @@ void launch_zero_u64(hipStream_t st, unsigned long long *p, int n)
 #define LSMP_INST(T)                                                                                           \
     template void launch_adv_pass<T>(hipStream_t, int, int, int, T *, const T *, T *, T *, T *, T *, int, int, int, \
                                      T, T);                                                                    \
+    template void launch_combine_markers<T>(hipStream_t, const T *, const T *, T *, size_t, unsigned *);         \
     template void launch_redis_stage<T>(hipStream_t, int, int, int, const T *, T *, const T *, const T *,          \
                                         const RedisArgs &, unsigned long long *);                              \
     template void launch_sign_init<T>(hipStream_t, const T *, T *, size_t, T, int);                            \

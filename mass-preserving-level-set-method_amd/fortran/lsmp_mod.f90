@@ -117,6 +117,34 @@ interface
         type(c_ptr), value :: h
         integer(c_int), value :: scheme
     end function
+    ! two-marker MPLS branch (interface_solver.f90:34-121)
+    integer(c_int) function c_lsmp_set_marker(h, mid, box) bind(C, name="lsmp_set_marker")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int), value :: mid
+        real(c_double), intent(in) :: box(*)
+    end function
+    integer(c_int) function c_lsmp_get_marker(h, mid, box) bind(C, name="lsmp_get_marker")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int), value :: mid
+        real(c_double), intent(out) :: box(*)
+    end function
+    integer(c_int) function c_lsmp_marker_load(h, mid) bind(C, name="lsmp_marker_load")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int), value :: mid
+    end function
+    integer(c_int) function c_lsmp_marker_store(h, mid) bind(C, name="lsmp_marker_store")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int), value :: mid
+    end function
+    integer(c_int) function c_lsmp_combine_markers(h, reconstruct) bind(C, name="lsmp_combine_markers")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int), intent(out) :: reconstruct
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
         import :: c_size_t, c_ptr
         type(c_ptr), value :: s
@@ -127,6 +155,7 @@ public :: c_lsmp_init, c_lsmp_finalize, c_lsmp_upload_phi, c_lsmp_download_phi, 
 public :: c_lsmp_set_velocity, c_lsmp_set_velocity_analytic, c_lsmp_rk3_advect, c_lsmp_reinit
 public :: c_lsmp_mass_vol, c_lsmp_mass_fix, c_lsmp_interface_l1, c_lsmp_get_stats, c_lsmp_synchronize
 public :: c_lsmp_set_advection_scheme
+public :: c_lsmp_set_marker, c_lsmp_get_marker, c_lsmp_marker_load, c_lsmp_marker_store, c_lsmp_combine_markers
 public :: lsmp_check, lsmp_error_message
 
 contains

@@ -5,7 +5,8 @@ Case: sphere r = 0.15 at (0.35, 0.35, 0.35) in the unit cube, velocity of
 Vortex/src/setup/velocity.f90 with t2s = T, dt = rdt = 0.1 dx, ls_wid =
 1.5 dx, MPLS every step (Vortex/src/main.f90 loop with t2red = 1).  Prints
 one JSON line: ms/step, cell-updates/s, L1 shape error (output.f90 metric),
-|dM|/M, pseudo-steps per step and fallbacks of the learned form.
+|dM|/M, pseudo-steps per step and fallbacks of the learned form, and the same
+timings over steps 1-50 and over 25 steps around t = T/2.
 
     python mass-preserving-level-set-method_amd/leveque.py --n 256 --form fp32-learned
 """
@@ -52,9 +53,20 @@ def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsm
     nsteps = int(round(T / dt * frac))
     t = 0.0
     pseudo = fallbacks = 0
+    # timing windows: the first 50 steps (compact sphere) and 25 steps around
+    # t = T/2 (maximum stretch), beside the whole-run mean
+    half = int(round(0.5 * T / dt))
+    windows = {"steps_1_50": (1, min(50, nsteps))}
+    if half + 12 <= nsteps:
+        windows["around_T_half"] = (half - 12, half + 12)
+    marks, wres = {}, {}
     s.synchronize()
     t0 = time.perf_counter()
     for it in range(nsteps):
+        for name, (a, b) in windows.items():
+            if it + 1 == a:
+                s.synchronize()
+                marks[name] = (time.perf_counter(), pseudo)
         t += dt
         s.set_velocity_analytic(lsmp.VEL_LEVEQUE3D, t, T)
         s.rk3_advect()
@@ -65,6 +77,14 @@ def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsm
             _, k = s.reinit(1)
         pseudo += k
         s.mass_fix(imass)
+        for name, (a, b) in windows.items():
+            if it + 1 == b:
+                s.synchronize()
+                w0, p0 = marks[name]
+                el_w = time.perf_counter() - w0
+                wres[name] = dict(steps=b - a + 1, ms_per_step=el_w / (b - a + 1) * 1e3,
+                                  cell_updates_per_s=n ** 3 * (b - a + 1) / el_w,
+                                  pseudo_per_step=(pseudo - p0) / (b - a + 1))
         if log and (it + 1) % max(1, nsteps // 10) == 0:
             log(f"  {form} {n}^3 step {it + 1}/{nsteps} t={t:.3f}")
     s.synchronize()
@@ -72,7 +92,7 @@ def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsm
     mass, _ = s.mass_vol()
     out = dict(form=form, n=n, T=T, t_end=t, steps=nsteps, ms_per_step=el / nsteps * 1e3,
                cell_updates_per_s=n ** 3 * nsteps / el, l1=s.interface_l1(), dM_over_M=abs(mass - imass) / imass,
-               pseudo_per_step=pseudo / nsteps, fallbacks=fallbacks)
+               pseudo_per_step=pseudo / nsteps, fallbacks=fallbacks, windows=wres)
     s.close()
     if net is not None:
         net.close()

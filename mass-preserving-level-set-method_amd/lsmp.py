@@ -33,6 +33,7 @@ EXPORTS = [
     "lsmp_reinit", "lsmp_mass_vol", "lsmp_mass_fix", "lsmp_interface_l1", "lsmp_get_stats",
     "lsmp_set_profiling", "lsmp_kernel_time", "lsmp_synchronize", "lsmp_set_advection_scheme",
     "lsmp_component_masses",
+    "lsmp_set_marker", "lsmp_get_marker", "lsmp_marker_load", "lsmp_marker_store", "lsmp_combine_markers",
     "lsmp_net_create", "lsmp_net_destroy", "lsmp_net_param_count", "lsmp_net_set_params",
     "lsmp_net_get_params", "lsmp_net_dataset_size", "lsmp_net_collect", "lsmp_net_train",
     "lsmp_net_eval_grad", "lsmp_reinit_learned", "lsmp_reinit_residual", "lsmp_net_add_samples",
@@ -81,6 +82,11 @@ def load_library(path=LIB_PATH):
     L.lsmp_error_string.restype = C.c_char_p
     L.lsmp_upload_phi.argtypes = [H, _DP]
     L.lsmp_download_phi.argtypes = [H, _DP]
+    L.lsmp_set_marker.argtypes = [H, C.c_int, _DP]
+    L.lsmp_get_marker.argtypes = [H, C.c_int, _DP]
+    L.lsmp_marker_load.argtypes = [H, C.c_int]
+    L.lsmp_marker_store.argtypes = [H, C.c_int]
+    L.lsmp_combine_markers.argtypes = [H, C.POINTER(C.c_int)]
     L.lsmp_store_phi0.argtypes = [H]
     L.lsmp_set_velocity.argtypes = [H, _DP, _DP, _DP]
     L.lsmp_set_velocity_analytic.argtypes = [H, C.c_int, C.c_double, C.c_double]
@@ -305,6 +311,58 @@ class Solver:
                                                     fallback=fallback)
         _, nm = self.mass_fix(imass)
         return npseudo, nm
+
+    # -- two-marker MPLS branch (interface_solver.f90:34-121)
+    def set_marker(self, mid, box):
+        _check(self.lib.lsmp_set_marker(self.h, int(mid), _box_ptr(np.ascontiguousarray(box, np.float64))))
+
+    def get_marker(self, mid):
+        out = np.empty(self.shape, np.float64)
+        _check(self.lib.lsmp_get_marker(self.h, int(mid), _box_ptr(out)))
+        return out
+
+    def marker_load(self, mid):
+        """phi <- marker mid (interface_solver.f90:44-56)."""
+        _check(self.lib.lsmp_marker_load(self.h, int(mid)))
+
+    def marker_store(self, mid):
+        """marker mid <- phi (interface_solver.f90:64-76)."""
+        _check(self.lib.lsmp_marker_store(self.h, int(mid)))
+
+    def combine_markers(self):
+        """phi <- merge of the two markers (interface_solver.f90:85-111);
+        True when some cell is inside both."""
+        r = C.c_int()
+        _check(self.lib.lsmp_combine_markers(self.h, C.byref(r)))
+        return bool(r.value)
+
+    def mpls_step_markers(self, t, t2s, field, imass, marker_imass, merged, t2red=1, iteration=1):
+        """One MPLS step as mpls runs it (interface_solver.f90:34-128) after
+        the driver set the velocity at t^{n+1}: while not merged, each marker
+        is advected, reinitialised and mass corrected against its own initial
+        mass, then phi is their merge, reinitialised once when they overlap.
+        Returns (merged, n_pseudo, n_mass) with the counts summed over the
+        fields transported."""
+        self.set_velocity_analytic(field, t, t2s)
+        if merged:
+            npseudo, nm = 0, 0
+            self.rk3_advect()
+            if iteration % t2red == 0:
+                _, npseudo = self.reinit(1)
+            _, nm = self.mass_fix(imass)
+            return True, npseudo, nm
+        npseudo = nmass = 0
+        for mid in (1, 2):
+            self.marker_load(mid)
+            self.rk3_advect()
+            if iteration % t2red == 0:
+                npseudo += self.reinit(1)[1]
+            nmass += self.mass_fix(marker_imass[mid - 1])[1]
+            self.marker_store(mid)
+        if self.combine_markers():
+            npseudo += self.reinit(1)[1]
+            merged = True
+        return merged, npseudo, nmass
 
 
 NF, NH = 128, 64

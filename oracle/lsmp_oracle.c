@@ -26,6 +26,8 @@ struct orc_state {
     double *f[ORC_NFIELDS];
     /* scalars (glb) */
     double mass, vol, massv, volv, imass, ivol, red_error;
+    double imass1, imass2;  /* marker(1:2)%imass */
+    int merged;             /* glb%merged */
     double time;
     int iter;
     int last_redis_iters, last_mass_iters;
@@ -82,6 +84,9 @@ double orc_get(const orc_state *s, const char *n)
     if (!strcmp(n, "massv")) return s->massv;
     if (!strcmp(n, "volv")) return s->volv;
     if (!strcmp(n, "imass")) return s->imass;
+    if (!strcmp(n, "imass1")) return s->imass1;
+    if (!strcmp(n, "imass2")) return s->imass2;
+    if (!strcmp(n, "merged")) return s->merged;
     if (!strcmp(n, "ivol")) return s->ivol;
     if (!strcmp(n, "red_error")) return s->red_error;
     if (!strcmp(n, "time")) return s->time;
@@ -96,6 +101,9 @@ double orc_get(const orc_state *s, const char *n)
 void orc_set(orc_state *s, const char *n, double v)
 {
     if (!strcmp(n, "imass")) s->imass = v;
+    if (!strcmp(n, "imass1")) s->imass1 = v;
+    if (!strcmp(n, "imass2")) s->imass2 = v;
+    if (!strcmp(n, "merged")) s->merged = (int)v;
     else if (!strcmp(n, "ivol")) s->ivol = v;
     else if (!strcmp(n, "time")) s->time = v;
     else if (!strcmp(n, "iter")) s->iter = (int)v;
@@ -1199,4 +1207,57 @@ void orc_mpls_step(orc_state *s, int field, double t2s, int t2red)
     if (s->iter % t2red == 0) orc_redis(s, 1, -1.0);
     orc_mass_fix(s);
     orc_ls_mv(s);
+}
+
+int orc_combine_markers(orc_state *s)
+{
+    /* interface_solver.f90:85-111, over the whole box including ghosts */
+    const double *m1 = s->f[ORC_MK1], *m2 = s->f[ORC_MK2];
+    double *phi = s->f[ORC_PHI];
+    int reconstruct = 0;
+    for (long q = 0; q < s->nbox; ++q) {
+        const double tmp1 = m1[q], tmp2 = m2[q];
+        reconstruct = reconstruct || (tmp1 * tmp2 > 0.0 && tmp1 + tmp2 > 0.0);   /* :96 */
+        if (tmp1 * tmp2 > 0.0) {
+            if (tmp1 + tmp2 > 0.0) phi[q] = fmin(tmp1, tmp2);                    /* :99 */
+            else phi[q] = fmax(tmp1, tmp2);                                       /* :101 */
+        } else {
+            phi[q] = fmax(tmp1, tmp2);                                            /* :104 */
+        }
+    }
+    orc_bc(s, phi);   /* :111 */
+    return reconstruct;
+}
+
+void orc_mpls_step_markers(orc_state *s, int field, double t2s, int t2red)
+{
+    /* main.f90:17-26 */
+    if (s->time + s->p.dt > t2s) s->p.dt = t2s - s->time + 1.0e-10;
+    s->time = s->time + s->p.dt;
+    s->iter = s->iter + 1;
+    if (field >= 0) orc_velocity(s, field, s->time, t2s);
+    s->last_redis_iters = 0;
+    if (!s->merged) {                                                       /* interface_solver.f90:34 */
+        for (int mid = 1; mid <= 2; ++mid) {
+            /* :38-42: the marker's initial mass stands in for the global one */
+            const double keep = s->imass;
+            s->imass = mid == 1 ? s->imass1 : s->imass2;
+            double *mk = s->f[mid == 1 ? ORC_MK1 : ORC_MK2];
+            memcpy(s->f[ORC_PHI], mk, (size_t)s->nbox * sizeof(double));   /* :44-56 */
+            orc_rk3_advect(s);                                              /* :60 */
+            if (s->iter % t2red == 0) orc_redis(s, 1, -1.0);                /* :61 */
+            orc_mass_fix(s);                                                /* :62 */
+            memcpy(mk, s->f[ORC_PHI], (size_t)s->nbox * sizeof(double));   /* :64-76 */
+            s->imass = keep;                                                /* :80-81 */
+        }
+        if (orc_combine_markers(s)) {                                       /* :85-116 */
+            orc_redis(s, 1, -1.0);                                          /* :119 */
+            s->merged = 1;                                                  /* :120 */
+        }
+    } else {
+        orc_rk3_advect(s);                                                  /* :125-127 */
+        if (s->iter % t2red == 0) orc_redis(s, 1, -1.0);
+        orc_mass_fix(s);
+    }
+    orc_ls_mv(s);                                                           /* main.f90:34 */
 }

@@ -43,6 +43,7 @@ enum {
     ORC_S3, ORC_L1, ORC_L2, ORC_L3,
     ORC_NX, ORC_NY, ORC_NZ, ORC_NXX, ORC_NYY, ORC_NZZ,
     ORC_VOF,
+    ORC_MK1, ORC_MK2,    /* the two markers' level-set fields (marker(mid)%lsf) */
     ORC_NFIELDS
 };
 
@@ -106,6 +107,18 @@ void orc_store_initial(orc_state *s);
  * mod(iter,t2red)==0, mass fix, ls_mv.  field as in orc_velocity; field < 0
  * keeps the current u,v,w. */
 void orc_mpls_step(orc_state *s, int field, double t2s, int t2red);
+/* interface_solver.f90:85-111: phi <- merge of MK1 and MK2 over the whole
+ * box (same sign: the smaller inside both, the larger outside both; else the
+ * larger); returns 1 when some cell is inside both markers. */
+int orc_combine_markers(orc_state *s);
+/* one MPLS step of the two-marker branch (main.f90:17-34 with
+ * interface_solver.f90:34-128): while "merged" is 0, each marker in turn is
+ * copied into phi, advected, reinitialised (iter mod t2red == 0) and mass
+ * corrected against its own initial mass ("imass1", "imass2"), and copied
+ * back; phi then becomes the merge of the two, reinitialised once and
+ * "merged" set when they overlap.  Once merged, the one-field step.  ls_mv at
+ * the end. */
+void orc_mpls_step_markers(orc_state *s, int field, double t2s, int t2red);
 
 #ifdef __cplusplus
 }

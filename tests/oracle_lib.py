@@ -23,6 +23,7 @@ FIELDS = [
     "S3", "L1", "L2", "L3",
     "NX", "NY", "NZ", "NXX", "NYY", "NZZ",
     "VOF",
+    "MK1", "MK2",
 ]
 F = {n: i for i, n in enumerate(FIELDS)}
 
@@ -84,6 +85,8 @@ def lib():
         L.orc_velocity.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.orc_initial_phi.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_mpls_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.orc_combine_markers.argtypes = [C.c_void_p]
+        L.orc_mpls_step_markers.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
         _lib = L
     return _lib
 
