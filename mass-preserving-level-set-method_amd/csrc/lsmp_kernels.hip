@@ -502,6 +502,205 @@ __global__ __launch_bounds__(256) void k_adv_pair_x(const T *__restrict__ phi, c
     flush(0);
 }
 
+// Partitioned solve for lines of 34..257 cells: one wave per line, lanes
+// 0-31 on its g system and 32-63 on its f system; MODE and STAGE as in
+// k_adv_twist (0: R = t1 + t2, 1: R += t1 + t2, 2: RK stage in place).  With K = ceil((n-1)/32) (2..8), lane s of a half owns the
+// rows lo = sK+1 .. hi = (s+1)K-1 and the interface row (s+1)K; the last
+// lane owns the rows up to n-1 and no interface row.  Each lane runs the
+// Thomas recurrence (TDMA.f90:1-36) over its rows with the two neighbouring
+// interface values L and R as parameters, X_i = p_i + q_i L + r_i R, keeping
+// (c', d', e') of its rows in registers; every interface row then becomes one
+// row of a tridiagonal system in the interface values, one per lane, solved
+// by parallel cyclic reduction across the 32 lanes of the half (identity rows
+// stand in beyond the last interface); each lane back-substitutes its rows
+// from L and R and emits the flux differences of cells lo..hi+1.  Nothing
+// goes to scratch memory.  Same rows as the reference's (an f row is the g
+// row of the mirrored stencil with A and C exchanged); the solve differs from
+// one Thomas sweep only in rounding.  In MODE 2 the in-place update is safe:
+// a line's cells are read by its own wave only, all before the reduction.
+constexpr int PKX = 8;   // rows per lane at most: lines of up to 257 cells
+constexpr int PART_MAX_LINES_X = 32768, PART_MAX_LINES_STRIDED = 4096;
+
+template <typename T, int MODE, int STAGE>
+__global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
+                                                   int nlines, int n, int lw, long ostride, long istride,
+                                                   long estride, T dd, T dt)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int line = tid >> 6, lane = tid & 63;
+    if (line >= nlines) return;   // a whole wave per line
+    const bool isf = lane >= 32;
+    const int s = lane & 31;
+    const int N = n - 1, K = (N + 31) / 32;
+    const int lo = s * K + 1;
+    const bool active = lo <= N;
+    const bool last = (s + 1) * K >= N;          // no interface row: the right value is face n
+    const int hi = last ? N : (s + 1) * K - 1;
+    const int nrow = active ? hi - lo + 1 : 0;   // 1..K
+    const size_t base = (size_t)(line / lw) * (size_t)ostride + (size_t)(line % lw) * (size_t)istride;
+    const T inv_dd = T(1) / dd;
+    auto at = [&](int c) -> size_t { return base + (size_t)clampi(c - 1, 0, n - 1) * (size_t)estride; };
+    auto emit = [&](int c, T t1, T t2) {   // cell c of the line
+        const size_t a = at(c);
+        if (MODE == 0) {
+            R[a] = t1 + t2;
+        } else if (MODE == 1) {
+            R[a] = (R[a] + t1) + t2;
+        } else {
+            const T r = (R[a] + t1) + t2, Pv = phi[a];
+            if (STAGE == 1) {
+                L1[a] = r;
+                phi[a] = Pv + r * dt;
+            } else if (STAGE == 2) {
+                L2[a] = r;
+                phi[a] = Pv + (T(-3) * L1[a] + r) / T(4) * dt;
+            } else {
+                phi[a] = Pv + (-L1[a] - L2[a] + T(8) * r) / T(12) * dt;
+            }
+        }
+    };
+    auto flux = [&](int c) {
+        const size_t a = at(c);
+        T g, f;
+        flux_split(phi[a], vel[a], g, f);
+        return isf ? f : g;
+    };
+    // fl[t] = flux at cell lo - 2 + t: the stencils of rows lo .. lo + PKX (the
+    // last one is the interface row of a full segment)
+    T fl[PKX + 6];
+#pragma unroll
+    for (int t = 0; t < PKX + 6; ++t) fl[t] = flux(lo - 2 + t);
+    auto coeffs = [&](int j, T &A, T &B, T &C, T &S) {   // row lo + j
+        const T x0 = isf ? fl[j + 5] : fl[j], x1 = isf ? fl[j + 4] : fl[j + 1], x2 = isf ? fl[j + 3] : fl[j + 2];
+        const T x3 = isf ? fl[j + 2] : fl[j + 3], x4 = isf ? fl[j + 1] : fl[j + 4];
+        T Ag, Cg;
+        crweno_g_row_fast(x0, x1, x2, x3, x4, Ag, B, Cg, S);
+        A = isf ? Cg : Ag;
+        C = isf ? Ag : Cg;
+    };
+    // boundary faces of this system, where a lane needs them
+    T bc0 = T(0), bcN = T(0);
+    if (s == 0) {
+        T v[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) v[t] = flux(-2 + t);
+        bc0 = isf ? wenojs_f(v[1], v[2], v[3], v[4], v[5]) : wenojs_g(v[0], v[1], v[2], v[3], v[4]);
+    }
+    if (active && last) {
+        T v[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) v[t] = flux(n - 2 + t);
+        bcN = isf ? wenojs_f(v[1], v[2], v[3], v[4], v[5]) : wenojs_g(v[0], v[1], v[2], v[3], v[4]);
+    }
+
+    // local sweep: X_i = d'_i + e'_i L - c'_i X_{i+1}; the row after the last
+    // (j == nrow) is the interface row of a lane that has one
+    const bool irow = active && !last;
+    T cp[PKX], dp[PKX], ep[PKX];
+    T A = T(0), B = T(1), C = T(0), S = T(0);   // interface row
+#pragma unroll
+    for (int j = 0; j < PKX; ++j) {
+        if (j < nrow || (irow && j == nrow)) {
+            T a_, b_, c_, s_;
+            coeffs(j, a_, b_, c_, s_);
+            if (j == nrow) {
+                A = a_;
+                B = b_;
+                C = c_;
+                S = s_;
+            } else if (j == 0) {
+                const T r = rcp_pos(b_);
+                cp[j] = c_ * r;
+                dp[j] = s_ * r;
+                ep[j] = -a_ * r;
+            } else {
+                const T r = rcp_pos(b_ - a_ * cp[j - 1]);
+                cp[j] = c_ * r;
+                dp[j] = (s_ - a_ * dp[j - 1]) * r;
+                ep[j] = -a_ * ep[j - 1] * r;
+            }
+        }
+    }
+    // (p, q, r) of the last row and of the first
+    T ph = T(0), qh = T(1), rh = T(0), pl = T(0), ql = T(0), rl = T(1);
+#pragma unroll
+    for (int j = PKX - 1; j >= 0; --j) {
+        if (j < nrow) {
+            if (j == nrow - 1) {
+                pl = dp[j];
+                ql = ep[j];
+                rl = -cp[j];
+                ph = pl;
+                qh = ql;
+                rh = rl;
+            } else {
+                pl = dp[j] - cp[j] * pl;
+                ql = ep[j] - cp[j] * ql;
+                rl = -cp[j] * rl;
+            }
+        }
+    }
+    // interface row (s+1)K: X_hi of this lane and X_lo of the next enter it
+    T ra = T(0), rb = T(1), rc = T(0), rd = T(0);
+    const T npl = __shfl(pl, (lane + 1) & 63, 64), nql = __shfl(ql, (lane + 1) & 63, 64);
+    const T nrl = __shfl(rl, (lane + 1) & 63, 64), nbcN = __shfl(bcN, (lane + 1) & 63, 64);
+    const bool next_last = (s + 2) * K >= N;
+    if (irow) {
+        ra = A * qh;
+        rb = B + A * rh + C * nql;
+        rc = C * nrl;
+        rd = S - A * ph - C * npl;
+        if (s == 0) {
+            rd = rd - ra * bc0;
+            ra = T(0);
+        }
+        if (next_last) {
+            rd = rd - rc * nbcN;
+            rc = T(0);
+        }
+    }
+    // parallel cyclic reduction over the 32 lanes of the half
+#pragma unroll
+    for (int st = 1; st < 32; st <<= 1) {
+        const bool hasm = s - st >= 0, hasp = s + st < 32;
+        const int lm = hasm ? lane - st : lane, lp = hasp ? lane + st : lane;
+        T am = __shfl(ra, lm, 64), bm = __shfl(rb, lm, 64), cm = __shfl(rc, lm, 64), dm = __shfl(rd, lm, 64);
+        T ap = __shfl(ra, lp, 64), bp = __shfl(rb, lp, 64), cq = __shfl(rc, lp, 64), dq = __shfl(rd, lp, 64);
+        if (!hasm) { am = T(0); bm = T(1); cm = T(0); dm = T(0); }
+        if (!hasp) { ap = T(0); bp = T(1); cq = T(0); dq = T(0); }
+        const T k1 = ra / bm, k2 = rc / bp;
+        const T na = -am * k1, nc = -cq * k2;
+        const T nb = rb - cm * k1 - ap * k2, nd = rd - dm * k1 - dq * k2;
+        ra = na;
+        rb = nb;
+        rc = nc;
+        rd = nd;
+    }
+    const T Xb = rd / rb;
+    const T Xprev = __shfl(Xb, (lane - 1) & 63, 64);
+    const T Lv = s == 0 ? bc0 : Xprev;
+    const T Rv = last ? bcN : Xb;
+
+    // back-substitution and the flux differences of cells lo .. hi + 1
+    T x = Rv;
+#pragma unroll
+    for (int j = PKX - 1; j >= 0; --j) {
+        if (j < nrow) {
+            const T xn = dp[j] + ep[j] * Lv - cp[j] * x;
+            const T t = (xn - x) * inv_dd;
+            const T t2 = __shfl_xor(t, 32, 64);
+            if (!isf) emit(lo + j + 1, t, t2);
+            x = xn;
+        }
+    }
+    {
+        const T t = (Lv - x) * inv_dd;
+        const T t2 = __shfl_xor(t, 32, 64);
+        if (!isf && active) emit(lo, t, t2);
+    }
+}
+
+
 // ---------------------------------------------------------------------------
 // Reinitialisation stage, 3D (level_set_rk3_redis.f90:176-264 with the
 // gradient :266-369, lambda :371-454 and ls_funs ns_solver/others.f90:1-45).
@@ -1405,13 +1604,33 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
 #define LSMP_ADV(M, S)                                                                                          \
     hipLaunchKernelGGL((k_adv_twist<T, M, S>), g4, b, 0, st, phi, vel, R, L1, L2, scr, nlines, n, lw, ostride, \
                        istride, estride, dd, dt)
+    // Few lines: the paired and twisted kernels leave most SIMDs idle and each
+    // lane walks the whole line, so the partitioned kernel (one wave per
+    // line, rows in registers) takes over.  Measured: 2D 256^2 x pass
+    // 129 -> 15 us, y pass 99 -> 21 us; 128^3 x pass 70 -> 67 us (FP32
+    // 58 -> 49); but strided y and z lines of 128^3 (16384 lines) 50 -> 103 us,
+    // its loads being one line per lane, and at 256^3 the paired x kernel is
+    // as fast (FP64) or faster (FP32 182 against 220 us).
+    const bool part = n >= 34 && n <= 32 * PKX + 1 &&
+                      nlines <= (dir == 0 ? PART_MAX_LINES_X : PART_MAX_LINES_STRIDED);
+    const dim3 gp((unsigned)(((size_t)nlines * 64 + 255) / 256));
+#define LSMP_PART(M, S)                                                                                      \
+    hipLaunchKernelGGL((k_adv_part<T, M, S>), gp, b, 0, st, phi, vel, R, L1, L2, nlines, n, lw, ostride, \
+                       istride, estride, dd, dt)
     if (mode == 0) {
-        hipLaunchKernelGGL((k_adv_pair_x<T>), g, b, 0, st, phi, vel, R, scr, nlines, n, dd);
+        if (part) LSMP_PART(0, 0);
+        else hipLaunchKernelGGL((k_adv_pair_x<T>), g, b, 0, st, phi, vel, R, scr, nlines, n, dd);
+    } else if (part) {
+        if (mode == 1) LSMP_PART(1, 1);
+        else if (stage == 1) LSMP_PART(2, 1);
+        else if (stage == 2) LSMP_PART(2, 2);
+        else LSMP_PART(2, 3);
     } else if (mode == 1) LSMP_ADV(1, 1);
     else if (stage == 1) LSMP_ADV(2, 1);
     else if (stage == 2) LSMP_ADV(2, 2);
     else LSMP_ADV(2, 3);
 #undef LSMP_ADV
+#undef LSMP_PART
 }
 
 template <typename T>
