@@ -1273,6 +1273,215 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
     }
 }
 
+// CCD line solve for few lines (the 2D grids): 32 lanes per line, two lines
+// per wave.  With the factorisation fixed per row, both sweeps are affine
+// recurrences whose 2x2 matrices are the same for every line: forward
+// y_b = r_b - M_b y_{b-1}, M_b = [[A, B], [AA, BB]] of row b (the frow
+// update); backward x_b = Q_b (f_b - N_b x_{b+1}), N_b the row's third
+// block and Q_b its 2x2 solve (TWIN_BKS, matrix_solver.f90:47-78).  Lane s
+// owns the forward rows lo..hi (at most KCC of rows 1..nb-2).  Each lane
+// composes its rows into one affine map, an inclusive scan over the 32 lanes
+// (five shuffle rounds) gives every lane the state entering its rows, and
+// the lane re-runs its rows from that state with the reference's per-row
+// arithmetic; the backward sweep does the same from the closure row down.
+// Only the states entering each lane's rows carry the scan's rounding.
+constexpr int KCC = 9;   // rows per lane at most: lines of up to 282 cells
+template <typename T, int DIM, bool FINAL, bool TLDS>
+__global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, const T *__restrict__ coef,
+                                                   T *__restrict__ out, const T *__restrict__ dxv,
+                                                   const T *__restrict__ dyv, T *__restrict__ gr,
+                                                   double *__restrict__ partial, int nlines, int n, int lw,
+                                                   long ostride, long istride, long estride, CcdArgs ca)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char ccd_lds[];
+    const int nb = n + 6;
+    const T *tab = coef;
+    if (TLDS) {
+        T *t = reinterpret_cast<T *>(ccd_lds);
+        for (int i = threadIdx.x; i < 13 * nb; i += blockDim.x) t[i] = coef[i];
+        __syncthreads();
+        tab = t;
+    }
+    const T *A = tab, *B = tab + 3 * nb, *AA = tab + 6 * nb, *BB = tab + 9 * nb, *IDEN = tab + 12 * nb;
+    const int lane = threadIdx.x & 63, s = lane & 31, half = lane & 32;
+    const int line = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
+    const bool act = line < nlines;
+    const int Lc = act ? line : nlines - 1;   // idle halves shadow the last line and emit nothing
+    const size_t base = (size_t)(Lc / lw) * (size_t)ostride + (size_t)(Lc % lw) * (size_t)istride;
+    const T idx = T(1) / (T)ca.dx, idx2 = T(1) / ((T)ca.dx * (T)ca.dx);
+    auto ldq = [&](int q) -> T { return phi[base + (size_t)clampi(q, 0, n - 1) * (size_t)estride]; };
+    const int NR = nb - 2, K = (NR + 31) / 32;
+    const int lo = 1 + s * K, hi = min(lo + K - 1, NR);
+    const int nrow = hi >= lo ? hi - lo + 1 : 0;
+
+    // row 0: one-sided closure, the state the forward sweep starts from
+    T S0, SS0;
+    {
+        const T a0 = ldq(-3), a1 = ldq(-2), a2 = ldq(-1), a3 = ldq(0);
+        S0 = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) * idx;
+        SS0 = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) * idx2;
+    }
+    T f[KCC + 2];   // phi(b-4) for b = lo .. lo + KCC + 1
+#pragma unroll
+    for (int r = 0; r < KCC + 2; ++r) f[r] = ldq(lo - 4 + r);
+    auto rhs = [&](int r, T &rs, T &rss) {   // the row's right-hand side from f(b-1), f(b), f(b+1)
+        rs = T(15.0 / 16.0) * (-f[r] + f[r + 2]) * idx;
+        rss = (T(3) * f[r] - T(6) * f[r + 1] + T(3) * f[r + 2]) * idx2;
+    };
+
+    // forward: this lane's rows as y_out = P y_in + v
+    T p00 = T(1), p01 = T(0), p10 = T(0), p11 = T(1), v0 = T(0), v1 = T(0);
+#pragma unroll
+    for (int r = 0; r < KCC; ++r) {
+        if (r < nrow) {
+            const int b = lo + r;
+            const T m00 = A[3 * b], m01 = B[3 * b], m10 = AA[3 * b], m11 = BB[3 * b];
+            T rs, rss;
+            rhs(r, rs, rss);
+            const T nv0 = rs - (m00 * v0 + m01 * v1), nv1 = rss - (m10 * v0 + m11 * v1);
+            const T q00 = -(m00 * p00 + m01 * p10), q01 = -(m00 * p01 + m01 * p11);
+            const T q10 = -(m10 * p00 + m11 * p10), q11 = -(m10 * p01 + m11 * p11);
+            v0 = nv0; v1 = nv1;
+            p00 = q00; p01 = q01; p10 = q10; p11 = q11;
+        }
+    }
+    // inclusive scan: map of lanes 0..s composed
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const int src = s >= d ? lane - d : lane;
+        const T a00 = __shfl(p00, src, 64), a01 = __shfl(p01, src, 64), a10 = __shfl(p10, src, 64);
+        const T a11 = __shfl(p11, src, 64), w0 = __shfl(v0, src, 64), w1 = __shfl(v1, src, 64);
+        if (s >= d) {   // this ∘ earlier
+            const T nv0 = p00 * w0 + p01 * w1 + v0, nv1 = p10 * w0 + p11 * w1 + v1;
+            const T q00 = p00 * a00 + p01 * a10, q01 = p00 * a01 + p01 * a11;
+            const T q10 = p10 * a00 + p11 * a10, q11 = p10 * a01 + p11 * a11;
+            v0 = nv0; v1 = nv1;
+            p00 = q00; p01 = q01; p10 = q10; p11 = q11;
+        }
+    }
+    const T e0 = p00 * S0 + p01 * SS0 + v0, e1 = p10 * S0 + p11 * SS0 + v1;   // state after this lane's rows
+    const T in0 = __shfl(e0, (lane - 1) & 63, 64), in1 = __shfl(e1, (lane - 1) & 63, 64);
+    const T last0 = __shfl(e0, half + 31, 64), last1 = __shfl(e1, half + 31, 64);   // after row nb-2
+    // this lane's forward states with the reference's per-row arithmetic
+    T fs[KCC], fss[KCC];
+    {
+        T cs = s == 0 ? S0 : in0, css = s == 0 ? SS0 : in1;
+#pragma unroll
+        for (int r = 0; r < KCC; ++r) {
+            if (r < nrow) {
+                const int b = lo + r;
+                T rs, rss;
+                rhs(r, rs, rss);
+                rs = rs - (A[3 * b] * cs + B[3 * b] * css);
+                rss = rss - (AA[3 * b] * cs + BB[3 * b] * css);
+                cs = rs;
+                css = rss;
+            }
+            fs[r] = cs;
+            fss[r] = css;
+        }
+    }
+    // closure row nb-1 and its 2x2 solve (the same on every lane of the half)
+    T XS, XSS;
+    {
+        const int b = nb - 1;
+        const T g0 = ldq(b - 3), g1 = ldq(b - 4), g2 = ldq(b - 5), g3 = ldq(b - 6);
+        T s_ = -(T(-3.5) * g0 + T(4) * g1 - T(0.5) * g2) * idx;
+        T ss_ = (T(34.0 / 3.0) * g0 - T(83.0 / 4.0) * g1 + T(10) * g2 - T(7.0 / 12.0) * g3) * idx2;
+        s_ = s_ - (A[3 * b] * last0 + B[3 * b] * last1);
+        ss_ = ss_ - (AA[3 * b] * last0 + BB[3 * b] * last1);
+        const T iden = IDEN[b];
+        XS = -(B[3 * b + 1] * ss_ - BB[3 * b + 1] * s_) * iden;
+        XSS = (A[3 * b + 1] * ss_ - AA[3 * b + 1] * s_) * iden;
+    }
+    // backward: this lane's rows as x_lo = G x_{hi+1} + w
+    T g00 = T(1), g01 = T(0), g10 = T(0), g11 = T(1), w0 = T(0), w1 = T(0);
+#pragma unroll
+    for (int r = KCC - 1; r >= 0; --r) {
+        if (r < nrow) {
+            const int b = lo + r;
+            const T n00 = A[3 * b + 2], n01 = B[3 * b + 2], n10 = AA[3 * b + 2], n11 = BB[3 * b + 2];
+            const T iden = IDEN[b];
+            const T q00 = BB[3 * b + 1] * iden, q01 = -B[3 * b + 1] * iden;
+            const T q10 = -AA[3 * b + 1] * iden, q11 = A[3 * b + 1] * iden;
+            // w <- Q (f - N w), G <- -Q N G
+            const T t0 = fs[r] - (n00 * w0 + n01 * w1), t1 = fss[r] - (n10 * w0 + n11 * w1);
+            const T nw0 = q00 * t0 + q01 * t1, nw1 = q10 * t0 + q11 * t1;
+            const T k00 = n00 * g00 + n01 * g10, k01 = n00 * g01 + n01 * g11;
+            const T k10 = n10 * g00 + n11 * g10, k11 = n10 * g01 + n11 * g11;
+            g00 = -(q00 * k00 + q01 * k10); g01 = -(q00 * k01 + q01 * k11);
+            g10 = -(q10 * k00 + q11 * k10); g11 = -(q10 * k01 + q11 * k11);
+            w0 = nw0; w1 = nw1;
+        }
+    }
+    // inclusive suffix scan: map of lanes s..31 composed (this ∘ later)
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const int src = s + d < 32 ? lane + d : lane;
+        const T a00 = __shfl(g00, src, 64), a01 = __shfl(g01, src, 64), a10 = __shfl(g10, src, 64);
+        const T a11 = __shfl(g11, src, 64), u0 = __shfl(w0, src, 64), u1 = __shfl(w1, src, 64);
+        if (s + d < 32) {
+            const T nw0 = g00 * u0 + g01 * u1 + w0, nw1 = g10 * u0 + g11 * u1 + w1;
+            const T q00 = g00 * a00 + g01 * a10, q01 = g00 * a01 + g01 * a11;
+            const T q10 = g10 * a00 + g11 * a10, q11 = g10 * a01 + g11 * a11;
+            w0 = nw0; w1 = nw1;
+            g00 = q00; g01 = q01; g10 = q10; g11 = q11;
+        }
+    }
+    const T lo0 = g00 * XS + g01 * XSS + w0, lo1 = g10 * XS + g11 * XSS + w1;   // x at this lane's first row
+    const T nx0 = __shfl(lo0, (lane + 1) & 63, 64), nx1 = __shfl(lo1, (lane + 1) & 63, 64);
+    // this lane's rows, top down, with the reference's arithmetic from x_{hi+1}
+    double acc[3] = {0.0, 0.0, 0.0};
+    T S = s == 31 ? XS : nx0, SS = s == 31 ? XSS : nx1;
+#pragma unroll
+    for (int r = KCC - 1; r >= 0; --r) {
+        if (r < nrow) {
+            const int b = lo + r;
+            const T s_ = fs[r] - (A[3 * b + 2] * S + B[3 * b + 2] * SS);
+            const T ss_ = fss[r] - (AA[3 * b + 2] * S + BB[3 * b + 2] * SS);
+            const T iden = IDEN[b];
+            const T solS = -(B[3 * b + 1] * ss_ - BB[3 * b + 1] * s_) * iden;
+            const T solSS = (A[3 * b + 1] * ss_ - AA[3 * b + 1] * s_) * iden;
+            S = solS;
+            SS = solSS;
+            const int q = b - 3;
+            if (act && q >= 0 && q < n) {
+                const size_t a = base + (size_t)q * (size_t)estride;
+                if (!FINAL) {
+                    out[a] = S;
+                } else {
+                    const T dxa = dxv[a], dya = DIM == 3 ? dyv[a] : T(0), pa = phi[a];
+                    T g;
+                    if (DIM == 3) g = sqrt(sq(dxa) + sq(dya) + sq(S));
+                    else g = sqrt(sq(dxa) + sq(S));
+                    gr[a] = g;
+                    double h, dd;
+                    ls_funs<double>((double)pa, ca.ls_wid, h, dd);
+                    const double gg = (double)g;
+                    const double r12 = ca.rho_12;
+                    double rho, plam;
+                    acc[0] += h * ca.dv;
+                    if (!ca.inverse) rho = h + r12 * (1.0 - h);
+                    else rho = r12 * h + (1.0 - h);
+                    acc[1] += rho * h * ca.dv;
+                    plam = dd * dd * gg;
+                    if (ca.inverse) plam = plam * (2.0 * (r12 - 1.0) * h + 1.0) * ca.dx * ca.dy * ca.dz;
+                    else plam = plam * (2.0 * (1.0 - r12) * h + r12) * ca.dx * ca.dy * ca.dz;
+                    acc[2] += plam;
+                }
+            }
+        }
+    }
+    if (FINAL) {
+        block_sum<3>(acc);
+        if (threadIdx.x == 0) {
+            partial[3 * blockIdx.x + 0] = acc[0];
+            partial[3 * blockIdx.x + 1] = acc[1];
+            partial[3 * blockIdx.x + 2] = acc[2];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // UCCD advection (level_set_rk3_source_ccd, level_set_rk3_solver.f90:223-286;
 // root_solve "uccd", ccd_solvers.f90:428-438): per line, the upwind (U) and
@@ -1699,11 +1908,28 @@ void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T 
             hipLaunchKernelGGL((k_ccd_seg<T, D, F, false>), g, b, 0, st, phi, coef, scr, out, dxv, dyv, gr,      \
                                partial, nlines, n, lw, ostride, istride, estride, ca);                         \
     } while (0)
-    if (!final_pass) LSMP_CCD(3, false);
+    // few lines (the 2D grids): 32 lanes per line instead of one
+    const bool part = nlines <= PART_MAX_LINES_STRIDED && n + 4 <= 32 * KCC;
+    const dim3 gp(nblk(nlines, 8));   // two lines per wave, four waves per block
+#define LSMP_CCDP(D, F)                                                                                        \
+    do {                                                                                                       \
+        if (lds)                                                                                               \
+            hipLaunchKernelGGL((k_ccd_part<T, D, F, true>), gp, b, shm, st, phi, coef, out, dxv, dyv, gr,        \
+                               partial, nlines, n, lw, ostride, istride, estride, ca);                         \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_ccd_part<T, D, F, false>), gp, b, 0, st, phi, coef, out, dxv, dyv, gr,         \
+                               partial, nlines, n, lw, ostride, istride, estride, ca);                         \
+    } while (0)
+    if (part) {
+        if (!final_pass) LSMP_CCDP(3, false);
+        else if (dim == 3) LSMP_CCDP(3, true);
+        else LSMP_CCDP(2, true);
+    } else if (!final_pass) LSMP_CCD(3, false);
     else if (dim == 3) LSMP_CCD(3, true);
     else LSMP_CCD(2, true);
 #undef LSMP_CCD
-    if (nblocks_out) *nblocks_out = final_pass ? (int)g.x : 0;
+#undef LSMP_CCDP
+    if (nblocks_out) *nblocks_out = final_pass ? (int)(part ? gp.x : g.x) : 0;
 }
 
 template <typename T>

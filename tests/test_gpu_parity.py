@@ -258,6 +258,25 @@ def test_minimum_size_grids():
         s.close()
 
 
+def test_mass_fix_minimum_size_grids():
+    """The CCD normals of the mass correction on the smallest grids, where
+    most of the 32 segments of a line are empty."""
+    for case in (dict(nx=4, dim=3), dict(nx=4, ny=5, dim=2), dict(nx=5, ny=4, nz=6, dim=3), dict(nx=7, dim=2)):
+        say("minimum mass fix", case)
+        o, s = pair(**case)
+        o.initial_phi(2 if case["dim"] == 3 else 1, 0)
+        o.store_initial()
+        imass = o.get("imass")
+        o.field("PHI")[:] += 0.1 * o.p.dx
+        s.upload_phi(o.field("PHI"))
+        no = o.mass_fix()
+        _, ng = s.mass_fix(imass)
+        e = rel_err(s.download_phi(), o.field("PHI"))
+        say(f"  iterations {no}/{ng}  rel err {e:.3e}")
+        assert abs(ng - no) <= 1 and e <= TOL
+        s.close()
+
+
 def test_empty_band_is_an_error():
     """No cell inside the band: the reference divides by zero
     (mass_preserving_level_set.f90:46); the library reports it."""
