@@ -518,10 +518,10 @@ __global__ __launch_bounds__(256) void k_adv_pair_x(const T *__restrict__ phi, c
 // row of the mirrored stencil with A and C exchanged); the solve differs from
 // one Thomas sweep only in rounding.  In MODE 2 the in-place update is safe:
 // a line's cells are read by its own wave only, all before the reduction.
-constexpr int PKX = 8;   // rows per lane at most: lines of up to 257 cells
+constexpr int PKX = 16;   // rows per lane at most (8 up to 257 cells, 16 up to 513)
 constexpr int PART_MAX_LINES_X = 32768, PART_MAX_LINES_STRIDED = 4096;
 
-template <typename T, int MODE, int STAGE>
+template <typename T, int MODE, int STAGE, int KR>
 __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
                                                    int nlines, int n, int lw, long ostride, long istride,
                                                    long estride, T dd, T dt)
@@ -565,11 +565,11 @@ __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ 
         flux_split(phi[a], vel[a], g, f);
         return isf ? f : g;
     };
-    // fl[t] = flux at cell lo - 2 + t: the stencils of rows lo .. lo + PKX (the
+    // fl[t] = flux at cell lo - 2 + t: the stencils of rows lo .. lo + KR (the
     // last one is the interface row of a full segment)
-    T fl[PKX + 6];
+    T fl[KR + 6];
 #pragma unroll
-    for (int t = 0; t < PKX + 6; ++t) fl[t] = flux(lo - 2 + t);
+    for (int t = 0; t < KR + 6; ++t) fl[t] = flux(lo - 2 + t);
     auto coeffs = [&](int j, T &A, T &B, T &C, T &S) {   // row lo + j
         const T x0 = isf ? fl[j + 5] : fl[j], x1 = isf ? fl[j + 4] : fl[j + 1], x2 = isf ? fl[j + 3] : fl[j + 2];
         const T x3 = isf ? fl[j + 2] : fl[j + 3], x4 = isf ? fl[j + 1] : fl[j + 4];
@@ -596,10 +596,10 @@ __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ 
     // local sweep: X_i = d'_i + e'_i L - c'_i X_{i+1}; the row after the last
     // (j == nrow) is the interface row of a lane that has one
     const bool irow = active && !last;
-    T cp[PKX], dp[PKX], ep[PKX];
+    T cp[KR], dp[KR], ep[KR];
     T A = T(0), B = T(1), C = T(0), S = T(0);   // interface row
 #pragma unroll
-    for (int j = 0; j < PKX; ++j) {
+    for (int j = 0; j < KR; ++j) {
         if (j < nrow || (irow && j == nrow)) {
             T a_, b_, c_, s_;
             coeffs(j, a_, b_, c_, s_);
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ 
     // (p, q, r) of the last row and of the first
     T ph = T(0), qh = T(1), rh = T(0), pl = T(0), ql = T(0), rl = T(1);
 #pragma unroll
-    for (int j = PKX - 1; j >= 0; --j) {
+    for (int j = KR - 1; j >= 0; --j) {
         if (j < nrow) {
             if (j == nrow - 1) {
                 pl = dp[j];
@@ -684,7 +684,7 @@ __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ 
     // back-substitution and the flux differences of cells lo .. hi + 1
     T x = Rv;
 #pragma unroll
-    for (int j = PKX - 1; j >= 0; --j) {
+    for (int j = KR - 1; j >= 0; --j) {
         if (j < nrow) {
             const T xn = dp[j] + ep[j] * Lv - cp[j] * x;
             const T t = (xn - x) * inv_dd;
@@ -1285,8 +1285,8 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
 // the lane re-runs its rows from that state with the reference's per-row
 // arithmetic; the backward sweep does the same from the closure row down.
 // Only the states entering each lane's rows carry the scan's rounding.
-constexpr int KCC = 9;   // rows per lane at most: lines of up to 282 cells
-template <typename T, int DIM, bool FINAL, bool TLDS>
+constexpr int KCC = 17;   // rows per lane at most (9 up to 284 cells, 17 up to 540)
+template <typename T, int DIM, bool FINAL, bool TLDS, int KC>
 __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, const T *__restrict__ coef,
                                                    T *__restrict__ out, const T *__restrict__ dxv,
                                                    const T *__restrict__ dyv, T *__restrict__ gr,
@@ -1321,9 +1321,9 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
         S0 = (T(-3.5) * a0 + T(4) * a1 - T(0.5) * a2) * idx;
         SS0 = (T(34.0 / 3.0) * a0 - T(83.0 / 4.0) * a1 + T(10) * a2 - T(7.0 / 12.0) * a3) * idx2;
     }
-    T f[KCC + 2];   // phi(b-4) for b = lo .. lo + KCC + 1
+    T f[KC + 2];   // phi(b-4) for b = lo .. lo + KC + 1
 #pragma unroll
-    for (int r = 0; r < KCC + 2; ++r) f[r] = ldq(lo - 4 + r);
+    for (int r = 0; r < KC + 2; ++r) f[r] = ldq(lo - 4 + r);
     auto rhs = [&](int r, T &rs, T &rss) {   // the row's right-hand side from f(b-1), f(b), f(b+1)
         rs = T(15.0 / 16.0) * (-f[r] + f[r + 2]) * idx;
         rss = (T(3) * f[r] - T(6) * f[r + 1] + T(3) * f[r + 2]) * idx2;
@@ -1332,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     // forward: this lane's rows as y_out = P y_in + v
     T p00 = T(1), p01 = T(0), p10 = T(0), p11 = T(1), v0 = T(0), v1 = T(0);
 #pragma unroll
-    for (int r = 0; r < KCC; ++r) {
+    for (int r = 0; r < KC; ++r) {
         if (r < nrow) {
             const int b = lo + r;
             const T m00 = A[3 * b], m01 = B[3 * b], m10 = AA[3 * b], m11 = BB[3 * b];
@@ -1363,11 +1363,11 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     const T in0 = __shfl(e0, (lane - 1) & 63, 64), in1 = __shfl(e1, (lane - 1) & 63, 64);
     const T last0 = __shfl(e0, half + 31, 64), last1 = __shfl(e1, half + 31, 64);   // after row nb-2
     // this lane's forward states with the reference's per-row arithmetic
-    T fs[KCC], fss[KCC];
+    T fs[KC], fss[KC];
     {
         T cs = s == 0 ? S0 : in0, css = s == 0 ? SS0 : in1;
 #pragma unroll
-        for (int r = 0; r < KCC; ++r) {
+        for (int r = 0; r < KC; ++r) {
             if (r < nrow) {
                 const int b = lo + r;
                 T rs, rss;
@@ -1397,7 +1397,7 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     // backward: this lane's rows as x_lo = G x_{hi+1} + w
     T g00 = T(1), g01 = T(0), g10 = T(0), g11 = T(1), w0 = T(0), w1 = T(0);
 #pragma unroll
-    for (int r = KCC - 1; r >= 0; --r) {
+    for (int r = KC - 1; r >= 0; --r) {
         if (r < nrow) {
             const int b = lo + r;
             const T n00 = A[3 * b + 2], n01 = B[3 * b + 2], n10 = AA[3 * b + 2], n11 = BB[3 * b + 2];
@@ -1434,7 +1434,7 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     double acc[3] = {0.0, 0.0, 0.0};
     T S = s == 31 ? XS : nx0, SS = s == 31 ? XSS : nx1;
 #pragma unroll
-    for (int r = KCC - 1; r >= 0; --r) {
+    for (int r = KC - 1; r >= 0; --r) {
         if (r < nrow) {
             const int b = lo + r;
             const T s_ = fs[r] - (A[3 * b + 2] * S + B[3 * b + 2] * SS);
@@ -1822,10 +1822,17 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
     // as fast (FP64) or faster (FP32 182 against 220 us).
     const bool part = n >= 34 && n <= 32 * PKX + 1 &&
                       nlines <= (dir == 0 ? PART_MAX_LINES_X : PART_MAX_LINES_STRIDED);
+    const bool part8 = n <= 32 * 8 + 1;   // eight rows per lane suffice
     const dim3 gp((unsigned)(((size_t)nlines * 64 + 255) / 256));
-#define LSMP_PART(M, S)                                                                                      \
-    hipLaunchKernelGGL((k_adv_part<T, M, S>), gp, b, 0, st, phi, vel, R, L1, L2, nlines, n, lw, ostride, \
-                       istride, estride, dd, dt)
+#define LSMP_PART(M, S)                                                                                       \
+    do {                                                                                                      \
+        if (part8)                                                                                            \
+            hipLaunchKernelGGL((k_adv_part<T, M, S, 8>), gp, b, 0, st, phi, vel, R, L1, L2, nlines, n, lw,     \
+                               ostride, istride, estride, dd, dt);                                            \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_adv_part<T, M, S, PKX>), gp, b, 0, st, phi, vel, R, L1, L2, nlines, n, lw,   \
+                               ostride, istride, estride, dd, dt);                                            \
+    } while (0)
     if (mode == 0) {
         if (part) LSMP_PART(0, 0);
         else hipLaunchKernelGGL((k_adv_pair_x<T>), g, b, 0, st, phi, vel, R, scr, nlines, n, dd);
@@ -1910,15 +1917,21 @@ void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T 
     } while (0)
     // few lines (the 2D grids): 32 lanes per line instead of one
     const bool part = nlines <= PART_MAX_LINES_STRIDED && n + 4 <= 32 * KCC;
+    const bool part9 = n + 4 <= 32 * 9;
     const dim3 gp(nblk(nlines, 8));   // two lines per wave, four waves per block
-#define LSMP_CCDP(D, F)                                                                                        \
+#define LSMP_CCDQ(D, F, KC)                                                                                    \
     do {                                                                                                       \
         if (lds)                                                                                               \
-            hipLaunchKernelGGL((k_ccd_part<T, D, F, true>), gp, b, shm, st, phi, coef, out, dxv, dyv, gr,        \
+            hipLaunchKernelGGL((k_ccd_part<T, D, F, true, KC>), gp, b, shm, st, phi, coef, out, dxv, dyv, gr,    \
                                partial, nlines, n, lw, ostride, istride, estride, ca);                         \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_ccd_part<T, D, F, false>), gp, b, 0, st, phi, coef, out, dxv, dyv, gr,         \
+            hipLaunchKernelGGL((k_ccd_part<T, D, F, false, KC>), gp, b, 0, st, phi, coef, out, dxv, dyv, gr,     \
                                partial, nlines, n, lw, ostride, istride, estride, ca);                         \
+    } while (0)
+#define LSMP_CCDP(D, F)                                                                                        \
+    do {                                                                                                       \
+        if (part9) LSMP_CCDQ(D, F, 9);                                                                         \
+        else LSMP_CCDQ(D, F, KCC);                                                                             \
     } while (0)
     if (part) {
         if (!final_pass) LSMP_CCDP(3, false);
@@ -1929,6 +1942,7 @@ void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T 
     else LSMP_CCD(2, true);
 #undef LSMP_CCD
 #undef LSMP_CCDP
+#undef LSMP_CCDQ
     if (nblocks_out) *nblocks_out = final_pass ? (int)(part ? gp.x : g.x) : 0;
 }
 

@@ -62,6 +62,8 @@ CASES = [
     # odd line lengths: the two halves of a twisted line solve differ by a row
     dict(nx=23, ny=17, nz=19, dim=3),
     dict(nx=45, ny=31, dim=2),
+    # x lines of 300 cells: the partitioned solves with 16 (CRWENO) and 17 (CCD) rows per lane
+    dict(nx=300, ny=36, dim=2),
     # x lines of 640 cells: the CCD and UCCD coefficient tables exceed 64 KB and
     # are read from global memory instead of LDS
     dict(nx=640, ny=40, dim=2),
