@@ -54,21 +54,39 @@ def sphere_box(n):
 
 # Algorithmic bytes per interior cell per launch, in units of V = sizeof(real):
 # the minimum each kernel must read and write (scratch of the line solves and
-# halo re-reads are not counted).
+# halo re-reads are not counted).  The mass correction's kernels work on the
+# cells of the lines that cross the band (BAND_CELLS); their figures count
+# those cells, and the mass update's only its read of phi (the read of
+# |grad phi| and the store happen on the band cells alone).
 BYTES_V = {
     "adv_x": 3, "adv_y": 4,            # 3D y pass accumulates into R
     "redis_stage": 11.0 / 3.0,         # stages read phi,S0[,phi0], write phi: 3, 4, 4
-    "ccd_x": 2, "ccd_y": 2, "ccd_z": 4, "mass_update": 3,
+    "ccd_x": 2, "ccd_y": 2, "ccd_z": 4, "mass_update": 1, "line_hsums": 1, "band_lines": 1,
     "velocity": 3, "sign_init": 2,
 }
 ADV_Z_V = (5 + 6 + 6) / 3.0            # last pass: R, w, phi (+ l1, l2) in, phi (+ l1/l2) out
+BYTES_V_FULL_MASS_UPDATE = 3           # the update over every cell: phi and |grad phi| in, phi out
+
+
+def kernel_cells(name, n, lines):
+    """Interior cells one launch of `name` covers: the listed lines of its
+    direction for the mass correction's CCD passes and update (lines = the
+    last correction's counts along x, y, z), the unlisted z lines for
+    line_hsums, every cell otherwise."""
+    d = {"ccd_x": 0, "ccd_y": 1, "ccd_z": 2, "mass_update": 2}.get(name)
+    if d is not None and lines[d] >= 0:
+        return lines[d] * n
+    if name == "line_hsums":
+        return (n * n - lines[2]) * n
+    return n ** 3
 
 
 # kernel names of the committed counter passes for each timed kernel
 PMC_NAMES = {
     "redis_stage": "k_redis_stage3d", "adv_x": "k_adv_pair_x", "adv_y": "k_adv_twist<double, 1,",
-    "adv_z": "k_adv_twist<double, 2,", "ccd_x": "k_ccd_seg<double, 3, false", "ccd_y": "k_ccd_seg<double, 3, false",
-    "ccd_z": "k_ccd_seg<double, 3, true", "mass_update": "k_mass_update",
+    "adv_z": "k_adv_twist<double, 2,", "ccd_x": "k_ccd_part<double, 3, false", "ccd_y": "k_ccd_part<double, 3, false",
+    "ccd_z": "k_ccd_part<double, 3, true", "mass_update": "k_mass_update_lines<",
+    "mass_update_full": "k_mass_update<",
 }
 FP64_PEAK_TFS = 78.6
 
@@ -230,20 +248,21 @@ def main():
     s.set_profiling(True)
     t, pp, pm = run_steps(s, args.profile_steps, t, imass, lsmp.VEL_LEVEQUE3D)
     kernels = {}
-    for name in ("adv_x", "adv_y", "adv_z", "redis_stage", "sign_init", "ccd_x", "ccd_y", "ccd_z",
-                 "finalize", "mass_update", "velocity"):
+    for name in ("adv_x", "adv_y", "adv_z", "redis_stage", "sign_init", "band_lines", "line_hsums", "ccd_x",
+                 "ccd_y", "ccd_z", "finalize", "mass_update", "velocity"):
         cnt, ms = s.kernel_time(name)
         if cnt:
             kernels[name] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / cnt, 2)}
+    lines = s.band_line_counts()
     s.set_profiling(False)
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     V = 8 if prec == lsmp.FP64 else 4
     bv = ADV_Z_V if dom == "adv_z" else BYTES_V.get(dom, 0.0)
-    bytes_per_launch = bv * V * n ** 3
+    bytes_per_launch = bv * V * kernel_cells(dom, n, lines)
     avg_s = kernels[dom]["avg_us"] * 1e-6
     achieved = bytes_per_launch / avg_s / 1e9
     for k, v in kernels.items():
-        b = (ADV_Z_V if k == "adv_z" else BYTES_V.get(k, 0.0)) * V * n ** 3
+        b = (ADV_Z_V if k == "adv_z" else BYTES_V.get(k, 0.0)) * V * kernel_cells(k, n, lines)
         v["algo_GBs"] = round(b / (v["avg_us"] * 1e-6) / 1e9, 1) if b else None
         tb, fl = pmc_figures(k, n, prec == lsmp.FP64)
         if tb:
@@ -251,12 +270,23 @@ def main():
         if fl:
             v["fp64_TFs"] = round(fl / (v["avg_us"] * 1e-6) / 1e12, 2)
     # on-box streaming ceiling: the runtime's device-to-device copy, or the
-    # library's own streaming kernel (mass update: 2 fields read, 1 written)
-    # at its counter-measured bytes, whichever moves more
+    # library's own streaming kernel (the mass update over every cell: 2
+    # fields read, 1 written; one correction with band lines off) at its
+    # counter-measured bytes, whichever moves more
     copy_gbs = stream_copy_gbs()
-    mu = kernels.get("mass_update", {}).get("traffic_GBs")
+    s.set_band_lines(False)
+    s.set_profiling(True)
+    s.mass_fix(imass)
+    cnt, ms = s.kernel_time("mass_update")
+    s.set_profiling(False)
+    s.set_band_lines(True)
+    tb_full, _ = pmc_figures("mass_update_full", n, prec == lsmp.FP64)
+    mu = round(tb_full / (ms / cnt * 1e-3) / 1e9, 1) if cnt and tb_full else None
+    full_update = {"launches": cnt, "avg_us": round(1e3 * ms / cnt, 2) if cnt else None,
+                   "algo_GBs": round(BYTES_V_FULL_MASS_UPDATE * V * n ** 3 / (ms / cnt * 1e-3) / 1e9, 1)
+                   if cnt else None, "traffic_GBs": mu}
     stream, stream_src = max([(copy_gbs or 0.0, "hipMemcpy device-to-device"),
-                              (mu or 0.0, "k_mass_update, counter-measured bytes")])
+                              (mu or 0.0, "k_mass_update over every cell, counter-measured bytes")])
     stream = stream or None
     _, dom_flop = pmc_figures(dom, n, prec == lsmp.FP64)
 
@@ -303,6 +333,10 @@ def main():
                       "frac": round(dom_flop / avg_s / 1e12 / FP64_PEAK_TFS, 4)} if dom_flop else None),
         },
         "kernels": kernels,
+        # lines the mass correction solved along x, y, z (the lines crossing
+        # the band |phi| < ls_wid; n^2 per direction in all)
+        "band_lines": lines,
+        "mass_update_every_cell": full_update,
     }
     s.close()
     if not args.no_forms:

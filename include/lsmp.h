@@ -118,6 +118,17 @@ int lsmp_mass_vol(lsmp_handle h, double *mass, double *vol);
  * value the exit test used), n_iter the iterations run. */
 int lsmp_mass_fix(lsmp_handle h, double imass, double *mass, int *n_iter);
 
+/* Band lines in the mass correction (no counterpart in the reference, whose
+ * surface_norms solves every line, derivatives.f90:11-50).  on = 1 (the
+ * default): the CCD normals are solved only on the lines that cross the
+ * interface band |phi| < ls_wid, the only cells where the correction
+ * delta |grad phi| and its sum are nonzero; the results are bitwise those of
+ * on = 0, which solves every line. */
+int lsmp_set_band_lines(lsmp_handle h, int on);
+/* Lines listed by the last mass correction along x, y and z (z: 0 in 2D);
+ * all -1 when it solved every line. */
+int lsmp_band_line_counts(lsmp_handle h, int counts[3]);
+
 /* L1 shape error sum |H(phi0/eps) - H(phi/eps)| / (nx ny nz)
  * (output.f90:169-204, 226, 233-248). */
 int lsmp_interface_l1(lsmp_handle h, double *l1);

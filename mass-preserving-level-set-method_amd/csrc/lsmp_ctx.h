@@ -37,6 +37,11 @@ struct lsmp_ctx {
     unsigned long long *rctl = nullptr;   // pseudo-time loop control block (k_redis_ctl)
     double *hbuf = nullptr;   // pinned readback
     int npartial = 0;
+    bool band_lines = true;          // mass correction: CCD normals on band lines only
+    unsigned char *lflag = nullptr;  // band flags of the x, y, z lines (band_flag_bytes)
+    unsigned *lcount = nullptr;      // listed lines per direction
+    int *llist[3] = {nullptr, nullptr, nullptr};
+    int last_lines[3] = {-1, -1, -1};  // listed lines of the last mass correction
     int *band_list = nullptr;        // narrow-band cell indices (learned reinitialisation)
     unsigned *band_count = nullptr;
     void *mk[2] = {nullptr, nullptr};   // the two markers' fields (two-marker MPLS branch)

@@ -415,25 +415,59 @@ template <typename T> int mass_fix(lsmp_ctx *c, double imass, double *mass_out, 
     T *phi = (T *)c->phi;
     int iter, nblk = 0, rc = LSMP_OK;
     double mass = 0.0, prev_rel = 0.0;
+    // The correction changes phi only where delta != 0, so no cell enters the
+    // band during the iterations: the lines crossing it at the start cover
+    // every later iteration.
+    const int *lst[3] = {nullptr, nullptr, nullptr};
+    int nlst[3] = {0, 0, 0};
+    const int dfin = c->dim == 3 ? 2 : 1;   // direction of the final pass (its sums)
+    for (int d = 0; d < 3; ++d) c->last_lines[d] = -1;
+    int nfixed = 0;                         // partials of the unlisted lines, constant over the iterations
+    if (c->band_lines) {
+        KT(c, "band_lines", (launch_band_lines<T>(c->st, c->dim, phi, c->nx, c->ny, c->nz,
+                                                  c->p.ls_wid * (1.0 + 1.0e-6), c->lflag, c->lcount, c->llist[0],
+                                                  c->llist[1], c->llist[2])));
+        unsigned *hc = reinterpret_cast<unsigned *>(c->hbuf);
+        LSMP_HIP(hipMemcpyAsync(hc, c->lcount, (size_t)c->dim * sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+        for (int d = 0; d < c->dim; ++d) {
+            lst[d] = c->llist[d];
+            nlst[d] = (int)hc[d];
+        }
+        for (int d = 0; d < 3; ++d) c->last_lines[d] = nlst[d];
+        const unsigned char *ffin =
+            c->lflag + band_flag_offset(dfin, (size_t)c->ny * c->nz, (size_t)c->nx * c->nz);
+        // the unlisted lines have delta = 0 on every cell, so the correction
+        // leaves them as they are: their mass and volume are summed once
+        KT(c, "line_hsums", (launch_line_hsums<T>(c->st, dfin, phi, ffin, c->nx, c->ny, c->nz, ca, c->partial,
+                                                  &nfixed)));
+    }
+    double *psolve = c->partial + (size_t)3 * nfixed;
     for (iter = 1; iter <= 15; ++iter) {
         KT(c, "ccd_x", (launch_ccd_pass<T>(c->st, 0, c->dim, false, phi, (const T *)c->coef[0], (T *)c->scr,
                                            (T *)c->dxv, nullptr, nullptr, nullptr, nullptr, c->nx, c->ny, c->nz,
-                                           ca, nullptr)));
+                                           ca, nullptr, lst[0], nlst[0])));
         if (c->dim == 3) {
             KT(c, "ccd_y", (launch_ccd_pass<T>(c->st, 1, 3, false, phi, (const T *)c->coef[1], (T *)c->scr,
                                                (T *)c->dyv, nullptr, nullptr, nullptr, nullptr, c->nx, c->ny, c->nz,
-                                               ca, nullptr)));
+                                               ca, nullptr, lst[1], nlst[1])));
             KT(c, "ccd_z", (launch_ccd_pass<T>(c->st, 2, 3, true, phi, (const T *)c->coef[2], (T *)c->scr, nullptr,
-                                               (const T *)c->dxv, (const T *)c->dyv, (T *)c->gr, c->partial, c->nx,
-                                               c->ny, c->nz, ca, &nblk)));
+                                               (const T *)c->dxv, (const T *)c->dyv, (T *)c->gr, psolve, c->nx,
+                                               c->ny, c->nz, ca, &nblk, lst[2], nlst[2])));
         } else {
             KT(c, "ccd_y", (launch_ccd_pass<T>(c->st, 1, 2, true, phi, (const T *)c->coef[1], (T *)c->scr, nullptr,
-                                               (const T *)c->dxv, nullptr, (T *)c->gr, c->partial, c->nx, c->ny,
-                                               c->nz, ca, &nblk)));
+                                               (const T *)c->dxv, nullptr, (T *)c->gr, psolve, c->nx, c->ny,
+                                               c->nz, ca, &nblk, lst[1], nlst[1])));
         }
+        nblk += nfixed;
         if (nblk > c->npartial) return LSMP_E_INVALID;
         KT(c, "finalize", (launch_finalize3(c->st, c->partial, nblk, imass, c->scal)));
-        KT(c, "mass_update", (launch_mass_update<T>(c->st, phi, (const T *)c->gr, c->ncell, (T)c->p.ls_wid, c->scal)));
+        if (c->band_lines)
+            KT(c, "mass_update", (launch_mass_update_lines<T>(c->st, dfin, phi, (const T *)c->gr, (T)c->p.ls_wid,
+                                                              c->scal, lst[dfin], nlst[dfin], c->nx, c->ny, c->nz)));
+        else
+            KT(c, "mass_update",
+               (launch_mass_update<T>(c->st, phi, (const T *)c->gr, c->ncell, (T)c->p.ls_wid, c->scal)));
         LSMP_HIP(hipMemcpyAsync(c->hbuf, c->scal, 5 * sizeof(double), hipMemcpyDeviceToHost, c->st));
         LSMP_HIP(hipStreamSynchronize(c->st));
         mass = c->hbuf[1];
@@ -515,7 +549,7 @@ void free_all(lsmp_ctx *c)
 {
     void *bufs[] = {c->rctl, c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count,
-                    c->mk[0], c->mk[1], c->mflag};
+                    c->mk[0], c->mk[1], c->mflag, c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
@@ -608,18 +642,29 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
             return fail(LSMP_E_HIP);
         }
     }
-    // partial sums: one slot per block of the last CCD pass or reduction kernel
+    // partial sums: one slot per block of the last CCD pass (a solve with up to
+    // 8 lines per block, then the unlisted lines' sums) or reduction kernel
     size_t maxlines = 0;
     for (int d = 1; d < c->dim; ++d) maxlines = std::max(maxlines, c->ncell / (size_t)ns[d]);
-    c->npartial = (int)std::max<size_t>(1024, (maxlines + 255) / 256);
+    c->npartial = (int)std::max<size_t>(1024, (maxlines + 7) / 8 + (maxlines + 255) / 256);
     if (alloc((void **)&c->partial, (size_t)3 * c->npartial * sizeof(double))) return fail(LSMP_E_HIP);
     if (alloc((void **)&c->scal, 8 * sizeof(double))) return fail(LSMP_E_HIP);
     if (alloc((void **)&c->u64, 4 * sizeof(unsigned long long))) return fail(LSMP_E_HIP);
     if (hipHostMalloc((void **)&c->hbuf, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return fail(LSMP_E_HIP);
-    // velocity starts at zero so that an advect before set_velocity is defined
+    // band lines of the mass correction: flags of every x, y, z line, lists of the x and y lines
+    const size_t nfx = (size_t)c->ny * c->nz, nfy = (size_t)c->nx * c->nz, nfz = (size_t)c->nx * c->ny;
+    if (alloc((void **)&c->lflag, band_flag_bytes(nfx, nfy, nfz)) ||
+        alloc((void **)&c->lcount, 4 * sizeof(unsigned)) || alloc((void **)&c->llist[0], nfx * sizeof(int)) ||
+        alloc((void **)&c->llist[1], nfy * sizeof(int)) || alloc((void **)&c->llist[2], nfz * sizeof(int)))
+        return fail(LSMP_E_HIP);
+    // velocity starts at zero so that an advect before set_velocity is defined;
+    // the normals start at zero because a mass correction on band lines leaves
+    // the other lines' values as they were (finite, and multiplied by delta = 0)
     if (hipMemsetAsync(c->u, 0, fb, c->st) != hipSuccess || hipMemsetAsync(c->v, 0, fb, c->st) != hipSuccess ||
-        hipMemsetAsync(c->w, 0, fb, c->st) != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess)
+        hipMemsetAsync(c->w, 0, fb, c->st) != hipSuccess || hipMemsetAsync(c->dxv, 0, fb, c->st) != hipSuccess ||
+        hipMemsetAsync(c->dyv, 0, fb, c->st) != hipSuccess || hipMemsetAsync(c->gr, 0, fb, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
         return fail(LSMP_E_HIP);
     *out = c;
     return LSMP_OK;
@@ -819,6 +864,20 @@ int lsmp_synchronize(lsmp_handle h)
 {
     if (!h) return LSMP_E_INVALID;
     LSMP_HIP(hipStreamSynchronize(h->st));
+    return LSMP_OK;
+}
+
+int lsmp_set_band_lines(lsmp_handle h, int on)
+{
+    if (!h || (on != 0 && on != 1)) return LSMP_E_INVALID;
+    h->band_lines = on == 1;
+    return LSMP_OK;
+}
+
+int lsmp_band_line_counts(lsmp_handle h, int counts[3])
+{
+    if (!h || !counts) return LSMP_E_INVALID;
+    for (int d = 0; d < 3; ++d) counts[d] = h->last_lines[d];
     return LSMP_OK;
 }
 

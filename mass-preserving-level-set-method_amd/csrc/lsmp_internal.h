@@ -8,6 +8,17 @@ namespace lsmp {
 
 constexpr int CSEG = 16;   // rows per CCD checkpoint segment
 
+// listed band lines solved with 32 lanes per line up to these counts (x lines
+// are contiguous; y and z lines are strided, and 32 lanes on one of them read
+// 32 separate cache lines per row)
+#ifndef LSMP_PART_LISTED_X
+#define LSMP_PART_LISTED_X (1 << 30)
+#endif
+#ifndef LSMP_PART_LISTED_STRIDED
+#define LSMP_PART_LISTED_STRIDED (1 << 30)
+#endif
+constexpr int PART_MAX_LISTED_X = LSMP_PART_LISTED_X, PART_MAX_LISTED_STRIDED = LSMP_PART_LISTED_STRIDED;
+
 // reinitialisation tile: RTX x RTY interior cells, one ring for the lambda
 // box, four for the WENO-HJ stencil; RKC planes per workgroup in z
 constexpr int RTX = 32, RTY = 16;
@@ -58,7 +69,30 @@ template <typename T> void launch_scale_by_max(hipStream_t st, T *phi, size_t n,
 template <typename T>
 void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T *phi, const T *coef, T *scr, T *out,
                      const T *dxv, const T *dyv, T *gr, double *partial, int nx, int ny, int nz, const CcdArgs &ca,
-                     int *nblocks_out);
+                     int *nblocks_out, const int *lines = nullptr, int nlist = 0);
+// mass and volume partial sums of the lines of direction dir with lflag 0
+template <typename T>
+void launch_line_hsums(hipStream_t st, int dir, const T *phi, const unsigned char *lflag, int nx, int ny, int nz,
+                       const CcdArgs &ca, double *partial, int *nblocks_out);
+// the mass update on the cells of the nlist listed lines of direction dir
+template <typename T>
+void launch_mass_update_lines(hipStream_t st, int dir, T *phi, const T *gr, T ls_wid, const double *scal,
+                              const int *lines, int nlist, int nx, int ny, int nz);
+
+// Lines that cross the interface band: one flag byte per line of each
+// direction (x: ny nz, y: nx nz, z: nx ny), each direction's flags starting
+// on a 16-byte boundary, and the ascending lists of the flagged lines.
+constexpr size_t pad16(size_t n) { return (n + 15) & ~(size_t)15; }
+__host__ __device__ constexpr size_t band_flag_offset(int d, size_t nfx, size_t nfy)
+{
+    return d == 0 ? 0 : d == 1 ? pad16(nfx) : pad16(nfx) + pad16(nfy);
+}
+constexpr size_t band_flag_bytes(size_t nfx, size_t nfy, size_t nfz) { return pad16(nfx) + pad16(nfy) + pad16(nfz); }
+// flags of the lines with a cell |phi| < wband, their lists (listx, listy and,
+// in 3D, listz) and counts (count[0 .. dim-1])
+template <typename T>
+void launch_band_lines(hipStream_t st, int dim, const T *phi, int nx, int ny, int nz, double wband,
+                       unsigned char *flags, unsigned *count, int *listx, int *listy, int *listz);
 void launch_finalize3(hipStream_t st, const double *partial, int nblk, double imass, double *scal);
 template <typename T>
 void launch_mass_update(hipStream_t st, T *phi, const T *gr, size_t n, T ls_wid, const double *scal);

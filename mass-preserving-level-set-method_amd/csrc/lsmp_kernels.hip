@@ -1126,7 +1126,7 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
                                                   const T *__restrict__ dxv, const T *__restrict__ dyv,
                                                   T *__restrict__ gr, double *__restrict__ partial, int nlines,
                                                   int n, int lw, long ostride, long istride, long estride,
-                                                  CcdArgs ca)
+                                                  CcdArgs ca, const int *__restrict__ lines)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char ccd_lds[];
     const int nb = n + 6;
@@ -1141,7 +1141,8 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
     const int L = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = L < nlines;
     const int Lc = act ? L : nlines - 1;
-    const size_t base = (size_t)(Lc / lw) * (size_t)ostride + (size_t)(Lc % lw) * (size_t)istride;
+    const int line = lines ? lines[Lc] : Lc;   // with a list, lane L solves listed line L of nlines
+    const size_t base = (size_t)(line / lw) * (size_t)ostride + (size_t)(line % lw) * (size_t)istride;
     const size_t NL = (size_t)gridDim.x * blockDim.x;
     const T idx = T(1) / (T)ca.dx, idx2 = T(1) / ((T)ca.dx * (T)ca.dx);
     auto ldq = [&](int q) -> T { return phi[base + (size_t)clampi(q, 0, n - 1) * (size_t)estride]; };
@@ -1291,7 +1292,8 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
                                                    T *__restrict__ out, const T *__restrict__ dxv,
                                                    const T *__restrict__ dyv, T *__restrict__ gr,
                                                    double *__restrict__ partial, int nlines, int n, int lw,
-                                                   long ostride, long istride, long estride, CcdArgs ca)
+                                                   long ostride, long istride, long estride, CcdArgs ca,
+                                                   const int *__restrict__ lines)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char ccd_lds[];
     const int nb = n + 6;
@@ -1307,7 +1309,8 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     const int line = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
     const bool act = line < nlines;
     const int Lc = act ? line : nlines - 1;   // idle halves shadow the last line and emit nothing
-    const size_t base = (size_t)(Lc / lw) * (size_t)ostride + (size_t)(Lc % lw) * (size_t)istride;
+    const int bl = lines ? lines[Lc] : Lc;     // with a list, half `line` solves listed line `line` of nlines
+    const size_t base = (size_t)(bl / lw) * (size_t)ostride + (size_t)(bl % lw) * (size_t)istride;
     const T idx = T(1) / (T)ca.dx, idx2 = T(1) / ((T)ca.dx * (T)ca.dx);
     auto ldq = [&](int q) -> T { return phi[base + (size_t)clampi(q, 0, n - 1) * (size_t)estride]; };
     const int NR = nb - 2, K = (NR + 31) / 32;
@@ -1684,9 +1687,139 @@ __global__ void k_mass_update(T *__restrict__ phi, const T *__restrict__ gr, siz
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const double lam = scal[3];
+    const T p = phi[q];
     double H, D;
-    ls_funs<double>((double)phi[q], (double)ls_wid, H, D);
-    phi[q] = (T)((double)phi[q] + lam * D * (double)gr[q]);
+    ls_funs<double>((double)p, (double)ls_wid, H, D);
+    if (D == 0.0) return;   // phi + lambda 0 |grad phi| is phi: no read of |grad phi|, no store
+    phi[q] = (T)((double)p + lam * D * (double)gr[q]);
+}
+
+// The same update on the cells of the listed lines (every cell where delta can
+// be nonzero lies on one); thread t updates cell t / nlist of listed line
+// t % nlist, so neighbouring lanes take neighbouring lines.
+template <typename T>
+__global__ void k_mass_update_lines(T *__restrict__ phi, const T *__restrict__ gr, T ls_wid,
+                                    const double *__restrict__ scal, const int *__restrict__ lines, int nlist, int n,
+                                    int lw, long ostride, long istride, long estride)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nlist * n) return;
+    const int l = lines[t % (size_t)nlist], q = (int)(t / (size_t)nlist);
+    const size_t a = (size_t)(l / lw) * (size_t)ostride + (size_t)(l % lw) * (size_t)istride +
+                     (size_t)q * (size_t)estride;
+    const double lam = scal[3];
+    const T p = phi[a];
+    double H, D;
+    ls_funs<double>((double)p, (double)ls_wid, H, D);
+    if (D == 0.0) return;
+    phi[a] = (T)((double)p + lam * D * (double)gr[a]);
+}
+
+// Lines that cross the interface band, for the mass correction's CCD passes:
+// flag x line (j, k), y line (i, k) and z line (i, j) of every cell with
+// |phi| < wband.  wband = ls_wid (1 + 1e-6) covers every cell where ls_funs
+// gives delta != 0 (|phi| <= ls_wid (1 - 1e-12)).  The flags must be zero on
+// entry; fz is unused in 2D.
+template <typename T>
+__global__ void k_band_line_flags(const T *__restrict__ phi, int nx, int ny, int nz, double wband,
+                                  unsigned char *__restrict__ fx, unsigned char *__restrict__ fy,
+                                  unsigned char *__restrict__ fz)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)nx * ny * nz) return;
+    if (!(fabs((double)phi[q]) < wband)) return;
+    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+    fx[j + (size_t)ny * k] = 1;
+    fy[i + (size_t)nx * k] = 1;
+    if (nz > 1) fz[i + (size_t)nx * j] = 1;
+}
+
+// Ordered lists of the flagged lines, one workgroup per direction d (blockIdx.x):
+// each thread counts the flags of its run of 16-byte words, an exclusive scan
+// over the workgroup gives its first slot, and it writes its lines in
+// ascending order; count[d] receives the total.  The flags of direction d
+// start at byte band_flag_offset(d) and their padding bytes are zero.
+__global__ __launch_bounds__(1024) void k_compact_lines(const unsigned char *__restrict__ flags, int nfx, int nfy,
+                                                        int nfz, int *__restrict__ lx, int *__restrict__ ly,
+                                                        int *__restrict__ lz, unsigned *__restrict__ count)
+{
+    __shared__ int wsum[16];
+    const int d = blockIdx.x;
+    const int n = d == 0 ? nfx : d == 1 ? nfy : nfz;
+    int *list = d == 0 ? lx : d == 1 ? ly : lz;
+    const uint4 *f = reinterpret_cast<const uint4 *>(flags + band_flag_offset(d, nfx, nfy));
+    const int nw = (n + 15) / 16, cw = (nw + 1023) / 1024;
+    const int w0 = min((int)threadIdx.x * cw, nw), w1 = min(w0 + cw, nw);
+    int cnt = 0;
+    for (int w = w0; w < w1; ++w) {   // flags are 0 or 1: one set bit per flagged byte
+        const uint4 v = f[w];
+        cnt += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int inc = cnt;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+        const int y = __shfl_up(inc, k, 64);
+        if (lane >= k) inc += y;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int wbase = 0;
+    for (int k = 0; k < wid; ++k) wbase += wsum[k];
+    int o = wbase + inc - cnt;
+    for (int w = w0; w < w1; ++w) {
+        const uint4 v = f[w];
+        const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if ((u[e] >> (8 * b)) & 0xffu) list[o++] = w * 16 + e * 4 + b;
+    }
+    if (threadIdx.x == 1023) count[d] = (unsigned)(wbase + inc);
+}
+
+// Mass and volume sums (the ls_mv terms of the final CCD pass) of the lines
+// without a band cell (lflag 0), one lane per line: delta = 0 on them, so
+// they add nothing to the correction's denominator and need no solve.
+template <typename T>
+__global__ __launch_bounds__(256) void k_line_hsums(const T *__restrict__ phi, const unsigned char *__restrict__ lflag,
+                                                    int nlines, int n, int lw, long ostride, long istride,
+                                                    long estride, CcdArgs ca, double *__restrict__ partial)
+{
+    const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (L < nlines && !lflag[L]) {
+        const size_t base = (size_t)(L / lw) * (size_t)ostride + (size_t)(L % lw) * (size_t)istride;
+        const double r12 = ca.rho_12;
+        for (int q0 = 0; q0 < n; q0 += 16) {
+            T v[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = phi[base + (size_t)min(q0 + r, n - 1) * (size_t)estride];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                if (q0 + r < n) {
+                    double h, dd;
+                    ls_funs<double>((double)v[r], ca.ls_wid, h, dd);
+                    acc[0] += h * ca.dv;
+                    const double rho = ca.inverse ? r12 * h + (1.0 - h) : h + r12 * (1.0 - h);
+                    acc[1] += rho * h * ca.dv;
+                }
+            }
+        }
+    }
+    block_sum<3>(acc);
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x + 0] = acc[0];
+        partial[3 * blockIdx.x + 1] = acc[1];
+        partial[3 * blockIdx.x + 2] = acc[2];
+    }
+}
+
+__global__ void k_zero_bytes(unsigned char *__restrict__ p, size_t n)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
 }
 
 // LS mass and volume (ls_mv, ns_solver/others.f90:185-212) as block partials.
@@ -1891,14 +2024,17 @@ void launch_scale_by_max(hipStream_t st, T *phi, size_t n, const unsigned long l
     hipLaunchKernelGGL((k_scale_by_max<T>), dim3(nblk(n, 256)), dim3(256), 0, st, phi, n, gmax);
 }
 
+// lines == nullptr: every line of direction dir; else the nlist listed lines
+// (a final pass over a list then sums the listed lines only).
 template <typename T>
 void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T *phi, const T *coef, T *scr, T *out,
                      const T *dxv, const T *dyv, T *gr, double *partial, int nx, int ny, int nz, const CcdArgs &ca,
-                     int *nblocks_out)
+                     int *nblocks_out, const int *lines, int nlist)
 {
     const int n = dir == 0 ? nx : dir == 1 ? ny : nz;
-    const int nlines = dir == 0 ? ny * nz : dir == 1 ? nx * nz : nx * ny;
-    const int lw = dir == 0 ? nlines : nx;
+    const int nall = dir == 0 ? ny * nz : dir == 1 ? nx * nz : nx * ny;
+    const int nlines = lines ? nlist : nall;
+    const int lw = dir == 0 ? nall : nx;
     const long ostride = dir == 0 ? 0 : dir == 1 ? (long)nx * ny : (long)nx;
     const long istride = dir == 0 ? (long)nx : 1;
     const long estride = dir == 0 ? 1 : dir == 1 ? (long)nx : (long)nx * ny;
@@ -1910,40 +2046,77 @@ void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T 
     do {                                                                                                       \
         if (lds)                                                                                               \
             hipLaunchKernelGGL((k_ccd_seg<T, D, F, true>), g, b, shm, st, phi, coef, scr, out, dxv, dyv, gr,     \
-                               partial, nlines, n, lw, ostride, istride, estride, ca);                         \
+                               partial, nlines, n, lw, ostride, istride, estride, ca, lines);                  \
         else                                                                                                   \
             hipLaunchKernelGGL((k_ccd_seg<T, D, F, false>), g, b, 0, st, phi, coef, scr, out, dxv, dyv, gr,      \
-                               partial, nlines, n, lw, ostride, istride, estride, ca);                         \
+                               partial, nlines, n, lw, ostride, istride, estride, ca, lines);                  \
     } while (0)
-    // few lines (the 2D grids): 32 lanes per line instead of one
-    const bool part = nlines <= PART_MAX_LINES_STRIDED && n + 4 <= 32 * KCC;
+    // few lines (the 2D grids, or the lines crossing the band): 32 lanes per
+    // line instead of one
+    const int part_max = !lines ? PART_MAX_LINES_STRIDED : dir == 0 ? PART_MAX_LISTED_X : PART_MAX_LISTED_STRIDED;
+    const bool part = nlines <= part_max && n + 4 <= 32 * KCC;
     const bool part9 = n + 4 <= 32 * 9;
     const dim3 gp(nblk(nlines, 8));   // two lines per wave, four waves per block
 #define LSMP_CCDQ(D, F, KC)                                                                                    \
     do {                                                                                                       \
         if (lds)                                                                                               \
             hipLaunchKernelGGL((k_ccd_part<T, D, F, true, KC>), gp, b, shm, st, phi, coef, out, dxv, dyv, gr,    \
-                               partial, nlines, n, lw, ostride, istride, estride, ca);                         \
+                               partial, nlines, n, lw, ostride, istride, estride, ca, lines);                  \
         else                                                                                                   \
             hipLaunchKernelGGL((k_ccd_part<T, D, F, false, KC>), gp, b, 0, st, phi, coef, out, dxv, dyv, gr,     \
-                               partial, nlines, n, lw, ostride, istride, estride, ca);                         \
+                               partial, nlines, n, lw, ostride, istride, estride, ca, lines);                  \
     } while (0)
 #define LSMP_CCDP(D, F)                                                                                        \
     do {                                                                                                       \
         if (part9) LSMP_CCDQ(D, F, 9);                                                                         \
         else LSMP_CCDQ(D, F, KCC);                                                                             \
     } while (0)
-    if (part) {
-        if (!final_pass) LSMP_CCDP(3, false);
-        else if (dim == 3) LSMP_CCDP(3, true);
-        else LSMP_CCDP(2, true);
-    } else if (!final_pass) LSMP_CCD(3, false);
-    else if (dim == 3) LSMP_CCD(3, true);
-    else LSMP_CCD(2, true);
+    int nbl = 0;
+    if (nlines > 0) {
+        if (part) {
+            if (!final_pass) LSMP_CCDP(3, false);
+            else if (dim == 3) LSMP_CCDP(3, true);
+            else LSMP_CCDP(2, true);
+        } else if (!final_pass) LSMP_CCD(3, false);
+        else if (dim == 3) LSMP_CCD(3, true);
+        else LSMP_CCD(2, true);
+        nbl = (int)(part ? gp.x : g.x);
+    }
 #undef LSMP_CCD
 #undef LSMP_CCDP
 #undef LSMP_CCDQ
-    if (nblocks_out) *nblocks_out = final_pass ? (int)(part ? gp.x : g.x) : 0;
+    if (nblocks_out) *nblocks_out = final_pass ? nbl : 0;
+}
+
+template <typename T>
+void launch_line_hsums(hipStream_t st, int dir, const T *phi, const unsigned char *lflag, int nx, int ny, int nz,
+                       const CcdArgs &ca, double *partial, int *nblocks_out)
+{
+    const int n = dir == 0 ? nx : dir == 1 ? ny : nz;
+    const int nall = dir == 0 ? ny * nz : dir == 1 ? nx * nz : nx * ny;
+    const int lw = dir == 0 ? nall : nx;
+    const long ostride = dir == 0 ? 0 : dir == 1 ? (long)nx * ny : (long)nx;
+    const long istride = dir == 0 ? (long)nx : 1;
+    const long estride = dir == 0 ? 1 : dir == 1 ? (long)nx : (long)nx * ny;
+    const dim3 g(nblk(nall, 256));
+    hipLaunchKernelGGL((k_line_hsums<T>), g, dim3(256), 0, st, phi, lflag, nall, n, lw, ostride, istride, estride,
+                       ca, partial);
+    *nblocks_out = (int)g.x;
+}
+
+template <typename T>
+void launch_mass_update_lines(hipStream_t st, int dir, T *phi, const T *gr, T ls_wid, const double *scal,
+                              const int *lines, int nlist, int nx, int ny, int nz)
+{
+    if (nlist <= 0) return;
+    const int n = dir == 0 ? nx : dir == 1 ? ny : nz;
+    const int lw = dir == 0 ? ny * nz : nx;
+    const long ostride = dir == 0 ? 0 : dir == 1 ? (long)nx * ny : (long)nx;
+    const long istride = dir == 0 ? (long)nx : 1;
+    const long estride = dir == 0 ? 1 : dir == 1 ? (long)nx : (long)nx * ny;
+    const size_t total = (size_t)nlist * n;
+    hipLaunchKernelGGL((k_mass_update_lines<T>), dim3(nblk(total, 256)), dim3(256), 0, st, phi, gr, ls_wid, scal,
+                       lines, nlist, n, lw, ostride, istride, estride);
 }
 
 template <typename T>
@@ -1979,6 +2152,20 @@ void launch_uccd_pass(hipStream_t st, int dir, int mode, int stage, const T *phi
 void launch_finalize3(hipStream_t st, const double *partial, int nblk_, double imass, double *scal)
 {
     hipLaunchKernelGGL(k_finalize3, dim3(1), dim3(1024), 0, st, partial, nblk_, imass, scal);
+}
+
+template <typename T>
+void launch_band_lines(hipStream_t st, int dim, const T *phi, int nx, int ny, int nz, double wband,
+                       unsigned char *flags, unsigned *count, int *listx, int *listy, int *listz)
+{
+    const int nfx = ny * nz, nfy = nx * nz, nfz = dim == 3 ? nx * ny : 0;
+    const size_t nbytes = band_flag_bytes(nfx, nfy, nfz);
+    hipLaunchKernelGGL(k_zero_bytes, dim3(nblk(nbytes, 256)), dim3(256), 0, st, flags, nbytes);
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL((k_band_line_flags<T>), dim3(nblk(n, 256)), dim3(256), 0, st, phi, nx, ny, nz, wband,
+                       flags, flags + band_flag_offset(1, nfx, nfy), flags + band_flag_offset(2, nfx, nfy));
+    hipLaunchKernelGGL(k_compact_lines, dim3(dim), dim3(1024), 0, st, flags, nfx, nfy, nfz, listx, listy, listz,
+                       count);
 }
 
 template <typename T>
@@ -2065,7 +2252,14 @@ void launch_zero_u64(hipStream_t st, unsigned long long *p, int n)
                                      unsigned long long *);                                                    \
     template void launch_scale_by_max<T>(hipStream_t, T *, size_t, const unsigned long long *);                \
     template void launch_ccd_pass<T>(hipStream_t, int, int, bool, const T *, const T *, T *, T *, const T *,     \
-                                     const T *, T *, double *, int, int, int, const CcdArgs &, int *);           \
+                                     const T *, T *, double *, int, int, int, const CcdArgs &, int *,          \
+                                     const int *, int);                                                        \
+    template void launch_line_hsums<T>(hipStream_t, int, const T *, const unsigned char *, int, int, int,          \
+                                       const CcdArgs &, double *, int *);                                      \
+    template void launch_mass_update_lines<T>(hipStream_t, int, T *, const T *, T, const double *, const int *,   \
+                                              int, int, int, int);                                             \
+    template void launch_band_lines<T>(hipStream_t, int, const T *, int, int, int, double, unsigned char *,      \
+                                       unsigned *, int *, int *, int *);                                       \
     template void launch_mass_update<T>(hipStream_t, T *, const T *, size_t, T, const double *);               \
     template void launch_mass_vol<T>(hipStream_t, const T *, size_t, const CcdArgs &, double *, int);           \
     template void launch_l1<T>(hipStream_t, const T *, const T *, size_t, double, double *, int);              \

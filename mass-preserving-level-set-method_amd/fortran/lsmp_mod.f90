@@ -117,6 +117,12 @@ interface
         type(c_ptr), value :: h
         integer(c_int), value :: scheme
     end function
+    ! mass correction on the lines crossing the band only (1, default) or on every line (0)
+    integer(c_int) function c_lsmp_set_band_lines(h, on) bind(C, name="lsmp_set_band_lines")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int), value :: on
+    end function
     ! two-marker MPLS branch (interface_solver.f90:34-121)
     integer(c_int) function c_lsmp_set_marker(h, mid, box) bind(C, name="lsmp_set_marker")
         import :: c_int, c_ptr, c_double
@@ -154,7 +160,7 @@ end interface
 public :: c_lsmp_init, c_lsmp_finalize, c_lsmp_upload_phi, c_lsmp_download_phi, c_lsmp_store_phi0
 public :: c_lsmp_set_velocity, c_lsmp_set_velocity_analytic, c_lsmp_rk3_advect, c_lsmp_reinit
 public :: c_lsmp_mass_vol, c_lsmp_mass_fix, c_lsmp_interface_l1, c_lsmp_get_stats, c_lsmp_synchronize
-public :: c_lsmp_set_advection_scheme
+public :: c_lsmp_set_advection_scheme, c_lsmp_set_band_lines
 public :: c_lsmp_set_marker, c_lsmp_get_marker, c_lsmp_marker_load, c_lsmp_marker_store, c_lsmp_combine_markers
 public :: lsmp_check, lsmp_error_message
 

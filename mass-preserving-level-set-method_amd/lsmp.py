@@ -32,7 +32,7 @@ EXPORTS = [
     "lsmp_store_phi0", "lsmp_set_velocity", "lsmp_set_velocity_analytic", "lsmp_rk3_advect",
     "lsmp_reinit", "lsmp_mass_vol", "lsmp_mass_fix", "lsmp_interface_l1", "lsmp_get_stats",
     "lsmp_set_profiling", "lsmp_kernel_time", "lsmp_synchronize", "lsmp_set_advection_scheme",
-    "lsmp_component_masses",
+    "lsmp_set_band_lines", "lsmp_band_line_counts", "lsmp_component_masses",
     "lsmp_set_marker", "lsmp_get_marker", "lsmp_marker_load", "lsmp_marker_store", "lsmp_combine_markers",
     "lsmp_net_create", "lsmp_net_destroy", "lsmp_net_param_count", "lsmp_net_set_params",
     "lsmp_net_get_params", "lsmp_net_dataset_size", "lsmp_net_collect", "lsmp_net_train",
@@ -100,6 +100,8 @@ def load_library(path=LIB_PATH):
     L.lsmp_kernel_time.argtypes = [H, C.c_char_p, C.POINTER(C.c_longlong), _DP]
     L.lsmp_synchronize.argtypes = [H]
     L.lsmp_set_advection_scheme.argtypes = [H, C.c_int]
+    L.lsmp_set_band_lines.argtypes = [H, C.c_int]
+    L.lsmp_band_line_counts.argtypes = [H, C.POINTER(C.c_int)]
     L.lsmp_component_masses.argtypes = [H, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_int), C.c_int, _DP, C.POINTER(C.c_int)]
     _FP = C.POINTER(C.c_float)
@@ -208,6 +210,17 @@ class Solver:
     def set_advection_scheme(self, scheme):
         """SCHEME_CRWENO (level_set_rk3_source_weno) or SCHEME_UCCD (source_ccd)."""
         _check(self.lib.lsmp_set_advection_scheme(self.h, int(scheme)))
+
+    def set_band_lines(self, on):
+        """Mass correction: CCD normals only on the lines crossing the band
+        (True, the default) or on every line (False); same bits either way."""
+        _check(self.lib.lsmp_set_band_lines(self.h, 1 if on else 0))
+
+    def band_line_counts(self):
+        """Lines listed by the last mass correction along x, y, z (-1: every line)."""
+        c = (C.c_int * 3)()
+        _check(self.lib.lsmp_band_line_counts(self.h, c))
+        return list(c)
 
     def component_masses(self, phi_min, query_cells=(), maxcomp=4096, phi_core=None):
         """Connected components of phi > phi_core (default phi_min), grown

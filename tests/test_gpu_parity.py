@@ -74,6 +74,11 @@ def ids(c):
     return "x".join(str(c.get(k)) for k in ("nx", "ny", "nz") if c.get(k)) + f"_{c['dim']}d"
 
 
+# more than 4096 lines per direction: the one-lane-per-line CCD kernel, with
+# the mass correction's line lists and flags, in 3D
+CASES_MANY_LINES = [dict(nx=72, ny=66, nz=68, dim=3)]
+
+
 @pytest.mark.parametrize("case", CASES, ids=ids)
 def test_upload_download_roundtrip(case):
     o, s = pair(**case)
@@ -193,7 +198,7 @@ def test_reinit_btn0_parity(case):
     s.close()
 
 
-@pytest.mark.parametrize("case", CASES, ids=ids)
+@pytest.mark.parametrize("case", CASES + CASES_MANY_LINES, ids=ids)
 def test_mass_fix_parity(case):
     o, s = pair(**case)
     say("mass fix", case)
@@ -217,7 +222,7 @@ def test_mass_fix_parity(case):
     s.close()
 
 
-@pytest.mark.parametrize("case", CASES, ids=ids)
+@pytest.mark.parametrize("case", CASES + CASES_MANY_LINES, ids=ids)
 def test_mpls_steps_parity(case):
     """Full merged-MPLS steps (interface_solver.f90:123-129) with reinit every
     step, velocity at t^{n+1} (main.f90:23-29)."""
