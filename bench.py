@@ -65,7 +65,6 @@ BYTES_V = {
     "velocity": 3, "sign_init": 2,
 }
 ADV_Z_V = (5 + 6 + 6) / 3.0            # last pass: R, w, phi (+ l1, l2) in, phi (+ l1/l2) out
-BYTES_V_FULL_MASS_UPDATE = 3           # the update over every cell: phi and |grad phi| in, phi out
 
 
 def kernel_cells(name, n, lines):
@@ -85,8 +84,8 @@ def kernel_cells(name, n, lines):
 PMC_NAMES = {
     "redis_stage": "k_redis_stage3d", "adv_x": "k_adv_pair_x", "adv_y": "k_adv_twist<double, 1,",
     "adv_z": "k_adv_twist<double, 2,", "ccd_x": "k_ccd_part<double, 3, false", "ccd_y": "k_ccd_part<double, 3, false",
-    "ccd_z": "k_ccd_part<double, 3, true", "mass_update": "k_mass_update_lines<",
-    "mass_update_full": "k_mass_update<",
+    "ccd_z": "k_ccd_part<double, 3, true", "mass_update": "k_mass_update_lines<", "line_hsums": "k_line_hsums<",
+    "sign_init": "k_sign_init<", "velocity": "k_velocity<",
 }
 FP64_PEAK_TFS = 78.6
 
@@ -104,7 +103,7 @@ def pmc_figures(kernel, n, fp64):
     if not hits:
         return None, None
     traffic = sum(v["traffic_bytes"] for v in hits) / len(hits)
-    flop = sum(v.get("fp64_flop_per_cell", 0.0) for v in hits) / len(hits) * n ** 3
+    flop = sum(v.get("fp64_flop_per_launch", 0.0) for v in hits) / len(hits)
     return round(traffic), flop
 
 
@@ -270,23 +269,12 @@ def main():
         if fl:
             v["fp64_TFs"] = round(fl / (v["avg_us"] * 1e-6) / 1e12, 2)
     # on-box streaming ceiling: the runtime's device-to-device copy, or the
-    # library's own streaming kernel (the mass update over every cell: 2
-    # fields read, 1 written; one correction with band lines off) at its
-    # counter-measured bytes, whichever moves more
+    # library's streaming kernels over every cell (sign init: phi in, S0 out;
+    # velocity: u, v, w out) at their counter-measured bytes, whichever moves most
     copy_gbs = stream_copy_gbs()
-    s.set_band_lines(False)
-    s.set_profiling(True)
-    s.mass_fix(imass)
-    cnt, ms = s.kernel_time("mass_update")
-    s.set_profiling(False)
-    s.set_band_lines(True)
-    tb_full, _ = pmc_figures("mass_update_full", n, prec == lsmp.FP64)
-    mu = round(tb_full / (ms / cnt * 1e-3) / 1e9, 1) if cnt and tb_full else None
-    full_update = {"launches": cnt, "avg_us": round(1e3 * ms / cnt, 2) if cnt else None,
-                   "algo_GBs": round(BYTES_V_FULL_MASS_UPDATE * V * n ** 3 / (ms / cnt * 1e-3) / 1e9, 1)
-                   if cnt else None, "traffic_GBs": mu}
-    stream, stream_src = max([(copy_gbs or 0.0, "hipMemcpy device-to-device"),
-                              (mu or 0.0, "k_mass_update over every cell, counter-measured bytes")])
+    stream, stream_src = max([(copy_gbs or 0.0, "hipMemcpy device-to-device")] +
+                             [(kernels.get(k, {}).get("traffic_GBs") or 0.0, f"k_{k}, counter-measured bytes")
+                              for k in ("sign_init", "velocity")])
     stream = stream or None
     _, dom_flop = pmc_figures(dom, n, prec == lsmp.FP64)
 
@@ -336,7 +324,6 @@ def main():
         # lines the mass correction solved along x, y, z (the lines crossing
         # the band |phi| < ls_wid; n^2 per direction in all)
         "band_lines": lines,
-        "mass_update_every_cell": full_update,
     }
     s.close()
     if not args.no_forms:
