@@ -129,6 +129,17 @@ int lsmp_set_band_lines(lsmp_handle h, int on);
  * all -1 when it solved every line. */
 int lsmp_band_line_counts(lsmp_handle h, int counts[3]);
 
+/* Narrow band for the chosen forms (no counterpart in the reference, which
+ * transports and reinitialises every cell).  width_cells = W / dx > 0 (3D
+ * only, at least 4): lsmp_rk3_advect (CRWENO) first clamps phi to [-W, W],
+ * then solves only the lines that cross the tube |phi| < W and updates only
+ * the cells whose three lines do; lsmp_reinit updates only the workgroups of
+ * 32 x 8 x 32 cells that hold a cell |phi| < W; every other cell keeps its
+ * value.  Results differ from the full-domain method away from the interface
+ * (the far field is the constant +-W), so the FP64 drop-in leaves it off
+ * (width_cells = 0, the default). */
+int lsmp_set_narrow_band(lsmp_handle h, double width_cells);
+
 /* L1 shape error sum |H(phi0/eps) - H(phi/eps)| / (nx ny nz)
  * (output.f90:169-204, 226, 233-248). */
 int lsmp_interface_l1(lsmp_handle h, double *l1);

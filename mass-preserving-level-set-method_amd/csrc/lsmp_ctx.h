@@ -42,6 +42,8 @@ struct lsmp_ctx {
     unsigned *lcount = nullptr;      // listed lines per direction
     int *llist[3] = {nullptr, nullptr, nullptr};
     int last_lines[3] = {-1, -1, -1};  // listed lines of the last mass correction
+    double nb_cells = 0.0;             // narrow band half-width in cells (0: off)
+    unsigned char *tiles = nullptr;    // narrow band: 3D reinitialisation workgroups holding a tube cell
     int *band_list = nullptr;        // narrow-band cell indices (learned reinitialisation)
     unsigned *band_count = nullptr;
     void *mk[2] = {nullptr, nullptr};   // the two markers' fields (two-marker MPLS branch)

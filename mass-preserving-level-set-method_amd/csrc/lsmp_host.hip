@@ -269,11 +269,37 @@ int alloc(void **p, size_t bytes)
 }
 
 // ------------------------------------------------------------------ phases
+// narrow band half-width W (the largest cell size times the cells asked for)
+double tube_width(const lsmp_ctx *c)
+{
+    return c->nb_cells * std::max(c->p.dx, std::max(c->p.dy, c->p.dz));
+}
+
 template <typename T> int advect(lsmp_ctx *c, double dt)
 {
     T *R = (T *)c->R, *L1 = (T *)c->L1, *L2 = (T *)c->L2, *scr = (T *)c->scr;
     const T *u = (const T *)c->u, *v = (const T *)c->v, *w = (const T *)c->w;
     const T dx = (T)c->p.dx, dy = (T)c->p.dy, dz = (T)c->p.dz, tdt = (T)dt;
+    // Narrow band (3D, CRWENO): phi is clamped to [-W, W], only the lines
+    // crossing the tube |phi| < W are solved, and the RK stage updates only
+    // cells whose x, y and z lines all cross it (every cell with |phi| < W);
+    // the other cells keep their (clamped) values.
+    const bool nb = c->nb_cells > 0.0 && c->dim == 3 && c->scheme == LSMP_SCHEME_CRWENO;
+    int nl[3] = {0, 0, 0};
+    TubeMask tm;
+    if (nb) {
+        KT(c, "tube_lines", (launch_band_lines<T>(c->st, 3, (T *)c->phi, c->nx, c->ny, c->nz, tube_width(c),
+                                                  c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2], true)));
+        unsigned *hc = reinterpret_cast<unsigned *>(c->hbuf);
+        LSMP_HIP(hipMemcpyAsync(hc, c->lcount, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+        for (int d = 0; d < 3; ++d) nl[d] = (int)hc[d];
+        const size_t nfx = (size_t)c->ny * c->nz, nfy = (size_t)c->nx * c->nz;
+        tm.fx = c->lflag;
+        tm.fy = c->lflag + band_flag_offset(1, nfx, nfy);
+        tm.nx = c->nx;
+        tm.ny = c->ny;
+    }
     for (int stage = 1; stage <= 3; ++stage) {
         T *phi = (T *)c->phi;
         if (c->scheme == LSMP_SCHEME_UCCD) {
@@ -293,6 +319,15 @@ template <typename T> int advect(lsmp_ctx *c, double dt)
                                                     c->ny, c->nz, dx, tdt)));
             }
             std::swap(c->phi, c->phiB);
+            continue;
+        }
+        if (nb) {
+            KT(c, "adv_x", (launch_adv_pass<T>(c->st, 0, 0, stage, phi, u, R, L1, L2, scr, c->nx, c->ny, c->nz, dx,
+                                               tdt, c->llist[0], nl[0])));
+            KT(c, "adv_y", (launch_adv_pass<T>(c->st, 1, 1, stage, phi, v, R, L1, L2, scr, c->nx, c->ny, c->nz, dy,
+                                               tdt, c->llist[1], nl[1])));
+            KT(c, "adv_z", (launch_adv_pass<T>(c->st, 2, 2, stage, phi, w, R, L1, L2, scr, c->nx, c->ny, c->nz, dz,
+                                               tdt, c->llist[2], nl[2], tm)));
             continue;
         }
         KT(c, "adv_x", (launch_adv_pass<T>(c->st, 0, 0, stage, phi, u, R, L1, L2, scr, c->nx, c->ny, c->nz, dx, tdt)));
@@ -362,6 +397,13 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
     // are queued CHUNK at a time and the host reads the control block per chunk
     constexpr int CHUNK = 16;
     unsigned long long *ctl = c->rctl;
+    // narrow band (3D): workgroups with no cell |phi| < W keep their cells
+    const unsigned char *tiles = nullptr;
+    if (c->nb_cells > 0.0 && c->dim == 3) {
+        KT(c, "tube_tiles", (launch_tube_tiles<T>(c->st, (const T *)c->phi, c->nx, c->ny, c->nz, tube_width(c),
+                                                  c->tiles)));
+        tiles = c->tiles;
+    }
     launch_redis_ctl(c->st, ctl, 0);
     int launched = 0;
     unsigned long long hc[4] = {0, 0, 0, 0};
@@ -371,9 +413,9 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
             T *cur = (T *)c->phi, *b = (T *)c->phiB, *cc = (T *)c->phiC;
             launch_redis_ctl(c->st, ctl, 1);
             LSMP_LAUNCHED();
-            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 1, btn, cur, b, cur, S0, ra, ctl)));
-            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 2, btn, b, cc, cur, S0, ra, ctl)));
-            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 3, btn, cc, b, cur, S0, ra, ctl)));
+            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 1, btn, cur, b, cur, S0, ra, ctl, tiles)));
+            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 2, btn, b, cc, cur, S0, ra, ctl, tiles)));
+            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 3, btn, cc, b, cur, S0, ra, ctl, tiles)));
             launch_redis_ctl(c->st, ctl, 2);
             // the new phi is in b; the old one becomes a spare
             c->phiB = c->phi;
@@ -426,7 +468,7 @@ template <typename T> int mass_fix(lsmp_ctx *c, double imass, double *mass_out, 
     if (c->band_lines) {
         KT(c, "band_lines", (launch_band_lines<T>(c->st, c->dim, phi, c->nx, c->ny, c->nz,
                                                   c->p.ls_wid * (1.0 + 1.0e-6), c->lflag, c->lcount, c->llist[0],
-                                                  c->llist[1], c->llist[2])));
+                                                  c->llist[1], c->llist[2], false)));
         unsigned *hc = reinterpret_cast<unsigned *>(c->hbuf);
         LSMP_HIP(hipMemcpyAsync(hc, c->lcount, (size_t)c->dim * sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
         LSMP_HIP(hipStreamSynchronize(c->st));
@@ -549,7 +591,7 @@ void free_all(lsmp_ctx *c)
 {
     void *bufs[] = {c->rctl, c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count,
-                    c->mk[0], c->mk[1], c->mflag, c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2]};
+                    c->mk[0], c->mk[1], c->mflag, c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2], c->tiles};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
@@ -657,6 +699,9 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
     if (alloc((void **)&c->lflag, band_flag_bytes(nfx, nfy, nfz)) ||
         alloc((void **)&c->lcount, 4 * sizeof(unsigned)) || alloc((void **)&c->llist[0], nfx * sizeof(int)) ||
         alloc((void **)&c->llist[1], nfy * sizeof(int)) || alloc((void **)&c->llist[2], nfz * sizeof(int)))
+        return fail(LSMP_E_HIP);
+    if (c->dim == 3 &&
+        alloc((void **)&c->tiles, (size_t)((c->nx + R3X - 1) / R3X) * ((c->ny + R3Y - 1) / R3Y) * ((c->nz + RKC - 1) / RKC)))
         return fail(LSMP_E_HIP);
     // velocity starts at zero so that an advect before set_velocity is defined;
     // the normals start at zero because a mass correction on band lines leaves
@@ -871,6 +916,14 @@ int lsmp_set_band_lines(lsmp_handle h, int on)
 {
     if (!h || (on != 0 && on != 1)) return LSMP_E_INVALID;
     h->band_lines = on == 1;
+    return LSMP_OK;
+}
+
+int lsmp_set_narrow_band(lsmp_handle h, double width_cells)
+{
+    if (!h || !(width_cells >= 0.0) || (width_cells > 0.0 && (h->dim != 3 || width_cells < 4.0)))
+        return LSMP_E_INVALID;
+    h->nb_cells = width_cells;
     return LSMP_OK;
 }
 

@@ -53,14 +53,40 @@ struct VelArgs {
     double xstart, ystart, zstart, dx, dy, dz, ct, omega;
 };
 
+// Narrow band: the RK stage of the last direction updates a cell only when its
+// x line and, in 3D, its y line cross the tube (fx, fy: the band line flags of
+// those directions); fx == nullptr updates every cell.
+struct TubeMask {
+    const unsigned char *fx = nullptr, *fy = nullptr;
+    int nx = 0, ny = 0;
+    __device__ bool in(size_t a) const
+    {
+        if (!fx) return true;
+        const size_t i = a % (size_t)nx, r = a / (size_t)nx, j = r % (size_t)ny, k = r / (size_t)ny;
+        return fx[j + (size_t)ny * k] && (!fy || fy[i + (size_t)nx * k]);
+    }
+};
+
+// listed strided lines solved one wave per line up to this count
+#ifndef LSMP_PART_LISTED_ADV
+#define LSMP_PART_LISTED_ADV 16384
+#endif
+constexpr int PART_MAX_LISTED_ADV = LSMP_PART_LISTED_ADV;
+
+// lines == nullptr: every line; else the nlist listed lines (the narrow band)
 template <typename T>
 void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const T *vel, T *R, T *L1, T *L2,
-                     T *scr, int nx, int ny, int nz, T dd, T dt);
+                     T *scr, int nx, int ny, int nz, T dd, T dt, const int *lines = nullptr, int nlist = 0,
+                     const TubeMask &tm = TubeMask());
+// narrow band: flags of the 3D reinitialisation workgroups with a cell |phi| < wband
+template <typename T>
+void launch_tube_tiles(hipStream_t st, const T *phi, int nx, int ny, int nz, double wband, unsigned char *tiles);
 template <typename T>
 void launch_combine_markers(hipStream_t st, const T *m1, const T *m2, T *phi, size_t n, unsigned *flag);
 template <typename T>
 void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *phi, T *phi_out, const T *phi0,
-                        const T *S0, const RedisArgs &ra, unsigned long long *ctl);
+                        const T *S0, const RedisArgs &ra, unsigned long long *ctl,
+                        const unsigned char *tiles = nullptr);
 template <typename T> void launch_sign_init(hipStream_t st, const T *phi, T *S0, size_t n, T ls_wid, int raw);
 template <typename T>
 void launch_grad_max(hipStream_t st, int dim, const T *phi, const T *S0, const RedisArgs &ra,
@@ -89,10 +115,12 @@ __host__ __device__ constexpr size_t band_flag_offset(int d, size_t nfx, size_t 
 }
 constexpr size_t band_flag_bytes(size_t nfx, size_t nfy, size_t nfz) { return pad16(nfx) + pad16(nfy) + pad16(nfz); }
 // flags of the lines with a cell |phi| < wband, their lists (listx, listy and,
-// in 3D, listz) and counts (count[0 .. dim-1])
+// in 3D, listz) and counts (count[0 .. dim-1]); clamp (the narrow band's tube,
+// 3D): phi is first clamped to [-wband, wband] and the lines of the band cells'
+// six neighbours are flagged as well
 template <typename T>
-void launch_band_lines(hipStream_t st, int dim, const T *phi, int nx, int ny, int nz, double wband,
-                       unsigned char *flags, unsigned *count, int *listx, int *listy, int *listz);
+void launch_band_lines(hipStream_t st, int dim, T *phi, int nx, int ny, int nz, double wband,
+                       unsigned char *flags, unsigned *count, int *listx, int *listy, int *listz, bool clamp);
 void launch_finalize3(hipStream_t st, const double *partial, int nblk, double imass, double *scal);
 template <typename T>
 void launch_mass_update(hipStream_t st, T *phi, const T *gr, size_t n, T ls_wid, const double *scal);

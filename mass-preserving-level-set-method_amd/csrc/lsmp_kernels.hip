@@ -81,10 +81,11 @@ constexpr int AXB = 8;
 template <typename T> struct TwistDepth { static constexpr int C = 8, B = 4; };
 template <> struct TwistDepth<double> { static constexpr int C = 4, B = 4; };
 
-template <typename T, int MODE, int STAGE>
+template <typename T, int MODE, int STAGE, bool TUBE>
 __global__ __launch_bounds__(256) void k_adv_twist(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
                                                     T *__restrict__ scr, int nlines, int n, int lw, long ostride,
-                                                    long istride, long estride, T dd, T dt)
+                                                    long istride, long estride, T dd, T dt,
+                                                    const int *__restrict__ lines, TubeMask tm)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -95,7 +96,8 @@ __global__ __launch_bounds__(256) void k_adv_twist(T *phi, const T *__restrict__
     const int L = wave * 16 + (lane & 15);
     const bool act = L < nlines;
     const int Lc = act ? L : nlines - 1;   // idle lanes shadow the last line and store nothing
-    const size_t base = (size_t)(Lc / lw) * (size_t)ostride + (size_t)(Lc % lw) * (size_t)istride;
+    const int bl = lines ? lines[Lc] : Lc;  // with a list, lane group L solves listed line L
+    const size_t base = (size_t)(bl / lw) * (size_t)ostride + (size_t)(bl % lw) * (size_t)istride;
     const size_t nslot = (size_t)nwaves * 64, slot = (size_t)wave * 64 + lane;
     const int k = n / 2;                         // twist face
     const int len = bot ? n - 1 - k : k - 1;     // rows this lane eliminates (>= 1 for n >= 4)
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(256) void k_adv_twist(T *phi, const T *__restrict__
     auto emit = [&](size_t a, T t1, T t2, T Rv, T Pv, T L1v, T L2v) {
         if (MODE == 1) {
             R[a] = (Rv + t1) + t2;
-        } else {
+        } else if (!TUBE || tm.in(a)) {
             const T r = (Rv + t1) + t2;
             if (STAGE == 1) {
                 L1[a] = r;
@@ -521,14 +523,16 @@ __global__ __launch_bounds__(256) void k_adv_pair_x(const T *__restrict__ phi, c
 constexpr int PKX = 16;   // rows per lane at most (8 up to 257 cells, 16 up to 513)
 constexpr int PART_MAX_LINES_X = 32768, PART_MAX_LINES_STRIDED = 4096;
 
-template <typename T, int MODE, int STAGE, int KR>
+template <typename T, int MODE, int STAGE, int KR, bool TUBE>
 __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
                                                    int nlines, int n, int lw, long ostride, long istride,
-                                                   long estride, T dd, T dt)
+                                                   long estride, T dd, T dt, const int *__restrict__ lines,
+                                                   TubeMask tm)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int line = tid >> 6, lane = tid & 63;
-    if (line >= nlines) return;   // a whole wave per line
+    const int wline = tid >> 6, lane = tid & 63;
+    if (wline >= nlines) return;   // a whole wave per line
+    const int line = lines ? lines[wline] : wline;   // with a list, wave w solves listed line w
     const bool isf = lane >= 32;
     const int s = lane & 31;
     const int N = n - 1, K = (N + 31) / 32;
@@ -546,7 +550,7 @@ __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ 
             R[a] = t1 + t2;
         } else if (MODE == 1) {
             R[a] = (R[a] + t1) + t2;
-        } else {
+        } else if (!TUBE || tm.in(a)) {
             const T r = (R[a] + t1) + t2, Pv = phi[a];
             if (STAGE == 1) {
                 L1[a] = r;
@@ -770,13 +774,26 @@ template <typename T> LSMP_DEV bool in_band(T x)
 template <typename T, int STAGE, int BTN>
 __global__ __launch_bounds__(RNT3) void k_redis_stage3d(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                         const T *__restrict__ phi0, const T *__restrict__ S0,
-                                                        RedisArgs ra, unsigned long long *ctl)
+                                                        RedisArgs ra, unsigned long long *ctl,
+                                                        const unsigned char *__restrict__ tiles)
 {
     __shared__ T sphi[P3Y][P3X];
     __shared__ T ss2[E3Y][E3X];
     __shared__ T ss3[E3Y][E3X];
     if (ctl[1]) return;   // the pseudo-time loop has stopped (k_redis_ctl)
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
+    // narrow band: a workgroup with no cell of the tube keeps its cells.  Stage
+    // 1 writes the step's output buffer and stage 2 the one stage 3 reads, so
+    // both copy; stage 3 writes the buffer stage 1 filled, which holds them.
+    if (tiles && !tiles[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x]) {
+        if (STAGE == 3) return;
+        const int ci = blockIdx.x * R3X + (int)threadIdx.x % R3X, cj = blockIdx.y * R3Y + (int)threadIdx.x / R3X;
+        if (ci >= nx || cj >= ny) return;
+        const size_t c0 = (size_t)cj * nx + ci, pl = (size_t)nx * ny;
+        for (int k = blockIdx.z * RKC; k < min((int)(blockIdx.z + 1) * RKC, nz); ++k)
+            phi_out[(size_t)k * pl + c0] = phi[(size_t)k * pl + c0];
+        return;
+    }
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy, inv12dz = (T)ra.inv12dz;
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
     const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
@@ -1720,18 +1737,46 @@ __global__ void k_mass_update_lines(T *__restrict__ phi, const T *__restrict__ g
 // |phi| < wband.  wband = ls_wid (1 + 1e-6) covers every cell where ls_funs
 // gives delta != 0 (|phi| <= ls_wid (1 - 1e-12)).  The flags must be zero on
 // entry; fz is unused in 2D.
-template <typename T>
-__global__ void k_band_line_flags(const T *__restrict__ phi, int nx, int ny, int nz, double wband,
+//
+// TUBE (the narrow band, 3D): phi is first clamped to [-wband, wband], so the
+// cells outside the tube hold a constant, which advection leaves as it is
+// (with a frozen distance field there, the mismatch with the advected tube
+// would grow into a jump at the tube's edge that the implicit line solves
+// carry back towards the interface); H, delta and the sign do not change,
+// wband being far beyond ls_wid.  The lines of the six neighbours of every
+// band cell are flagged too: a clamped cell next to the tube is then updated
+// with its full right-hand side, and the tube can grow into the clamped field.
+template <typename T, bool TUBE>
+__global__ void k_band_line_flags(T *__restrict__ phi, int nx, int ny, int nz, double wband,
                                   unsigned char *__restrict__ fx, unsigned char *__restrict__ fy,
                                   unsigned char *__restrict__ fz)
 {
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (size_t)nx * ny * nz) return;
-    if (!(fabs((double)phi[q]) < wband)) return;
+    const double p = (double)phi[q];
+    if (TUBE && fabs(p) > wband) phi[q] = (T)(p > 0.0 ? wband : -wband);
+    if (!(fabs(p) < wband)) return;
     const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
     fx[j + (size_t)ny * k] = 1;
     fy[i + (size_t)nx * k] = 1;
     if (nz > 1) fz[i + (size_t)nx * j] = 1;
+    if (TUBE) {
+        // x line (j, k), y line (i, k), z line (i, j) of the neighbours
+        const int im = max(i - 1, 0), ip = min(i + 1, nx - 1), jm = max(j - 1, 0), jp = min(j + 1, ny - 1);
+        const int km = max(k - 1, 0), kp = min(k + 1, nz - 1);
+        fx[jm + (size_t)ny * k] = 1;
+        fx[jp + (size_t)ny * k] = 1;
+        fx[j + (size_t)ny * km] = 1;
+        fx[j + (size_t)ny * kp] = 1;
+        fy[im + (size_t)nx * k] = 1;
+        fy[ip + (size_t)nx * k] = 1;
+        fy[i + (size_t)nx * km] = 1;
+        fy[i + (size_t)nx * kp] = 1;
+        fz[im + (size_t)nx * j] = 1;
+        fz[ip + (size_t)nx * j] = 1;
+        fz[i + (size_t)nx * jm] = 1;
+        fz[i + (size_t)nx * jp] = 1;
+    }
 }
 
 // Ordered lists of the flagged lines, one workgroup per direction d (blockIdx.x):
@@ -1931,40 +1976,64 @@ static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1
 
 template <typename T>
 void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const T *vel, T *R, T *L1, T *L2,
-                     T *scr, int nx, int ny, int nz, T dd, T dt)
+                     T *scr, int nx, int ny, int nz, T dd, T dt, const int *lines, int nlist, const TubeMask &tm)
 {
     const int n = dir == 0 ? nx : dir == 1 ? ny : nz;
-    const int nlines = dir == 0 ? ny * nz : dir == 1 ? nx * nz : nx * ny;
+    const int nall = dir == 0 ? ny * nz : dir == 1 ? nx * nz : nx * ny;
     // line L starts at (L / lw) * ostride + (L % lw) * istride
-    const int lw = dir == 0 ? nlines : nx;
+    const int lw = dir == 0 ? nall : nx;
     const long ostride = dir == 0 ? 0 : dir == 1 ? (long)nx * ny : (long)nx;
     const long istride = dir == 0 ? (long)nx : 1;
     const long estride = dir == 0 ? 1 : dir == 1 ? (long)nx : (long)nx * ny;
-    const size_t threads = (size_t)((nlines + 31) / 32) * 64;       // two lanes per line (x)
-    const size_t threads4 = (size_t)((nlines + 15) / 16) * 64;      // four lanes per line (y, z)
-    const dim3 g((unsigned)((threads + 255) / 256)), g4((unsigned)((threads4 + 255) / 256)), b(256);
-#define LSMP_ADV(M, S)                                                                                          \
-    hipLaunchKernelGGL((k_adv_twist<T, M, S>), g4, b, 0, st, phi, vel, R, L1, L2, scr, nlines, n, lw, ostride, \
-                       istride, estride, dd, dt)
     // Few lines: the paired and twisted kernels leave most SIMDs idle and each
     // lane walks the whole line, so the partitioned kernel (one wave per
     // line, rows in registers) takes over.  Measured: 2D 256^2 x pass
     // 129 -> 15 us, y pass 99 -> 21 us; 128^3 x pass 70 -> 67 us (FP32
     // 58 -> 49); but strided y and z lines of 128^3 (16384 lines) 50 -> 103 us,
     // its loads being one line per lane, and at 256^3 the paired x kernel is
-    // as fast (FP64) or faster (FP32 182 against 220 us).
+    // as fast (FP64) or faster (FP32 182 against 220 us).  Listed lines (the
+    // narrow band) take it up to PART_MAX_LISTED_ADV strided lines.
+    int nlines = lines ? nlist : nall;
     const bool part = n >= 34 && n <= 32 * PKX + 1 &&
-                      nlines <= (dir == 0 ? PART_MAX_LINES_X : PART_MAX_LINES_STRIDED);
+                      nlines <= (dir == 0 ? PART_MAX_LINES_X : lines ? PART_MAX_LISTED_ADV : PART_MAX_LINES_STRIDED);
+    if (mode == 0 && !part && lines) {   // the paired x kernel takes every line
+        lines = nullptr;
+        nlines = nall;
+    }
+    if (nlines == 0) return;
+    const size_t threads = (size_t)((nlines + 31) / 32) * 64;       // two lanes per line (x)
+    const size_t threads4 = (size_t)((nlines + 15) / 16) * 64;      // four lanes per line (y, z)
+    const dim3 g((unsigned)((threads + 255) / 256)), g4((unsigned)((threads4 + 255) / 256)), b(256);
+    // the tube mask exists in separate instantiations of the stage kernels
+    const bool tube = tm.fx != nullptr;
+#define LSMP_ADVK(M, S, TB)                                                                                     \
+    hipLaunchKernelGGL((k_adv_twist<T, M, S, TB>), g4, b, 0, st, phi, vel, R, L1, L2, scr, nlines, n, lw, ostride, \
+                       istride, estride, dd, dt, lines, tm)
+#define LSMP_ADV(M, S)                                                                                          \
+    do {                                                                                                        \
+        if constexpr (M == 2) {                                                                                 \
+            if (tube) LSMP_ADVK(M, S, true);                                                                    \
+            else LSMP_ADVK(M, S, false);                                                                        \
+        } else {                                                                                                \
+            LSMP_ADVK(M, S, false);                                                                             \
+        }                                                                                                       \
+    } while (0)
     const bool part8 = n <= 32 * 8 + 1;   // eight rows per lane suffice
     const dim3 gp((unsigned)(((size_t)nlines * 64 + 255) / 256));
+#define LSMP_PARTK(M, S, KR, TB)                                                                              \
+    hipLaunchKernelGGL((k_adv_part<T, M, S, KR, TB>), gp, b, 0, st, phi, vel, R, L1, L2, nlines, n, lw, ostride, \
+                       istride, estride, dd, dt, lines, tm)
 #define LSMP_PART(M, S)                                                                                       \
     do {                                                                                                      \
-        if (part8)                                                                                            \
-            hipLaunchKernelGGL((k_adv_part<T, M, S, 8>), gp, b, 0, st, phi, vel, R, L1, L2, nlines, n, lw,     \
-                               ostride, istride, estride, dd, dt);                                            \
-        else                                                                                                  \
-            hipLaunchKernelGGL((k_adv_part<T, M, S, PKX>), gp, b, 0, st, phi, vel, R, L1, L2, nlines, n, lw,   \
-                               ostride, istride, estride, dd, dt);                                            \
+        if constexpr (M == 2) {                                                                               \
+            if (part8 && tube) LSMP_PARTK(M, S, 8, true);                                                     \
+            else if (part8) LSMP_PARTK(M, S, 8, false);                                                       \
+            else if (tube) LSMP_PARTK(M, S, PKX, true);                                                       \
+            else LSMP_PARTK(M, S, PKX, false);                                                                \
+        } else {                                                                                              \
+            if (part8) LSMP_PARTK(M, S, 8, false);                                                            \
+            else LSMP_PARTK(M, S, PKX, false);                                                                \
+        }                                                                                                     \
     } while (0)
     if (mode == 0) {
         if (part) LSMP_PART(0, 0);
@@ -1979,16 +2048,42 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
     else if (stage == 2) LSMP_ADV(2, 2);
     else LSMP_ADV(2, 3);
 #undef LSMP_ADV
+#undef LSMP_ADVK
 #undef LSMP_PART
+#undef LSMP_PARTK
+}
+
+// Narrow band: flag the reinitialisation workgroups (R3X x R3Y x RKC cells)
+// holding a cell with |phi| < wband; the flags must be zero on entry.
+template <typename T>
+__global__ void k_tube_tiles(const T *__restrict__ phi, int nx, int ny, int nz, double wband,
+                             unsigned char *__restrict__ tiles)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)nx * ny * nz) return;
+    if (!(fabs((double)phi[q]) < wband)) return;
+    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+    const int gx = (nx + R3X - 1) / R3X, gy = (ny + R3Y - 1) / R3Y;
+    tiles[((size_t)(k / RKC) * gy + j / R3Y) * gx + i / R3X] = 1;
+}
+
+template <typename T>
+void launch_tube_tiles(hipStream_t st, const T *phi, int nx, int ny, int nz, double wband, unsigned char *tiles)
+{
+    const size_t nt = (size_t)nblk(nx, R3X) * nblk(ny, R3Y) * nblk(nz, RKC);
+    hipLaunchKernelGGL(k_zero_bytes, dim3(nblk(nt, 256)), dim3(256), 0, st, tiles, nt);
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL((k_tube_tiles<T>), dim3(nblk(n, 256)), dim3(256), 0, st, phi, nx, ny, nz, wband, tiles);
 }
 
 template <typename T>
 void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *phi, T *phi_out, const T *phi0,
-                        const T *S0, const RedisArgs &ra, unsigned long long *ctl)
+                        const T *S0, const RedisArgs &ra, unsigned long long *ctl, const unsigned char *tiles)
 {
     if (dim == 3) {
         const dim3 g(nblk(ra.nx, R3X), nblk(ra.ny, R3Y), nblk(ra.nz, RKC));
-#define LSMP_R3(S, B) hipLaunchKernelGGL((k_redis_stage3d<T, S, B>), g, dim3(RNT3), 0, st, phi, phi_out, phi0, S0, ra, ctl)
+#define LSMP_R3(S, B)                                                                                          \
+    hipLaunchKernelGGL((k_redis_stage3d<T, S, B>), g, dim3(RNT3), 0, st, phi, phi_out, phi0, S0, ra, ctl, tiles)
         if (btn) { if (stage == 1) LSMP_R3(1, 1); else if (stage == 2) LSMP_R3(2, 1); else LSMP_R3(3, 1); }
         else { if (stage == 1) LSMP_R3(1, 0); else if (stage == 2) LSMP_R3(2, 0); else LSMP_R3(3, 0); }
 #undef LSMP_R3
@@ -2155,15 +2250,20 @@ void launch_finalize3(hipStream_t st, const double *partial, int nblk_, double i
 }
 
 template <typename T>
-void launch_band_lines(hipStream_t st, int dim, const T *phi, int nx, int ny, int nz, double wband,
-                       unsigned char *flags, unsigned *count, int *listx, int *listy, int *listz)
+void launch_band_lines(hipStream_t st, int dim, T *phi, int nx, int ny, int nz, double wband,
+                       unsigned char *flags, unsigned *count, int *listx, int *listy, int *listz, bool clamp)
 {
     const int nfx = ny * nz, nfy = nx * nz, nfz = dim == 3 ? nx * ny : 0;
     const size_t nbytes = band_flag_bytes(nfx, nfy, nfz);
     hipLaunchKernelGGL(k_zero_bytes, dim3(nblk(nbytes, 256)), dim3(256), 0, st, flags, nbytes);
     const size_t n = (size_t)nx * ny * nz;
-    hipLaunchKernelGGL((k_band_line_flags<T>), dim3(nblk(n, 256)), dim3(256), 0, st, phi, nx, ny, nz, wband,
-                       flags, flags + band_flag_offset(1, nfx, nfy), flags + band_flag_offset(2, nfx, nfy));
+    unsigned char *fy = flags + band_flag_offset(1, nfx, nfy), *fz = flags + band_flag_offset(2, nfx, nfy);
+    if (clamp)
+        hipLaunchKernelGGL((k_band_line_flags<T, true>), dim3(nblk(n, 256)), dim3(256), 0, st, phi, nx, ny, nz,
+                           wband, flags, fy, fz);
+    else
+        hipLaunchKernelGGL((k_band_line_flags<T, false>), dim3(nblk(n, 256)), dim3(256), 0, st, phi, nx, ny, nz,
+                           wband, flags, fy, fz);
     hipLaunchKernelGGL(k_compact_lines, dim3(dim), dim3(1024), 0, st, flags, nfx, nfy, nfz, listx, listy, listz,
                        count);
 }
@@ -2243,10 +2343,11 @@ void launch_zero_u64(hipStream_t st, unsigned long long *p, int n)
 
 #define LSMP_INST(T)                                                                                           \
     template void launch_adv_pass<T>(hipStream_t, int, int, int, T *, const T *, T *, T *, T *, T *, int, int, int, \
-                                     T, T);                                                                    \
+                                     T, T, const int *, int, const TubeMask &);                                \
     template void launch_combine_markers<T>(hipStream_t, const T *, const T *, T *, size_t, unsigned *);         \
     template void launch_redis_stage<T>(hipStream_t, int, int, int, const T *, T *, const T *, const T *,          \
-                                        const RedisArgs &, unsigned long long *);                              \
+                                        const RedisArgs &, unsigned long long *, const unsigned char *);       \
+    template void launch_tube_tiles<T>(hipStream_t, const T *, int, int, int, double, unsigned char *);          \
     template void launch_sign_init<T>(hipStream_t, const T *, T *, size_t, T, int);                            \
     template void launch_grad_max<T>(hipStream_t, int, const T *, const T *, const RedisArgs &,                \
                                      unsigned long long *);                                                    \
@@ -2258,8 +2359,8 @@ void launch_zero_u64(hipStream_t st, unsigned long long *p, int n)
                                        const CcdArgs &, double *, int *);                                      \
     template void launch_mass_update_lines<T>(hipStream_t, int, T *, const T *, T, const double *, const int *,   \
                                               int, int, int, int);                                             \
-    template void launch_band_lines<T>(hipStream_t, int, const T *, int, int, int, double, unsigned char *,      \
-                                       unsigned *, int *, int *, int *);                                       \
+    template void launch_band_lines<T>(hipStream_t, int, T *, int, int, int, double, unsigned char *,            \
+                                       unsigned *, int *, int *, int *, bool);                                 \
     template void launch_mass_update<T>(hipStream_t, T *, const T *, size_t, T, const double *);               \
     template void launch_mass_vol<T>(hipStream_t, const T *, size_t, const CcdArgs &, double *, int);           \
     template void launch_l1<T>(hipStream_t, const T *, const T *, size_t, double, double *, int);              \

@@ -38,7 +38,7 @@ def sphere_box(n):
 
 
 def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsmp.FALLBACK_CONTINUE,
-        log=None):
+        log=None, narrow_band=0.0):
     prec = lsmp.FP64 if form.startswith("fp64") else lsmp.FP32
     learned = form.endswith("learned")
     net = None
@@ -46,6 +46,7 @@ def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsm
         net = lsmp.Net()
         net.load(weights or os.path.join(HERE, "weights", "reinit_mlp.npy"))
     s = lsmp.Solver(lsmp.make_params(n, dim=3, precision=prec))
+    s.set_narrow_band(narrow_band)
     s.upload_phi(sphere_box(n))
     s.store_phi0()
     imass, _ = s.mass_vol()
@@ -90,7 +91,7 @@ def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsm
     s.synchronize()
     el = time.perf_counter() - t0
     mass, _ = s.mass_vol()
-    out = dict(form=form, n=n, T=T, t_end=t, steps=nsteps, ms_per_step=el / nsteps * 1e3,
+    out = dict(form=form, narrow_band_cells=narrow_band, n=n, T=T, t_end=t, steps=nsteps, ms_per_step=el / nsteps * 1e3,
                cell_updates_per_s=n ** 3 * nsteps / el, l1=s.interface_l1(), dM_over_M=abs(mass - imass) / imass,
                pseudo_per_step=pseudo / nsteps, fallbacks=fallbacks, windows=wres)
     s.close()
@@ -107,9 +108,11 @@ def main(argv=None):
     ap.add_argument("--frac", type=float, default=1.0, help="run this fraction of T")
     ap.add_argument("--weights", default=None)
     ap.add_argument("--tol", type=float, default=1e-2)
+    ap.add_argument("--narrow-band", type=float, default=0.0,
+                    help="advect and reinitialise only where |phi| < this many cells (0: everywhere)")
     args = ap.parse_args(argv)
     r = run(args.n, args.form, T=args.T, frac=args.frac, weights=args.weights, tol=args.tol,
-            log=lambda m: print(m, file=sys.stderr, flush=True))
+            log=lambda m: print(m, file=sys.stderr, flush=True), narrow_band=args.narrow_band)
     print(json.dumps(r), flush=True)
     return 0
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "lsmp_store_phi0", "lsmp_set_velocity", "lsmp_set_velocity_analytic", "lsmp_rk3_advect",
     "lsmp_reinit", "lsmp_mass_vol", "lsmp_mass_fix", "lsmp_interface_l1", "lsmp_get_stats",
     "lsmp_set_profiling", "lsmp_kernel_time", "lsmp_synchronize", "lsmp_set_advection_scheme",
-    "lsmp_set_band_lines", "lsmp_band_line_counts", "lsmp_component_masses",
+    "lsmp_set_band_lines", "lsmp_band_line_counts", "lsmp_set_narrow_band", "lsmp_component_masses",
     "lsmp_set_marker", "lsmp_get_marker", "lsmp_marker_load", "lsmp_marker_store", "lsmp_combine_markers",
     "lsmp_net_create", "lsmp_net_destroy", "lsmp_net_param_count", "lsmp_net_set_params",
     "lsmp_net_get_params", "lsmp_net_dataset_size", "lsmp_net_collect", "lsmp_net_train",
@@ -102,6 +102,7 @@ def load_library(path=LIB_PATH):
     L.lsmp_set_advection_scheme.argtypes = [H, C.c_int]
     L.lsmp_set_band_lines.argtypes = [H, C.c_int]
     L.lsmp_band_line_counts.argtypes = [H, C.POINTER(C.c_int)]
+    L.lsmp_set_narrow_band.argtypes = [H, C.c_double]
     L.lsmp_component_masses.argtypes = [H, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_int), C.c_int, _DP, C.POINTER(C.c_int)]
     _FP = C.POINTER(C.c_float)
@@ -215,6 +216,11 @@ class Solver:
         """Mass correction: CCD normals only on the lines crossing the band
         (True, the default) or on every line (False); same bits either way."""
         _check(self.lib.lsmp_set_band_lines(self.h, 1 if on else 0))
+
+    def set_narrow_band(self, width_cells):
+        """Narrow band of half-width width_cells (3D; 0 turns it off): advection
+        and reinitialisation only where |phi| < width_cells * dx."""
+        _check(self.lib.lsmp_set_narrow_band(self.h, float(width_cells)))
 
     def band_line_counts(self):
         """Lines listed by the last mass correction along x, y, z (-1: every line)."""

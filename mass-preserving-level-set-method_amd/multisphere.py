@@ -97,7 +97,8 @@ def interface_masses(s, cells, ls_wid):
     return np.array([masses[c] if c >= 0 else np.nan for c in comp]), comp, masses
 
 
-def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.01, weights=None, log=print):
+def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.01, weights=None, log=print,
+        narrow_band=0.0):
     """The case at n^3.  Two interfaces are told apart only while their
     surfaces stay more than 2 ls_wid + dx = 3.5 dx apart (the regions
     phi > -ls_wid then share no face); the default gap 0.01 clears that at
@@ -113,6 +114,7 @@ def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.
         net = lsmp.Net()
         net.load(weights or os.path.join(HERE, "weights", "reinit_mlp.npy"))
     s = lsmp.Solver(lsmp.make_params(n, dim=3, precision=prec))
+    s.set_narrow_band(narrow_band)
     s.upload_phi(box)
     del box
     s.store_phi0()
@@ -151,7 +153,7 @@ def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.
     rel = np.abs(m1 - m0) / m0
     # components holding no sphere centre: pieces broken off, or islands
     frag = np.setdiff1d(np.arange(all1.size), comp1[comp1 >= 0])
-    out = dict(form=form, n=n, spheres=count, T=T, t_end=t, steps=nsteps, ms_per_step=el / nsteps * 1e3,
+    out = dict(form=form, narrow_band_cells=narrow_band, n=n, spheres=count, T=T, t_end=t, steps=nsteps, ms_per_step=el / nsteps * 1e3,
                cell_updates_per_s=n ** 3 * nsteps / el, l1=s.interface_l1(), dM_over_M=abs(mass - imass) / imass,
                components_start=int(len(set(comp0.tolist()) - {-1})),
                components_end=int(len(set(comp1.tolist()) - {-1})), components_all_end=int(all1.size),
@@ -176,8 +178,11 @@ def main(argv=None):
     ap.add_argument("--T", type=float, default=3.0)
     ap.add_argument("--frac", type=float, default=1.0)
     ap.add_argument("--spheres", type=int, default=64)
+    ap.add_argument("--narrow-band", type=float, default=0.0,
+                    help="advect and reinitialise only where |phi| < this many cells (0: everywhere)")
     args = ap.parse_args(argv)
-    r = run(args.n, args.form, args.T, args.frac, args.spheres, log=lambda m: print(m, file=sys.stderr, flush=True))
+    r = run(args.n, args.form, args.T, args.frac, args.spheres, log=lambda m: print(m, file=sys.stderr, flush=True),
+            narrow_band=args.narrow_band)
     print(json.dumps(r), flush=True)
     return 0
 
