@@ -134,7 +134,7 @@ int lsmp_band_line_counts(lsmp_handle h, int counts[3]);
  * only, at least 4): lsmp_rk3_advect (CRWENO) first clamps phi to [-W, W],
  * then solves only the lines that cross the tube |phi| < W and updates only
  * the cells whose three lines do; lsmp_reinit updates only the workgroups of
- * 32 x 8 x 32 cells that hold a cell |phi| < W; every other cell keeps its
+ * 32 x 8 x 8 cells that hold a cell |phi| < W; every other cell keeps its
  * value.  Results differ from the full-domain method away from the interface
  * (the far field is the constant +-W), so the FP64 drop-in leaves it off
  * (width_cells = 0, the default). */

@@ -701,7 +701,8 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
         alloc((void **)&c->llist[1], nfy * sizeof(int)) || alloc((void **)&c->llist[2], nfz * sizeof(int)))
         return fail(LSMP_E_HIP);
     if (c->dim == 3 &&
-        alloc((void **)&c->tiles, (size_t)((c->nx + R3X - 1) / R3X) * ((c->ny + R3Y - 1) / R3Y) * ((c->nz + RKC - 1) / RKC)))
+        alloc((void **)&c->tiles,
+              (size_t)((c->nx + R3X - 1) / R3X) * ((c->ny + R3Y - 1) / R3Y) * ((c->nz + RKC_NB - 1) / RKC_NB)))
         return fail(LSMP_E_HIP);
     // velocity starts at zero so that an advect before set_velocity is defined;
     // the normals start at zero because a mass correction on band lines leaves

@@ -37,6 +37,9 @@ constexpr int R3X = 32, R3Y = LSMP_R3Y;
 constexpr int E3X = R3X + 2, E3Y = R3Y + 2;
 constexpr int P3X = R3X + 8, P3Y = R3Y + 8;
 constexpr int RKC = 32;
+// the narrow band's workgroups march RKC_NB planes: a workgroup without a tube
+// cell is skipped, and an active one's march is the kernel's critical path
+constexpr int RKC_NB = 8;
 
 struct RedisArgs {
     int nx, ny, nz;

@@ -771,7 +771,7 @@ template <typename T> LSMP_DEV bool in_band(T x)
     return !(x > T(1) - T(1.0e-12)) && !(x < T(-1) + T(1.0e-12));
 }
 
-template <typename T, int STAGE, int BTN>
+template <typename T, int STAGE, int BTN, int KC>
 __global__ __launch_bounds__(RNT3) void k_redis_stage3d(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                         const T *__restrict__ phi0, const T *__restrict__ S0,
                                                         RedisArgs ra, unsigned long long *ctl,
@@ -790,7 +790,7 @@ __global__ __launch_bounds__(RNT3) void k_redis_stage3d(const T *__restrict__ ph
         const int ci = blockIdx.x * R3X + (int)threadIdx.x % R3X, cj = blockIdx.y * R3Y + (int)threadIdx.x / R3X;
         if (ci >= nx || cj >= ny) return;
         const size_t c0 = (size_t)cj * nx + ci, pl = (size_t)nx * ny;
-        for (int k = blockIdx.z * RKC; k < min((int)(blockIdx.z + 1) * RKC, nz); ++k)
+        for (int k = blockIdx.z * KC; k < min((int)(blockIdx.z + 1) * KC, nz); ++k)
             phi_out[(size_t)k * pl + c0] = phi[(size_t)k * pl + c0];
         return;
     }
@@ -798,8 +798,8 @@ __global__ __launch_bounds__(RNT3) void k_redis_stage3d(const T *__restrict__ ph
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
     const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
     const int t = threadIdx.x;
-    const int i0 = blockIdx.x * R3X, j0 = blockIdx.y * R3Y, k0 = blockIdx.z * RKC;
-    const int k1 = min(k0 + RKC, nz);
+    const int i0 = blockIdx.x * R3X, j0 = blockIdx.y * R3Y, k0 = blockIdx.z * KC;
+    const int k1 = min(k0 + KC, nz);
     const int tx = t % R3X, ty = t / R3X;
     const int ex = tx + 1, ey = ty + 1;
     // cells past the domain edge hold the boundary cell (zero-gradient ghosts)
@@ -2053,7 +2053,7 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
 #undef LSMP_PARTK
 }
 
-// Narrow band: flag the reinitialisation workgroups (R3X x R3Y x RKC cells)
+// Narrow band: flag the reinitialisation workgroups (R3X x R3Y x RKC_NB cells)
 // holding a cell with |phi| < wband; the flags must be zero on entry.
 template <typename T>
 __global__ void k_tube_tiles(const T *__restrict__ phi, int nx, int ny, int nz, double wband,
@@ -2064,13 +2064,13 @@ __global__ void k_tube_tiles(const T *__restrict__ phi, int nx, int ny, int nz, 
     if (!(fabs((double)phi[q]) < wband)) return;
     const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
     const int gx = (nx + R3X - 1) / R3X, gy = (ny + R3Y - 1) / R3Y;
-    tiles[((size_t)(k / RKC) * gy + j / R3Y) * gx + i / R3X] = 1;
+    tiles[((size_t)(k / RKC_NB) * gy + j / R3Y) * gx + i / R3X] = 1;
 }
 
 template <typename T>
 void launch_tube_tiles(hipStream_t st, const T *phi, int nx, int ny, int nz, double wband, unsigned char *tiles)
 {
-    const size_t nt = (size_t)nblk(nx, R3X) * nblk(ny, R3Y) * nblk(nz, RKC);
+    const size_t nt = (size_t)nblk(nx, R3X) * nblk(ny, R3Y) * nblk(nz, RKC_NB);
     hipLaunchKernelGGL(k_zero_bytes, dim3(nblk(nt, 256)), dim3(256), 0, st, tiles, nt);
     const size_t n = (size_t)nx * ny * nz;
     hipLaunchKernelGGL((k_tube_tiles<T>), dim3(nblk(n, 256)), dim3(256), 0, st, phi, nx, ny, nz, wband, tiles);
@@ -2081,9 +2081,18 @@ void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *ph
                         const T *S0, const RedisArgs &ra, unsigned long long *ctl, const unsigned char *tiles)
 {
     if (dim == 3) {
-        const dim3 g(nblk(ra.nx, R3X), nblk(ra.ny, R3Y), nblk(ra.nz, RKC));
+        // the narrow band (tiles) marches RKC_NB planes per workgroup
+        const dim3 g(nblk(ra.nx, R3X), nblk(ra.ny, R3Y), nblk(ra.nz, RKC)),
+            gt(nblk(ra.nx, R3X), nblk(ra.ny, R3Y), nblk(ra.nz, RKC_NB));
 #define LSMP_R3(S, B)                                                                                          \
-    hipLaunchKernelGGL((k_redis_stage3d<T, S, B>), g, dim3(RNT3), 0, st, phi, phi_out, phi0, S0, ra, ctl, tiles)
+    do {                                                                                                       \
+        if (tiles)                                                                                             \
+            hipLaunchKernelGGL((k_redis_stage3d<T, S, B, RKC_NB>), gt, dim3(RNT3), 0, st, phi, phi_out, phi0,   \
+                               S0, ra, ctl, tiles);                                                            \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_redis_stage3d<T, S, B, RKC>), g, dim3(RNT3), 0, st, phi, phi_out, phi0, S0,   \
+                               ra, ctl, tiles);                                                                \
+    } while (0)
         if (btn) { if (stage == 1) LSMP_R3(1, 1); else if (stage == 2) LSMP_R3(2, 1); else LSMP_R3(3, 1); }
         else { if (stage == 1) LSMP_R3(1, 0); else if (stage == 2) LSMP_R3(2, 0); else LSMP_R3(3, 0); }
 #undef LSMP_R3

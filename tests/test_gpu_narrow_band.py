@@ -1,7 +1,7 @@
 """Narrow band (lsmp_set_narrow_band, the chosen forms' option): advection
 clamps phi to [-W, W], solves only the lines crossing the tube |phi| < W and
 updates only the cells whose three lines cross it; reinitialisation updates
-only the workgroups of 32 x 8 x 32 cells that hold a tube cell.  Checked:
+only the workgroups of 32 x 8 x 8 cells that hold a tube cell.  Checked:
 every other cell keeps its (clamped) value bit for bit, and near the
 interface (|phi| < 3 dx) the field follows the full-domain method to the
 tolerances written in each test."""
@@ -16,7 +16,7 @@ from conftest import load_lsmp
 pytestmark = pytest.mark.gpu
 
 W = 10.0          # tube half-width in cells
-TILE = (32, 8, 32)  # reinitialisation workgroup: x, y, z cells
+TILE = (32, 8, 8)   # reinitialisation workgroup of the narrow band: x, y, z cells
 
 
 def say(*a):
