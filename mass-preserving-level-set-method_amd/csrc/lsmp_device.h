@@ -13,6 +13,17 @@ template <typename T> LSMP_DEV T sq(T x) { return x * x; }
 template <typename T> LSMP_DEV T dabs(T x) { return fabs(x); }
 LSMP_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Workgroups are handed to the 8 XCDs of the device in turn, each XCD with its
+// own L2.  This renumbers workgroup b so that XCD x takes a contiguous range
+// (the first nb % 8 XCDs one workgroup more): neighbouring strided lines, which
+// share cache lines, then run on the same XCD at about the same time.
+LSMP_DEV int xcd_block()
+{
+    const int b = (int)blockIdx.x, nb = (int)gridDim.x;
+    const int x = b % 8, i = b / 8, q = nb / 8, r = nb % 8;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // Pointwise flux split g = 1/2 (u + |u|) phi, f = 1/2 (u - |u|) phi
 // (level_set_rk3_solver.f90:118-125).
 template <typename T> LSMP_DEV void flux_split(T phi, T u, T &g, T &f)

@@ -529,7 +529,7 @@ __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ 
                                                    long estride, T dd, T dt, const int *__restrict__ lines,
                                                    TubeMask tm)
 {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = xcd_block() * blockDim.x + threadIdx.x;
     const int wline = tid >> 6, lane = tid & 63;
     if (wline >= nlines) return;   // a whole wave per line
     const int line = lines ? lines[wline] : wline;   // with a list, wave w solves listed line w
@@ -1323,7 +1323,7 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     }
     const T *A = tab, *B = tab + 3 * nb, *AA = tab + 6 * nb, *BB = tab + 9 * nb, *IDEN = tab + 12 * nb;
     const int lane = threadIdx.x & 63, s = lane & 31, half = lane & 32;
-    const int line = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
+    const int line = (int)((xcd_block() * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
     const bool act = line < nlines;
     const int Lc = act ? line : nlines - 1;   // idle halves shadow the last line and emit nothing
     const int bl = lines ? lines[Lc] : Lc;     // with a list, half `line` solves listed line `line` of nlines
