@@ -123,6 +123,18 @@ interface
         type(c_ptr), value :: h
         integer(c_int), value :: on
     end function
+    ! lines the last mass correction solved along x, y, z (-1: every line)
+    integer(c_int) function c_lsmp_band_line_counts(h, counts) bind(C, name="lsmp_band_line_counts")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int), intent(out) :: counts(3)
+    end function
+    ! narrow band of half-width width_cells (3D; 0 = off): the chosen forms only
+    integer(c_int) function c_lsmp_set_narrow_band(h, width_cells) bind(C, name="lsmp_set_narrow_band")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), value :: width_cells
+    end function
     ! two-marker MPLS branch (interface_solver.f90:34-121)
     integer(c_int) function c_lsmp_set_marker(h, mid, box) bind(C, name="lsmp_set_marker")
         import :: c_int, c_ptr, c_double
@@ -160,7 +172,7 @@ end interface
 public :: c_lsmp_init, c_lsmp_finalize, c_lsmp_upload_phi, c_lsmp_download_phi, c_lsmp_store_phi0
 public :: c_lsmp_set_velocity, c_lsmp_set_velocity_analytic, c_lsmp_rk3_advect, c_lsmp_reinit
 public :: c_lsmp_mass_vol, c_lsmp_mass_fix, c_lsmp_interface_l1, c_lsmp_get_stats, c_lsmp_synchronize
-public :: c_lsmp_set_advection_scheme, c_lsmp_set_band_lines
+public :: c_lsmp_set_advection_scheme, c_lsmp_set_band_lines, c_lsmp_band_line_counts, c_lsmp_set_narrow_band
 public :: c_lsmp_set_marker, c_lsmp_get_marker, c_lsmp_marker_load, c_lsmp_marker_store, c_lsmp_combine_markers
 public :: lsmp_check, lsmp_error_message
 
