@@ -154,20 +154,34 @@ __global__ __launch_bounds__(256) void k_band_count(const T *__restrict__ phi, s
     if (threadIdx.x == 0) cnt[blockIdx.x] = (unsigned)total;
 }
 
-// Exclusive scan of cnt[0..nblk) into off[], total into off[nblk]; one workgroup.
-__global__ __launch_bounds__(256) void k_band_scan(const unsigned *__restrict__ cnt, int nblk,
-                                                   unsigned *__restrict__ off)
+// Exclusive scan of cnt[0..nblk) into off[], total into off[nblk]; one
+// workgroup of 1024 threads, each summing a contiguous run of counts, an
+// exclusive scan of the runs' sums, then each run written from its offset.
+__global__ __launch_bounds__(1024) void k_band_scan(const unsigned *__restrict__ cnt, int nblk,
+                                                    unsigned *__restrict__ off)
 {
-    unsigned carry = 0;
-    for (int b0 = 0; b0 < nblk; b0 += 256) {
-        const int b = b0 + threadIdx.x;
-        const int v = b < nblk ? (int)cnt[b] : 0;
-        int total;
-        const int ex = block_scan256(v, total);
-        if (b < nblk) off[b] = carry + (unsigned)ex;
-        carry += (unsigned)total;
+    __shared__ unsigned wsum[16];
+    const int per = (nblk + 1023) / 1024;
+    const int b0 = min((int)threadIdx.x * per, nblk), b1 = min(b0 + per, nblk);
+    unsigned run = 0;
+    for (int b = b0; b < b1; ++b) run += cnt[b];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    unsigned inc = run;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+        const unsigned y = __shfl_up(inc, k, 64);
+        if (lane >= k) inc += y;
     }
-    if (threadIdx.x == 0) off[nblk] = carry;
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    unsigned base = 0;
+    for (int w = 0; w < wid; ++w) base += wsum[w];
+    unsigned o = base + inc - run;
+    for (int b = b0; b < b1; ++b) {
+        off[b] = o;
+        o += cnt[b];
+    }
+    if (threadIdx.x == 1023) off[nblk] = base + inc;
 }
 
 template <typename T>
@@ -674,7 +688,7 @@ template <typename T> int build_band(lsmp_ctx *c, const T *phi, double thr, unsi
     const int nblk = (int)((c->ncell + BAND_CHUNK - 1) / BAND_CHUNK);
     unsigned *cnt = c->band_count, *off = c->band_count + nblk;
     hipLaunchKernelGGL((k_band_count<T>), dim3(nblk), dim3(256), 0, c->st, phi, c->ncell, (T)thr, cnt);
-    hipLaunchKernelGGL(k_band_scan, dim3(1), dim3(256), 0, c->st, cnt, nblk, off);
+    hipLaunchKernelGGL(k_band_scan, dim3(1), dim3(1024), 0, c->st, cnt, nblk, off);
     hipLaunchKernelGGL((k_band_write<T>), dim3(nblk), dim3(256), 0, c->st, phi, c->ncell, (T)thr, off,
                        c->band_list);
     NET_HIP(hipGetLastError());
