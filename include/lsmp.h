@@ -119,12 +119,14 @@ int lsmp_mass_vol(lsmp_handle h, double *mass, double *vol);
 int lsmp_mass_fix(lsmp_handle h, double imass, double *mass, int *n_iter);
 
 /* Band lines in the mass correction (no counterpart in the reference, whose
- * surface_norms solves every line, derivatives.f90:11-50).  on = 1 (the
- * default): the CCD normals are solved only on the lines that cross the
- * interface band |phi| < ls_wid, the only cells where the correction
- * delta |grad phi| and its sum are nonzero; the results are bitwise those of
- * on = 0, which solves every line. */
-int lsmp_set_band_lines(lsmp_handle h, int on);
+ * surface_norms solves every line, derivatives.f90:11-50): the CCD normals
+ * solved only on the lines that cross the interface band |phi| < ls_wid, the
+ * only cells where the correction delta |grad phi| and its sum are nonzero.
+ * mode 0: every line; 1 (the default): band lines on 3D grids with more than
+ * 4096 lines in a direction and lines of at most 540 cells, where they pay,
+ * every line elsewhere; 2: band lines always.  The modes agree to the drop-in
+ * tolerance (the listed lines take the 32-lanes-per-line solve). */
+int lsmp_set_band_lines(lsmp_handle h, int mode);
 /* Lines listed by the last mass correction along x, y and z (z: 0 in 2D);
  * all -1 when it solved every line. */
 int lsmp_band_line_counts(lsmp_handle h, int counts[3]);

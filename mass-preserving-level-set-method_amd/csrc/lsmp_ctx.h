@@ -37,7 +37,7 @@ struct lsmp_ctx {
     unsigned long long *rctl = nullptr;   // pseudo-time loop control block (k_redis_ctl)
     double *hbuf = nullptr;   // pinned readback
     int npartial = 0;
-    bool band_lines = true;          // mass correction: CCD normals on band lines only
+    int band_lines = 1;              // mass correction on band lines: 0 never, 1 where it pays, 2 always
     unsigned char *lflag = nullptr;  // band flags of the x, y, z lines (band_flag_bytes)
     unsigned *lcount = nullptr;      // listed lines per direction
     int *llist[3] = {nullptr, nullptr, nullptr};

@@ -465,7 +465,20 @@ template <typename T> int mass_fix(lsmp_ctx *c, double imass, double *mass_out, 
     const int dfin = c->dim == 3 ? 2 : 1;   // direction of the final pass (its sums)
     for (int d = 0; d < 3; ++d) c->last_lines[d] = -1;
     int nfixed = 0;                         // partials of the unlisted lines, constant over the iterations
-    if (c->band_lines) {
+    // Band lines pay where solving every line would take the one-lane kernel
+    // (more than PART_MAX_LINES_STRIDED lines in a direction) and the listed
+    // lines fit the 32-lane one: 3D grids from about 65^3 with lines of at
+    // most 540 cells.  Elsewhere every line already goes through the 32-lane
+    // solve, or the one-lane solve of long lines gains nothing from fewer lanes.
+    const int ns[3] = {c->nx, c->ny, c->nz};
+    bool pays = c->dim == 3;
+    size_t maxl = 0;
+    for (int d = 0; d < c->dim; ++d) {
+        pays = pays && ns[d] + 4 <= 32 * KCC;
+        maxl = std::max(maxl, c->ncell / (size_t)ns[d]);
+    }
+    pays = pays && maxl > (size_t)PART_MAX_LINES_STRIDED;
+    if (c->band_lines == 2 || (c->band_lines == 1 && pays)) {
         KT(c, "band_lines", (launch_band_lines<T>(c->st, c->dim, phi, c->nx, c->ny, c->nz,
                                                   c->p.ls_wid * (1.0 + 1.0e-6), c->lflag, c->lcount, c->llist[0],
                                                   c->llist[1], c->llist[2], false)));
@@ -504,7 +517,7 @@ template <typename T> int mass_fix(lsmp_ctx *c, double imass, double *mass_out, 
         nblk += nfixed;
         if (nblk > c->npartial) return LSMP_E_INVALID;
         KT(c, "finalize", (launch_finalize3(c->st, c->partial, nblk, imass, c->scal)));
-        if (c->band_lines)
+        if (lst[dfin])
             KT(c, "mass_update", (launch_mass_update_lines<T>(c->st, dfin, phi, (const T *)c->gr, (T)c->p.ls_wid,
                                                               c->scal, lst[dfin], nlst[dfin], c->nx, c->ny, c->nz)));
         else
@@ -913,10 +926,10 @@ int lsmp_synchronize(lsmp_handle h)
     return LSMP_OK;
 }
 
-int lsmp_set_band_lines(lsmp_handle h, int on)
+int lsmp_set_band_lines(lsmp_handle h, int mode)
 {
-    if (!h || (on != 0 && on != 1)) return LSMP_E_INVALID;
-    h->band_lines = on == 1;
+    if (!h || mode < 0 || mode > 2) return LSMP_E_INVALID;
+    h->band_lines = mode;
     return LSMP_OK;
 }
 

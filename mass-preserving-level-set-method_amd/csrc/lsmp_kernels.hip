@@ -521,7 +521,6 @@ __global__ __launch_bounds__(256) void k_adv_pair_x(const T *__restrict__ phi, c
 // one Thomas sweep only in rounding.  In MODE 2 the in-place update is safe:
 // a line's cells are read by its own wave only, all before the reduction.
 constexpr int PKX = 16;   // rows per lane at most (8 up to 257 cells, 16 up to 513)
-constexpr int PART_MAX_LINES_X = 32768, PART_MAX_LINES_STRIDED = 4096;
 
 template <typename T, int MODE, int STAGE, int KR, bool TUBE>
 __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
@@ -1303,7 +1302,6 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
 // the lane re-runs its rows from that state with the reference's per-row
 // arithmetic; the backward sweep does the same from the closure row down.
 // Only the states entering each lane's rows carry the scan's rounding.
-constexpr int KCC = 17;   // rows per lane at most (9 up to 284 cells, 17 up to 540)
 template <typename T, int DIM, bool FINAL, bool TLDS, int KC>
 __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, const T *__restrict__ coef,
                                                    T *__restrict__ out, const T *__restrict__ dxv,

@@ -117,11 +117,11 @@ interface
         type(c_ptr), value :: h
         integer(c_int), value :: scheme
     end function
-    ! mass correction on the lines crossing the band only (1, default) or on every line (0)
-    integer(c_int) function c_lsmp_set_band_lines(h, on) bind(C, name="lsmp_set_band_lines")
+    ! mass correction on every line (0), on the band lines where that pays (1, default) or always (2)
+    integer(c_int) function c_lsmp_set_band_lines(h, mode) bind(C, name="lsmp_set_band_lines")
         import :: c_int, c_ptr
         type(c_ptr), value :: h
-        integer(c_int), value :: on
+        integer(c_int), value :: mode
     end function
     ! lines the last mass correction solved along x, y, z (-1: every line)
     integer(c_int) function c_lsmp_band_line_counts(h, counts) bind(C, name="lsmp_band_line_counts")

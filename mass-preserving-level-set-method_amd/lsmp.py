@@ -212,10 +212,11 @@ class Solver:
         """SCHEME_CRWENO (level_set_rk3_source_weno) or SCHEME_UCCD (source_ccd)."""
         _check(self.lib.lsmp_set_advection_scheme(self.h, int(scheme)))
 
-    def set_band_lines(self, on):
-        """Mass correction: CCD normals only on the lines crossing the band
-        (True, the default) or on every line (False); same bits either way."""
-        _check(self.lib.lsmp_set_band_lines(self.h, 1 if on else 0))
+    def set_band_lines(self, mode):
+        """Mass correction: CCD normals on every line (0), on the lines crossing
+        the band where that pays (1, the default: 3D grids with more than 4096
+        lines in a direction and lines of at most 540 cells) or always (2)."""
+        _check(self.lib.lsmp_set_band_lines(self.h, int(mode)))
 
     def set_narrow_band(self, width_cells):
         """Narrow band of half-width width_cells (3D; 0 turns it off): advection

@@ -75,7 +75,8 @@ def test_band_lines_match_all_lines(case):
     mtol = 1e-13 if case["precision"] == 0 else 1e-8
     say("band lines", case)
     with lsmp.Solver(prm) as a, lsmp.Solver(prm) as b:
-        b.set_band_lines(False)
+        a.set_band_lines(2)   # band lines always (the default takes them in 3D only)
+        b.set_band_lines(0)
         for s in (a, b):
             s.upload_phi(box)
             s.store_phi0()
@@ -113,7 +114,29 @@ def test_band_lines_match_all_lines(case):
 def test_band_lines_argument_checked():
     lsmp = load_lsmp()
     with lsmp.Solver(lsmp.make_params(8, dim=3)) as s:
-        assert s.lib.lsmp_set_band_lines(s.h, 2) == -1   # LSMP_E_INVALID
+        assert s.lib.lsmp_set_band_lines(s.h, 3) == -1   # LSMP_E_INVALID
+        assert s.lib.lsmp_set_band_lines(s.h, -1) == -1
         assert s.lib.lsmp_set_band_lines(None, 1) == -1
-        s.set_band_lines(False)
-        s.set_band_lines(True)
+        for mode in (0, 2, 1):
+            s.set_band_lines(mode)
+
+
+@pytest.mark.parametrize("case, listed", [(dict(nx=72, ny=66, nz=68, dim=3), True),
+                                          (dict(nx=40, ny=36, nz=32, dim=3), False),
+                                          (dict(nx=200, ny=180, dim=2), False)],
+                         ids=["72x66x68_3d", "40x36x32_3d", "200x180_2d"])
+def test_band_lines_default_where_they_pay(case, listed):
+    """The default lists lines on 3D grids with more than 4096 lines in a
+    direction, and solves every line elsewhere."""
+    lsmp = load_lsmp()
+    dim = case["dim"]
+    box = start_field(case["nx"], case["ny"], case.get("nz", 1), dim, False)
+    prm = lsmp.make_params(case["nx"], dim=dim, ny=case["ny"], nz=case.get("nz") if dim == 3 else None)
+    with lsmp.Solver(prm) as s:
+        s.upload_phi(box)
+        imass, _ = s.mass_vol()
+        s.upload_phi(box + 0.1 * prm.dx)
+        s.mass_fix(imass)
+        counts = s.band_line_counts()
+        say("default band lines", case, counts)
+        assert (min(counts[:dim]) > 0) if listed else counts == [-1, -1, -1]
