@@ -139,7 +139,8 @@ int lsmp_band_line_counts(lsmp_handle h, int counts[3]);
  * 32 x 8 x 8 cells that hold a cell |phi| < W; every other cell keeps its
  * value.  Results differ from the full-domain method away from the interface
  * (the far field is the constant +-W), so the FP64 drop-in leaves it off
- * (width_cells = 0, the default). */
+ * (width_cells = 0, the default).  With LSMP_SCHEME_UCCD the advection runs on
+ * every cell and only the reinitialisation is narrowed. */
 int lsmp_set_narrow_band(lsmp_handle h, double width_cells);
 
 /* L1 shape error sum |H(phi0/eps) - H(phi/eps)| / (nx ny nz)
