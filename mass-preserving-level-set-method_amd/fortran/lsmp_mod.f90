@@ -16,6 +16,7 @@ private
 integer(c_int), parameter, public :: LSMP_FP64 = 0, LSMP_FP32 = 1
 integer(c_int), parameter, public :: LSMP_VEL_ROTATION = 0, LSMP_VEL_VORTEX2D = 1, LSMP_VEL_LEVEQUE3D = 2
 integer(c_int), parameter, public :: LSMP_SCHEME_CRWENO = 0, LSMP_SCHEME_UCCD = 1
+integer(c_int), parameter, public :: LSMP_FALLBACK_RESTART = 0, LSMP_FALLBACK_CONTINUE = 1
 
 type, bind(C), public :: lsmp_params
     integer(c_int) :: nx, ny, nz, ghc, dim
@@ -163,6 +164,116 @@ interface
         type(c_ptr), value :: h
         integer(c_int), intent(out) :: reconstruct
     end function
+    ! device-time accounting per kernel (HIP events on the solver's stream)
+    integer(c_int) function c_lsmp_set_profiling(h, on) bind(C, name="lsmp_set_profiling")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int), value :: on
+    end function
+    integer(c_int) function c_lsmp_kernel_time(h, name, launches, ms) bind(C, name="lsmp_kernel_time")
+        import :: c_int, c_ptr, c_char, c_long_long, c_double
+        type(c_ptr), value :: h
+        character(kind=c_char), dimension(*), intent(in) :: name   ! NUL-terminated
+        integer(c_long_long), intent(out) :: launches
+        real(c_double), intent(out) :: ms
+    end function
+    ! connected components of phi > phi_core grown through phi_min < phi <= phi_core, and their masses
+    integer(c_int) function c_lsmp_component_masses(h, phi_core, phi_min, nquery, query_cells, query_comp, &
+                                                    maxcomp, comp_mass, ncomp) bind(C, name="lsmp_component_masses")
+        import :: c_int, c_ptr, c_double, c_long_long
+        type(c_ptr), value :: h
+        real(c_double), value :: phi_core, phi_min
+        integer(c_int), value :: nquery, maxcomp
+        integer(c_long_long), intent(in) :: query_cells(*)
+        integer(c_int), intent(out) :: query_comp(*)
+        real(c_double), intent(out) :: comp_mass(*)
+        integer(c_int), intent(out) :: ncomp
+    end function
+    ! learned reinitialisation: the network (BF16 MFMA, FP32 accumulation) and its use
+    integer(c_int) function c_lsmp_net_create(capacity, seed, net) bind(C, name="lsmp_net_create")
+        import :: c_int, c_ptr
+        integer(c_int), value :: capacity, seed
+        type(c_ptr), intent(out) :: net
+    end function
+    subroutine c_lsmp_net_destroy(net) bind(C, name="lsmp_net_destroy")
+        import :: c_ptr
+        type(c_ptr), value :: net
+    end subroutine
+    integer(c_int) function c_lsmp_net_param_count() bind(C, name="lsmp_net_param_count")
+        import :: c_int
+    end function
+    integer(c_int) function c_lsmp_net_set_params(net, params, n) bind(C, name="lsmp_net_set_params")
+        import :: c_int, c_ptr, c_float
+        type(c_ptr), value :: net
+        real(c_float), intent(in) :: params(*)
+        integer(c_int), value :: n
+    end function
+    integer(c_int) function c_lsmp_net_get_params(net, params, n) bind(C, name="lsmp_net_get_params")
+        import :: c_int, c_ptr, c_float
+        type(c_ptr), value :: net
+        real(c_float), intent(out) :: params(*)
+        integer(c_int), value :: n
+    end function
+    integer(c_int) function c_lsmp_net_dataset_size(net, n) bind(C, name="lsmp_net_dataset_size")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: net
+        integer(c_int), intent(out) :: n
+    end function
+    integer(c_int) function c_lsmp_net_collect(net, h, band_cells, stride, n_added) bind(C, name="lsmp_net_collect")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: net, h
+        real(c_double), value :: band_cells
+        integer(c_int), value :: stride
+        integer(c_int), intent(out) :: n_added
+    end function
+    integer(c_int) function c_lsmp_net_add_samples(net, x, y, n) bind(C, name="lsmp_net_add_samples")
+        import :: c_int, c_ptr, c_float
+        type(c_ptr), value :: net
+        real(c_float), intent(in) :: x(*), y(*)
+        integer(c_int), value :: n
+    end function
+    integer(c_int) function c_lsmp_net_clear_samples(net) bind(C, name="lsmp_net_clear_samples")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: net
+    end function
+    integer(c_int) function c_lsmp_net_apply(net, h, band_cells, n_band) bind(C, name="lsmp_net_apply")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: net, h
+        real(c_double), value :: band_cells
+        integer(c_int), intent(out) :: n_band
+    end function
+    integer(c_int) function c_lsmp_net_train(net, steps, batch, lr, seed, weight_c, loss) bind(C, name="lsmp_net_train")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: net
+        integer(c_int), value :: steps, batch, seed
+        real(c_double), value :: lr, weight_c
+        real(c_double), intent(out) :: loss
+    end function
+    integer(c_int) function c_lsmp_net_eval_grad(net, x, y, n, loss, grad) bind(C, name="lsmp_net_eval_grad")
+        import :: c_int, c_ptr, c_float, c_double
+        type(c_ptr), value :: net
+        real(c_float), intent(in) :: x(*), y(*)
+        integer(c_int), value :: n
+        real(c_double), intent(out) :: loss
+        real(c_float), intent(out) :: grad(*)
+    end function
+    ! in place of c_lsmp_reinit(gpu, 1, ...): network, sweeps plain pseudo-steps, residual check, fallback
+    integer(c_int) function c_lsmp_reinit_learned(h, net, band_cells, sweeps, residual_tol, residual_band, &
+                                                  fallback_mode, residual, fallback, n_pseudo) &
+                                                  bind(C, name="lsmp_reinit_learned")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h, net
+        real(c_double), value :: band_cells, residual_tol, residual_band
+        integer(c_int), value :: sweeps, fallback_mode
+        real(c_double), intent(out) :: residual
+        integer(c_int), intent(out) :: fallback, n_pseudo
+    end function
+    integer(c_int) function c_lsmp_reinit_residual(h, band_cells, residual) bind(C, name="lsmp_reinit_residual")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), value :: band_cells
+        real(c_double), intent(out) :: residual
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
         import :: c_size_t, c_ptr
         type(c_ptr), value :: s
@@ -174,6 +285,11 @@ public :: c_lsmp_set_velocity, c_lsmp_set_velocity_analytic, c_lsmp_rk3_advect, 
 public :: c_lsmp_mass_vol, c_lsmp_mass_fix, c_lsmp_interface_l1, c_lsmp_get_stats, c_lsmp_synchronize
 public :: c_lsmp_set_advection_scheme, c_lsmp_set_band_lines, c_lsmp_band_line_counts, c_lsmp_set_narrow_band
 public :: c_lsmp_set_marker, c_lsmp_get_marker, c_lsmp_marker_load, c_lsmp_marker_store, c_lsmp_combine_markers
+public :: c_lsmp_set_profiling, c_lsmp_kernel_time, c_lsmp_component_masses
+public :: c_lsmp_net_create, c_lsmp_net_destroy, c_lsmp_net_param_count, c_lsmp_net_set_params
+public :: c_lsmp_net_get_params, c_lsmp_net_dataset_size, c_lsmp_net_collect, c_lsmp_net_add_samples
+public :: c_lsmp_net_clear_samples, c_lsmp_net_apply, c_lsmp_net_train, c_lsmp_net_eval_grad
+public :: c_lsmp_reinit_learned, c_lsmp_reinit_residual
 public :: lsmp_check, lsmp_error_message
 
 contains
