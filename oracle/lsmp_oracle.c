@@ -1261,3 +1261,146 @@ void orc_mpls_step_markers(orc_state *s, int field, double t2s, int t2red)
     }
     orc_ls_mv(s);                                                           /* main.f90:34 */
 }
+
+/* ------------------------------------------------------------------------
+ * CLSVOF
+ * ---------------------------------------------------------------------- */
+void orc_velocity_faces(orc_state *s, int field, double time, double t2s)
+{
+    const double pi = acos(-1.0);
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz, ks = s->ks, ke = s->ke;
+    double *uf = s->f[ORC_UF], *vf = s->f[ORC_VF], *wf = s->f[ORC_WF];
+    const double ct = cos(pi * time / t2s), omega = 2.0 * pi / t2s;
+    for (int k = ks; k <= ke; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                double x = coord(s->p.xstart, s->p.xend, s->p.dx, nx, i);
+                double y = coord(s->p.ystart, s->p.yend, s->p.dy, ny, j);
+                double xx = 0.5 * (x + coord(s->p.xstart, s->p.xend, s->p.dx, nx, i + 1));
+                double yy = 0.5 * (y + coord(s->p.ystart, s->p.yend, s->p.dy, ny, j + 1));
+                if (field == 0) {
+                    /* Disk/src/setup/velocity.f90:88-89: the cell-centre formulas */
+                    AT(uf, i, j, k) = -omega * (y - 0.5);
+                    AT(vf, i, j, k) = omega * (x - 0.5);
+                } else if (field == 1) {
+                    /* Vortex2D/src/setup/velocity.f90:38-39 */
+                    AT(uf, i, j, k) = -sq(sin(pi * xx)) * sin(2.0 * pi * y) * ct;
+                    AT(vf, i, j, k) = sq(sin(pi * yy)) * sin(2.0 * pi * x) * ct;
+                } else {
+                    /* Vortex/src/setup/velocity.f90:31-33 */
+                    double z = coord(s->p.zstart, s->p.zend, s->p.dz, nz, k);
+                    double zz = 0.5 * (z + coord(s->p.zstart, s->p.zend, s->p.dz, nz, k + 1));
+                    AT(uf, i, j, k) = 2.0 * sq(sin(pi * xx)) * sin(2.0 * pi * y) * sin(2.0 * pi * z) * ct;
+                    AT(vf, i, j, k) = -sin(2.0 * pi * x) * sq(sin(pi * yy)) * sin(2.0 * pi * z) * ct;
+                    AT(wf, i, j, k) = -sin(2.0 * pi * x) * sin(2.0 * pi * y) * sq(sin(pi * zz)) * ct;
+                }
+            }
+    orc_bc(s, uf); orc_bc(s, vf); orc_bc(s, wf);
+}
+
+/* WLIC-THINC flux through the face between cells c and c + e along d
+ * (clsvof.f90:26-63 for x, :91-128 for y, :191-228 for z): the cell upwind
+ * of the face velocity uf gives vof; a full or empty cell is carried by the
+ * face velocity, otherwise the THINC profile's flux (slope alpha from the
+ * cells either side of it, ib = max(1, index - 1)) is weighted with the
+ * upwind one by |n_d| / (|n_x| + |n_y| + |n_z|). */
+static double wlic_flux(orc_state *s, int d, int i, int j, int k)
+{
+    const double eps = 1.0e-10, beta = 2.3, dt = s->p.dt;
+    const double *vof = s->f[ORC_VOF];
+    const double *vel = s->f[d == 0 ? ORC_UF : d == 1 ? ORC_VF : ORC_WF];
+    const double *N[3] = {s->f[ORC_NX], s->f[ORC_NY], s->f[ORC_NZ]};
+    const double dd = d == 0 ? s->p.dx : d == 1 ? s->p.dy : s->p.dz;
+    const double uf = AT(vel, i, j, k);
+    int c[3] = {i, j, k};
+    double isgn;
+    if (uf > 0.0) {
+        isgn = 1.0;
+    } else {
+        c[d] += 1;
+        isgn = 0.0;
+    }
+    const double v = AT(vof, c[0], c[1], c[2]);
+    if (v > 1.0 - eps || v < eps) return v * uf * dt;
+    int b[3] = {c[0], c[1], c[2]}, a[3] = {c[0], c[1], c[2]};
+    b[d] = c[d] - 1 > 1 ? c[d] - 1 : 1;   /* ib = max(1, ii - 1) */
+    a[d] = c[d] + 1;
+    const double alpha = AT(vof, b[0], b[1], b[2]) < AT(vof, a[0], a[1], a[2]) ? 1.0 : -1.0;
+    const double a1 = exp(beta * (2.0 * v - 1.0) / alpha);
+    const double a3 = exp(beta);
+    const double xc = 0.5 / beta * log((a3 * a3 - a1 * a3) / (a1 * a3 - 1.0));
+    const double a4 = cosh(beta * (isgn - uf * dt / dd - xc));
+    const double a5 = cosh(beta * (isgn - xc));
+    double w = fabs(AT(N[0], c[0], c[1], c[2])) + fabs(AT(N[1], c[0], c[1], c[2])) + fabs(AT(N[2], c[0], c[1], c[2]));
+    w = fabs(AT(N[d], c[0], c[1], c[2])) / w;
+    double f = 0.5 * (uf * dt - alpha * dd / beta * log(a4 / a5));
+    f = f * w + (1.0 - w) * v * uf * dt;
+    return f;
+}
+
+void orc_vof_wlic(orc_state *s)
+{
+    const int nx = s->p.nx, ny = s->p.ny, ks = s->ks, ke = s->ke;
+    const double dt = s->p.dt;
+    double *vof = s->f[ORC_VOF];
+    const double *old = s->f[ORC_VOF_OLD];
+    orc_surface_norms(s);   /* clsvof.f90:13 */
+    const int ndir = s->p.dim == 3 ? 3 : 2;
+    for (int d = 0; d < ndir; ++d) {
+        double *F = s->f[d == 0 ? ORC_SX1 : d == 1 ? ORC_SY1 : ORC_SZ1];   /* tdata%x/y/z%s1 */
+        const double *vel = s->f[d == 0 ? ORC_UF : d == 1 ? ORC_VF : ORC_WF];
+        const double dd = d == 0 ? s->p.dx : d == 1 ? s->p.dy : s->p.dz;
+        /* faces: index 0 .. n along d, interior along the others */
+        for (int k = (d == 2 ? ks - 1 : ks); k <= ke; ++k)
+            for (int j = (d == 1 ? 0 : 1); j <= ny; ++j)
+                for (int i = (d == 0 ? 0 : 1); i <= nx; ++i) AT(F, i, j, k) = wlic_flux(s, d, i, j, k);
+        for (int k = ks; k <= ke; ++k)
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    const int im = d == 0 ? i - 1 : i, jm = d == 1 ? j - 1 : j, km = d == 2 ? k - 1 : k;
+                    const double sflux = (AT(F, i, j, k) - AT(F, im, jm, km)) / dd;
+                    AT(vof, i, j, k) = AT(vof, i, j, k) - sflux +
+                                       AT(old, i, j, k) * dt * (AT(vel, i, j, k) - AT(vel, im, jm, km)) / dd;
+                    if (d == 2) AT(vof, i, j, k) = fmin(1.0, fmax(0.0, AT(vof, i, j, k)));   /* :256 */
+                }
+        orc_bc(s, vof);
+    }
+    if (ndir == 2) {
+        /* Vortex2D clsvof.f90:241-262: the z loop keeps only the clamp */
+        for (int k = ks; k <= ke; ++k)
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i) AT(vof, i, j, k) = fmin(1.0, fmax(0.0, AT(vof, i, j, k)));
+        orc_bc(s, vof);
+    }
+}
+
+void orc_clsvof_recon(orc_state *s)
+{
+    double *phi = s->f[ORC_PHI], *vof = s->f[ORC_VOF], *tmp = s->f[ORC_VOF_TMP];
+    for (long q = 0; q < s->nbox; ++q) {   /* :283-292, the whole box */
+        tmp[q] = phi[q];
+        phi[q] = 2.0 * vof[q] - 1.0;
+    }
+    orc_redis(s, 0, 5.0 * s->p.ls_wid);   /* :297 */
+    for (long q = 0; q < s->nbox; ++q)    /* :305-311 */
+        if (fabs(tmp[q]) > 1.5 * s->p.ls_wid) phi[q] = tmp[q];
+    orc_bc(s, phi);
+    orc_redis(s, 1, -1.0);                 /* :317 */
+}
+
+void orc_clsvof_step(orc_state *s, int field, double t2s, int t2red)
+{
+    if (s->time + s->p.dt > t2s) s->p.dt = t2s - s->time + 1.0e-10;   /* main.f90:19-21 */
+    s->time = s->time + s->p.dt;
+    s->iter = s->iter + 1;
+    if (field >= 0) {   /* main.f90:26 */
+        orc_velocity(s, field, s->time, t2s);
+        orc_velocity_faces(s, field, s->time, t2s);
+    }
+    memcpy(s->f[ORC_VOF_OLD], s->f[ORC_VOF], sizeof(double) * s->nbox);   /* p%switch, main.f90:27 */
+    orc_vof_wlic(s);                        /* interface_solver.f90:236 */
+    orc_rk3_advect(s);                      /* :238 */
+    s->last_redis_iters = 0;
+    if (s->iter % t2red == 0) orc_clsvof_recon(s);   /* :240-242 */
+    orc_ls_mv(s);                           /* main.f90:33 */
+}

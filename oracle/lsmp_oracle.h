@@ -44,6 +44,9 @@ enum {
     ORC_NX, ORC_NY, ORC_NZ, ORC_NXX, ORC_NYY, ORC_NZZ,
     ORC_VOF,
     ORC_MK1, ORC_MK2,    /* the two markers' level-set fields (marker(mid)%lsf) */
+    ORC_VOF_OLD,         /* vof%old: vof at the step's start (p%switch) */
+    ORC_UF, ORC_VF, ORC_WF,   /* face velocities vel%x/y/z%old (x face i + 1/2 at index i) */
+    ORC_VOF_TMP,         /* vof%tmp: clsvof_recon keeps the old phi there */
     ORC_NFIELDS
 };
 
@@ -119,6 +122,29 @@ int orc_combine_markers(orc_state *s);
  * "merged" set when they overlap.  Once merged, the one-field step.  ls_mv at
  * the end. */
 void orc_mpls_step_markers(orc_state *s, int field, double t2s, int t2red);
+
+/* ---- CLSVOF (coupled level set and VOF, the Vortex deck's default) ---- */
+/* face velocities vel%x/y/z (src/setup/velocity.f90 of each case): the
+ * LeVeque and vortex fields at the face midpoints xx = (x(i) + x(i+1)) / 2
+ * (Vortex/src/setup/velocity.f90:23-34; Vortex2D/src/setup/velocity.f90:
+ * 23-24, 38-39); the rotation's faces take the cell-centre formulas
+ * (Disk/src/setup/velocity.f90:81-92).  Writes UF, VF, WF incl. ghosts. */
+void orc_velocity_faces(orc_state *s, int field, double time, double t2s);
+/* vof_wlic_solver (Vortex/src/interface_solver/clsvof.f90:1-273): CCD
+ * normals, then per direction the WLIC-THINC face fluxes (beta = 2.3) and
+ * the update vof -= dF/dd - vof_old dt du/dd; the last direction clamps to
+ * [0, 1].  In 2D the z fluxes are dropped and the clamp stays
+ * (Vortex2D/src/interface_solver/clsvof.f90:183-256). */
+void orc_vof_wlic(orc_state *s);
+/* clsvof_recon (clsvof.f90:275-324): phi <- 2 vof - 1, reinitialised with
+ * btn = 0 to T = 5 ls_wid, the old phi kept where |phi_old| > 1.5 ls_wid,
+ * then the btn = 1 reinitialisation. */
+void orc_clsvof_recon(orc_state *s);
+/* one CLSVOF step of the merged branch (main.f90:17-34 with
+ * interface_solver.f90:234-244): dt clamp, t += dt, iter += 1, node and face
+ * velocity at t (field < 0 keeps them), vof_old <- vof, vof_wlic, rk3
+ * advection of phi, clsvof_recon if mod(iter, t2red) == 0, ls_mv. */
+void orc_clsvof_step(orc_state *s, int field, double t2s, int t2red);
 
 #ifdef __cplusplus
 }

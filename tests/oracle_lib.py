@@ -24,6 +24,7 @@ FIELDS = [
     "NX", "NY", "NZ", "NXX", "NYY", "NZZ",
     "VOF",
     "MK1", "MK2",
+    "VOF_OLD", "UF", "VF", "WF", "VOF_TMP",
 ]
 F = {n: i for i, n in enumerate(FIELDS)}
 
@@ -87,6 +88,10 @@ def lib():
         L.orc_mpls_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
         L.orc_combine_markers.argtypes = [C.c_void_p]
         L.orc_mpls_step_markers.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.orc_velocity_faces.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+        L.orc_vof_wlic.argtypes = [C.c_void_p]
+        L.orc_clsvof_recon.argtypes = [C.c_void_p]
+        L.orc_clsvof_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
         _lib = L
     return _lib
 
