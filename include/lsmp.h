@@ -197,6 +197,45 @@ int lsmp_marker_store(lsmp_handle h, int mid);
 int lsmp_combine_markers(lsmp_handle h, int *reconstruct);
 
 /* ------------------------------------------------------------------------
+ * CLSVOF (coupled level set and volume of fluid; clsvof, interface_solver.f90:
+ * 133-244, merged branch :234-244).  A step as the reference runs it, after
+ * the driver set the velocities at t^{n+1} (main.f90:23-27):
+ * lsmp_set_velocity[_analytic] and lsmp_set_face_velocity[_analytic],
+ * lsmp_store_vof_old, lsmp_vof_wlic(dt), lsmp_rk3_advect(dt), and every t2red
+ * steps lsmp_clsvof_recon.  The CLSVOF fields are allocated on the first of
+ * these calls.  Not available with the narrow band on (LSMP_E_STATE).
+ * ------------------------------------------------------------------------ */
+/* vof%now as a box like lsmp_upload_phi's (problem_init.f90 fills it);
+ * download: LSMP_E_STATE before an upload */
+int lsmp_upload_vof(lsmp_handle h, const double *vof_box);
+int lsmp_download_vof(lsmp_handle h, double *vof_box);
+/* vof%old <- vof%now, the VOF part of p%switch (Vortex/lib/data/tree.f90:383-392) */
+int lsmp_store_vof_old(lsmp_handle h);
+/* Face velocities vel%x/y/z%old: index i of the x box is the face i + 1/2
+ * (and likewise for y, z); the ghost faces are the zero-gradient fill of the
+ * first and last face, as the reference's bc sets them.  w is ignored in 2D. */
+int lsmp_set_face_velocity(lsmp_handle h, const double *u_box, const double *v_box, const double *w_box);
+/* The same from the prescribed fields at time t: LeVeque and vortex at the
+ * face midpoints (Vortex/src/setup/velocity.f90:20-33, Vortex2D/src/setup/
+ * velocity.f90:38-39), the rotation with the cell-centre formulas
+ * (Disk/src/setup/velocity.f90:89-93). */
+int lsmp_set_face_velocity_analytic(lsmp_handle h, int field, double t, double t2s);
+/* vof_wlic_solver (Vortex/src/interface_solver/clsvof.f90:1-273) with step
+ * dt: CCD normals of phi (surface_norms), then per direction the WLIC-THINC
+ * face fluxes and the update vof -= dF/dd - vof_old dt du/dd, the last
+ * direction clipping to [0, 1].  LSMP_E_STATE before lsmp_upload_vof,
+ * lsmp_store_vof_old or a face velocity.  Where the interface reaches a
+ * boundary cell, the normal along the pass's direction at the ghost cell
+ * beyond it is the boundary cell's (the reference takes the CCD derivative at
+ * the ghost point). */
+int lsmp_vof_wlic(lsmp_handle h, double dt);
+/* clsvof_recon (clsvof.f90:275-324): phi <- 2 vof - 1, level_set_rk3_redis(0,
+ * 5 ls_wid), the previous phi restored where its magnitude exceeds 1.5 ls_wid,
+ * level_set_rk3_redis(1).  n_pseudo0 / n_pseudo1 (may be null) receive the
+ * pseudo-steps of the two reinitialisations. */
+int lsmp_clsvof_recon(lsmp_handle h, int *n_pseudo0, int *n_pseudo1);
+
+/* ------------------------------------------------------------------------
  * Learned reinitialisation.  No counterpart in the reference: it replaces
  * most pseudo-steps of level_set_rk3_redis (level_set_rk3_redis.f90:25-74)
  * by one pass of a small network on the narrow band, followed by a few plain

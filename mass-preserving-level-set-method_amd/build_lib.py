@@ -24,7 +24,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-SOURCES = ["lsmp_kernels.hip", "lsmp_host.hip", "lsmp_learn.hip", "lsmp_metrics.hip"]
+SOURCES = ["lsmp_kernels.hip", "lsmp_host.hip", "lsmp_learn.hip", "lsmp_metrics.hip", "lsmp_clsvof.hip"]
 
 
 def _run(cmd, cwd=None):

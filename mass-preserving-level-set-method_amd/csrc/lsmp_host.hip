@@ -15,6 +15,7 @@
 #include "../../include/lsmp.h"
 #include "lsmp_internal.h"
 #include "lsmp_ctx.h"
+#include "lsmp_clsvof.h"
 
 using namespace lsmp;
 
@@ -600,11 +601,82 @@ template <typename T> int velocity_analytic(lsmp_ctx *c, int field, double t, do
     return LSMP_OK;
 }
 
+// ---------------------------------------------------------------- CLSVOF
+// device fields of the CLSVOF path, allocated (and zeroed) on first use
+int ensure_clsvof(lsmp_ctx *c)
+{
+    const size_t b = c->esz * c->ncell;
+    void **f[] = {&c->vof, &c->vofB, &c->vof_old, &c->vkeep, &c->uf, &c->vf, &c->wf, &c->nrm[0], &c->nrm[1], &c->nrm[2]};
+    for (void **p : f)
+        if (!*p) {
+            LSMP_HIP(hipMalloc(p, b));
+            LSMP_HIP(hipMemsetAsync(*p, 0, b, c->st));
+        }
+    if (!c->ftab) LSMP_HIP(hipMalloc((void **)&c->ftab, sizeof(double) * 3 * (size_t)(c->nx + c->ny + c->nz)));
+    LSMP_HIP(hipStreamSynchronize(c->st));
+    return LSMP_OK;
+}
+
+// face velocities vel%x/y/z of a prescribed field at time t (velocity.f90)
+template <typename T> int face_velocity_analytic(lsmp_ctx *c, int field, double t, double t2s)
+{
+    const double pi = std::acos(-1.0);
+    FaceVelArgs va;
+    va.nx = c->nx; va.ny = c->ny; va.nz = c->nz; va.field = field;
+    va.xstart = c->p.xstart; va.ystart = c->p.ystart; va.zstart = c->p.zstart;
+    va.dx = c->p.dx; va.dy = c->p.dy; va.dz = c->p.dz;
+    va.ct = std::cos(pi * t / t2s);
+    va.omega = 2.0 * pi / t2s;
+    KT(c, "face_velocity", (launch_face_velocity<T>(c->st, (T *)c->uf, (T *)c->vf, (T *)c->wf, va, c->p.xend,
+                                                     c->p.yend, c->p.zend, c->ftab)));
+    return LSMP_OK;
+}
+
+// vof_wlic_solver (clsvof.f90:1-273): CCD normals of phi (surface_norms), then
+// one WLIC-THINC pass per direction, vof -> vofB -> vof ...
+template <typename T> int vof_wlic(lsmp_ctx *c, double dt)
+{
+    const CcdArgs ca = ccd_args(c);
+    static const char *const nname[3] = {"norm_x", "norm_y", "norm_z"};
+    for (int d = 0; d < c->dim; ++d)
+        KT(c, nname[d], (launch_ccd_pass<T>(c->st, d, c->dim, false, (const T *)c->phi, (const T *)c->coef[d],
+                                            (T *)c->scr, (T *)c->nrm[d], nullptr, nullptr, nullptr, nullptr, c->nx,
+                                            c->ny, c->nz, ca, nullptr)));
+    const T *vel[3] = {(const T *)c->uf, (const T *)c->vf, (const T *)c->wf};
+    const double dd[3] = {c->p.dx, c->p.dy, c->p.dz};
+    const T *n2 = c->dim == 3 ? (const T *)c->nrm[2] : nullptr;
+    T *in = (T *)c->vof, *out = (T *)c->vofB;
+    for (int d = 0; d < c->dim; ++d) {
+        KT(c, "wlic", (launch_wlic_pass<T>(c->st, d, d == c->dim - 1, in, out, (const T *)c->vof_old, vel[d],
+                                           (const T *)c->nrm[0], (const T *)c->nrm[1], n2, c->nx, c->ny, c->nz, dt,
+                                           dd[d])));
+        std::swap(in, out);
+    }
+    c->vof = in;
+    c->vofB = out;
+    return LSMP_OK;
+}
+
+// clsvof_recon (clsvof.f90:275-324): phi <- 2 vof - 1, reinitialised with
+// btn = 0 to T = 5 ls_wid, the kept phi where |phi_kept| > 1.5 ls_wid, then
+// the btn = 1 reinitialisation
+template <typename T> int clsvof_recon(lsmp_ctx *c, int *n0, int *n1)
+{
+    KT(c, "recon_start", (launch_recon_start<T>(c->st, (T *)c->phi, (const T *)c->vof, (T *)c->vkeep, c->ncell)));
+    double err = 0.0;
+    int rc = reinit<T>(c, 0, 5.0 * c->p.ls_wid, 0, true, &err, n0);
+    if (rc) return rc;
+    KT(c, "recon_keep",
+       (launch_recon_keep<T>(c->st, (T *)c->phi, (const T *)c->vkeep, c->ncell, 1.5 * c->p.ls_wid)));
+    return reinit<T>(c, 1, -1.0, 0, true, &err, n1);
+}
+
 void free_all(lsmp_ctx *c)
 {
     void *bufs[] = {c->rctl, c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count,
-                    c->mk[0], c->mk[1], c->mflag, c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2], c->tiles};
+                    c->mk[0], c->mk[1], c->mflag, c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2], c->tiles,
+                    c->vof, c->vofB, c->vof_old, c->vkeep, c->uf, c->vf, c->wf, c->nrm[0], c->nrm[1], c->nrm[2], c->ftab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
@@ -972,6 +1044,79 @@ int lsmp_set_advection_scheme(lsmp_handle h, int scheme)
     }
     h->scheme = scheme;
     return LSMP_OK;
+}
+
+// ------------------------------------------------------------------ CLSVOF
+int lsmp_upload_vof(lsmp_handle h, const double *vof_box)
+{
+    if (!h || !vof_box) return LSMP_E_INVALID;
+    int rc = ensure_clsvof(h);
+    if (!rc) rc = upload_field(h, h->vof, vof_box);
+    if (!rc) h->vof_set = true;
+    return rc;
+}
+
+int lsmp_download_vof(lsmp_handle h, double *vof_box)
+{
+    if (!h || !vof_box) return LSMP_E_INVALID;
+    if (!h->vof_set) return LSMP_E_STATE;
+    return download_field(h, h->vof, vof_box);
+}
+
+int lsmp_store_vof_old(lsmp_handle h)
+{
+    if (!h) return LSMP_E_INVALID;
+    if (!h->vof_set) return LSMP_E_STATE;
+    LSMP_HIP(hipMemcpyAsync(h->vof_old, h->vof, h->ncell * h->esz, hipMemcpyDeviceToDevice, h->st));
+    h->vof_old_set = true;
+    return LSMP_OK;
+}
+
+int lsmp_set_face_velocity(lsmp_handle h, const double *u_box, const double *v_box, const double *w_box)
+{
+    if (!h || !u_box || !v_box || (h->dim == 3 && !w_box)) return LSMP_E_INVALID;
+    int rc = ensure_clsvof(h);
+    if (!rc) rc = upload_field(h, h->uf, u_box);
+    if (!rc) rc = upload_field(h, h->vf, v_box);
+    if (!rc && h->dim == 3) rc = upload_field(h, h->wf, w_box);
+    if (!rc) h->face_set = true;
+    return rc;
+}
+
+int lsmp_set_face_velocity_analytic(lsmp_handle h, int field, double t, double t2s)
+{
+    if (!h || field < 0 || field > 2 || !(t2s > 0)) return LSMP_E_INVALID;
+    if (field == LSMP_VEL_LEVEQUE3D && h->dim != 3) return LSMP_E_INVALID;
+    int rc = ensure_clsvof(h);
+    if (!rc) rc = h->fp32 ? face_velocity_analytic<float>(h, field, t, t2s) : face_velocity_analytic<double>(h, field, t, t2s);
+    if (!rc) h->face_set = true;
+    return rc;
+}
+
+int lsmp_vof_wlic(lsmp_handle h, double dt)
+{
+    if (!h || !(dt > 0)) return LSMP_E_INVALID;
+    if (!h->vof_set || !h->vof_old_set || !h->face_set || h->nb_cells > 0.0) return LSMP_E_STATE;
+    Timer tm(h, 0, nullptr, true);
+    int rc = h->fp32 ? vof_wlic<float>(h, dt) : vof_wlic<double>(h, dt);
+    tm.stop();
+    resolve(h, false);
+    return rc;
+}
+
+int lsmp_clsvof_recon(lsmp_handle h, int *n_pseudo0, int *n_pseudo1)
+{
+    if (!h) return LSMP_E_INVALID;
+    if (!h->vof_set || h->nb_cells > 0.0) return LSMP_E_STATE;
+    Timer tm(h, 1, nullptr, true);
+    int a = 0, b = 0;
+    int rc = h->fp32 ? clsvof_recon<float>(h, &a, &b) : clsvof_recon<double>(h, &a, &b);
+    tm.stop();
+    h->stats.n_reinit++;
+    resolve(h, false);
+    if (n_pseudo0) *n_pseudo0 = a;
+    if (n_pseudo1) *n_pseudo1 = b;
+    return rc;
 }
 
 }  // extern "C"

@@ -141,6 +141,20 @@ template <typename T>
 void launch_uccd_pass(hipStream_t st, int dir, int mode, int stage, const T *phi, T *phi_out, const T *vel, T *R,
                       T *L1, T *L2, const T *coef, T *ck, int nx, int ny, int nz, T dx, T dt);
 template <typename TO, typename TI> void launch_convert(hipStream_t st, TO *out, const TI *in, size_t n);
+
+// CLSVOF (lsmp_clsvof.hip).  Face velocities of a prescribed field at time
+// va.ct / va.omega into uf, vf, wf; tab holds 3 (nx + ny + nz) doubles.
+struct FaceVelArgs;
+template <typename T>
+void launch_face_velocity(hipStream_t st, T *uf, T *vf, T *wf, const FaceVelArgs &va, double xend, double yend,
+                          double zend, double *tab);
+// one direction's WLIC-THINC pass, in -> out (n2 null in 2D; clamp on the
+// last direction)
+template <typename T>
+void launch_wlic_pass(hipStream_t st, int dir, bool clamp, const T *in, T *out, const T *old, const T *vel,
+                      const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd);
+template <typename T> void launch_recon_start(hipStream_t st, T *phi, const T *vof, T *keep, size_t n);
+template <typename T> void launch_recon_keep(hipStream_t st, T *phi, const T *keep, size_t n, double w);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
 void launch_redis_ctl(hipStream_t st, unsigned long long *ctl, int op);
 

@@ -274,6 +274,43 @@ interface
         real(c_double), value :: band_cells
         real(c_double), intent(out) :: residual
     end function
+    ! CLSVOF (clsvof, interface_solver.f90:133-244)
+    integer(c_int) function c_lsmp_upload_vof(h, vof) bind(C, name="lsmp_upload_vof")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(in) :: vof(*)
+    end function
+    integer(c_int) function c_lsmp_download_vof(h, vof) bind(C, name="lsmp_download_vof")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(out) :: vof(*)
+    end function
+    integer(c_int) function c_lsmp_store_vof_old(h) bind(C, name="lsmp_store_vof_old")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+    end function
+    integer(c_int) function c_lsmp_set_face_velocity(h, u, v, w) bind(C, name="lsmp_set_face_velocity")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(in) :: u(*), v(*), w(*)
+    end function
+    integer(c_int) function c_lsmp_set_face_velocity_analytic(h, field, t, t2s) &
+                                                              bind(C, name="lsmp_set_face_velocity_analytic")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int), value :: field
+        real(c_double), value :: t, t2s
+    end function
+    integer(c_int) function c_lsmp_vof_wlic(h, dt) bind(C, name="lsmp_vof_wlic")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), value :: dt
+    end function
+    integer(c_int) function c_lsmp_clsvof_recon(h, n_pseudo0, n_pseudo1) bind(C, name="lsmp_clsvof_recon")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int), intent(out) :: n_pseudo0, n_pseudo1
+    end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
         import :: c_size_t, c_ptr
         type(c_ptr), value :: s
@@ -290,6 +327,8 @@ public :: c_lsmp_net_create, c_lsmp_net_destroy, c_lsmp_net_param_count, c_lsmp_
 public :: c_lsmp_net_get_params, c_lsmp_net_dataset_size, c_lsmp_net_collect, c_lsmp_net_add_samples
 public :: c_lsmp_net_clear_samples, c_lsmp_net_apply, c_lsmp_net_train, c_lsmp_net_eval_grad
 public :: c_lsmp_reinit_learned, c_lsmp_reinit_residual
+public :: c_lsmp_upload_vof, c_lsmp_download_vof, c_lsmp_store_vof_old, c_lsmp_set_face_velocity
+public :: c_lsmp_set_face_velocity_analytic, c_lsmp_vof_wlic, c_lsmp_clsvof_recon
 public :: lsmp_check, lsmp_error_message
 
 contains

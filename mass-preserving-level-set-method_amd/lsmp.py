@@ -38,6 +38,8 @@ EXPORTS = [
     "lsmp_net_get_params", "lsmp_net_dataset_size", "lsmp_net_collect", "lsmp_net_train",
     "lsmp_net_eval_grad", "lsmp_reinit_learned", "lsmp_reinit_residual", "lsmp_net_add_samples",
     "lsmp_net_clear_samples", "lsmp_net_apply",
+    "lsmp_upload_vof", "lsmp_download_vof", "lsmp_store_vof_old", "lsmp_set_face_velocity",
+    "lsmp_set_face_velocity_analytic", "lsmp_vof_wlic", "lsmp_clsvof_recon",
 ]
 NO_STATUS = ("lsmp_finalize", "lsmp_error_string", "lsmp_net_destroy")
 
@@ -122,6 +124,13 @@ def load_library(path=LIB_PATH):
     L.lsmp_net_add_samples.argtypes = [H, _FP, _FP, C.c_int]
     L.lsmp_net_clear_samples.argtypes = [H]
     L.lsmp_net_apply.argtypes = [H, H, C.c_double, _IP]
+    L.lsmp_upload_vof.argtypes = [H, _DP]
+    L.lsmp_download_vof.argtypes = [H, _DP]
+    L.lsmp_store_vof_old.argtypes = [H]
+    L.lsmp_set_face_velocity.argtypes = [H, _DP, _DP, _DP]
+    L.lsmp_set_face_velocity_analytic.argtypes = [H, C.c_int, C.c_double, C.c_double]
+    L.lsmp_vof_wlic.argtypes = [H, C.c_double]
+    L.lsmp_clsvof_recon.argtypes = [H, _IP, _IP]
     for name in EXPORTS:
         if name not in NO_STATUS:
             getattr(L, name).restype = C.c_int
@@ -331,6 +340,55 @@ class Solver:
                                                     fallback=fallback)
         _, nm = self.mass_fix(imass)
         return npseudo, nm
+
+    # -- CLSVOF (clsvof, interface_solver.f90:133-244)
+    def upload_vof(self, box):
+        """vof%now from a host box."""
+        _check(self.lib.lsmp_upload_vof(self.h, _box_ptr(np.ascontiguousarray(box, np.float64))))
+
+    def download_vof(self):
+        out = np.empty(self.shape, np.float64)
+        _check(self.lib.lsmp_download_vof(self.h, _box_ptr(out)))
+        return out
+
+    def store_vof_old(self):
+        """vof%old <- vof%now (the VOF part of p%switch, tree.f90:383-392)."""
+        _check(self.lib.lsmp_store_vof_old(self.h))
+
+    def set_face_velocity(self, u, v, w=None):
+        """vel%x/y/z%old: index i of a box holds the face i + 1/2."""
+        u = np.ascontiguousarray(u, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        w = np.zeros_like(u) if w is None else np.ascontiguousarray(w, np.float64)
+        _check(self.lib.lsmp_set_face_velocity(self.h, _box_ptr(u), _box_ptr(v), _box_ptr(w)))
+
+    def set_face_velocity_analytic(self, field, t, t2s):
+        _check(self.lib.lsmp_set_face_velocity_analytic(self.h, field, t, t2s))
+
+    def vof_wlic(self, dt=None):
+        """vof_wlic_solver (clsvof.f90:1-273)."""
+        _check(self.lib.lsmp_vof_wlic(self.h, self.params.dt if dt is None else dt))
+
+    def clsvof_recon(self):
+        """clsvof_recon (clsvof.f90:275-324).  Returns the pseudo-steps of its
+        two reinitialisations (btn = 0, then btn = 1)."""
+        a, b = C.c_int(), C.c_int()
+        _check(self.lib.lsmp_clsvof_recon(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def clsvof_step(self, t, t2s, field, t2red=1, iteration=1, dt=None):
+        """One CLSVOF step as the merged branch of clsvof runs it
+        (interface_solver.f90:234-244) after the driver set the velocities at
+        t^{n+1} and switched vof%old (main.f90:23-27).  Returns the pseudo-steps
+        of the reconstruction's reinitialisations ((0, 0) without one)."""
+        self.set_velocity_analytic(field, t, t2s)
+        self.set_face_velocity_analytic(field, t, t2s)
+        self.store_vof_old()
+        self.vof_wlic(dt)
+        self.rk3_advect(dt)
+        if iteration % t2red == 0:
+            return self.clsvof_recon()
+        return 0, 0
 
     # -- two-marker MPLS branch (interface_solver.f90:34-121)
     def set_marker(self, mid, box):

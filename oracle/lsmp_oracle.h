@@ -128,7 +128,7 @@ void orc_mpls_step_markers(orc_state *s, int field, double t2s, int t2red);
  * LeVeque and vortex fields at the face midpoints xx = (x(i) + x(i+1)) / 2
  * (Vortex/src/setup/velocity.f90:23-34; Vortex2D/src/setup/velocity.f90:
  * 23-24, 38-39); the rotation's faces take the cell-centre formulas
- * (Disk/src/setup/velocity.f90:81-92).  Writes UF, VF, WF incl. ghosts. */
+ * (Disk/src/setup/velocity.f90:89-93).  Writes UF, VF, WF incl. ghosts. */
 void orc_velocity_faces(orc_state *s, int field, double time, double t2s);
 /* vof_wlic_solver (Vortex/src/interface_solver/clsvof.f90:1-273): CCD
  * normals, then per direction the WLIC-THINC face fluxes (beta = 2.3) and
