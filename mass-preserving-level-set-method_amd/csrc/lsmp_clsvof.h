@@ -39,7 +39,18 @@ LSMP_HD WlicLine<T> wlic_line(const T *vof, const T *vel, const T *n0, const T *
                               int ny, int nz, int *p)
 {
     const size_t nxy = (size_t)nx * ny;
-    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / nxy);
+    int i, j, k;
+    if (nxy * (size_t)nz <= 0xffffffffull) {
+        // 32-bit quotients: a 64-bit division is a long integer sequence on the GPU
+        const unsigned u = (unsigned)q, r = u / (unsigned)nx;
+        i = (int)(u - r * (unsigned)nx);
+        k = (int)(r / (unsigned)ny);
+        j = (int)(r - (unsigned)k * (unsigned)ny);
+    } else {
+        i = (int)(q % nx);
+        j = (int)((q / nx) % ny);
+        k = (int)(q / nxy);
+    }
     WlicLine<T> L;
     L.vof = vof;
     L.vel = vel;
