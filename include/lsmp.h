@@ -211,6 +211,14 @@ int lsmp_upload_vof(lsmp_handle h, const double *vof_box);
 int lsmp_download_vof(lsmp_handle h, double *vof_box);
 /* vof%old <- vof%now, the VOF part of p%switch (Vortex/lib/data/tree.f90:383-392) */
 int lsmp_store_vof_old(lsmp_handle h);
+/* vof%tmp2 <- vof%now, the initial VOF of the shape error (problem_init's loc%init) */
+int lsmp_store_vof0(lsmp_handle h);
+/* VOF half of ls_mv (ns_solver/others.f90:215-247): volume sum vof dv and
+ * mass sum rho(vof) vof dv (p%glb%volv, p%glb%massv) */
+int lsmp_vof_mass_vol(lsmp_handle h, double *mass, double *vol);
+/* VOF part of calculate_interface_loss (output.f90:178-204, "VOF Int"):
+ * sum |vof0 - vof| / (nx ny nz); LSMP_E_STATE before lsmp_store_vof0 */
+int lsmp_vof_interface_l1(lsmp_handle h, double *l1);
 /* Face velocities vel%x/y/z%old: index i of the x box is the face i + 1/2
  * (and likewise for y, z); the ghost faces are the zero-gradient fill of the
  * first and last face, as the reference's bc sets them.  w is ignored in 2D. */

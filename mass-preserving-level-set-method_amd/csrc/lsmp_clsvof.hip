@@ -83,7 +83,41 @@ __global__ void k_recon_keep(T *__restrict__ phi, const T *__restrict__ keep, si
     if (fabs((double)k) > w) phi[q] = k;
 }
 
+// Block partial sums over the VOF: l1 == 0, the VOF half of ls_mv
+// (ns_solver/others.f90:215-247): sum vof dv and sum rho(vof) vof dv; l1 != 0,
+// the VOF part of calculate_interface_loss (output.f90:178-204): sum |vof0 - vof|.
+template <typename T>
+__global__ void k_vof_sums(const T *__restrict__ vof, const T *__restrict__ vof0, size_t n, double dv, double r12,
+                           int inverse, int l1, double *__restrict__ partial)
+{
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const double v = (double)vof[q];
+        if (l1) {
+            acc[0] += fabs((double)vof0[q] - v);
+        } else {
+            acc[0] += v * dv;
+            const double rho = inverse ? r12 * v + (1.0 - v) : v + r12 * (1.0 - v);
+            acc[1] += rho * v * dv;
+        }
+    }
+    block_sum<3>(acc);
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x + 0] = acc[0];
+        partial[3 * blockIdx.x + 1] = acc[1];
+        partial[3 * blockIdx.x + 2] = 0.0;
+    }
+}
+
 }  // namespace
+
+template <typename T>
+void launch_vof_sums(hipStream_t st, const T *vof, const T *vof0, size_t n, double dv, double r12, int inverse,
+                     int l1, double *partial, int nblocks)
+{
+    hipLaunchKernelGGL((k_vof_sums<T>), dim3(nblocks), dim3(256), 0, st, vof, vof0, n, dv, r12, inverse, l1,
+                       partial);
+}
 
 template <typename T>
 void launch_face_velocity(hipStream_t st, T *uf, T *vf, T *wf, const FaceVelArgs &va, double xend, double yend,
@@ -134,7 +168,9 @@ template <typename T> void launch_recon_keep(hipStream_t st, T *phi, const T *ke
     template void launch_wlic_pass<T>(hipStream_t, int, bool, const T *, T *, const T *, const T *, const T *, \
                                       const T *, const T *, int, int, int, double, double);                    \
     template void launch_recon_start<T>(hipStream_t, T *, const T *, T *, size_t);                             \
-    template void launch_recon_keep<T>(hipStream_t, T *, const T *, size_t, double);
+    template void launch_recon_keep<T>(hipStream_t, T *, const T *, size_t, double);                          \
+    template void launch_vof_sums<T>(hipStream_t, const T *, const T *, size_t, double, double, int, int,      \
+                                     double *, int);
 
 LSMP_CLSVOF_INST(double)
 LSMP_CLSVOF_INST(float)

@@ -50,13 +50,14 @@ struct lsmp_ctx {
     unsigned *mflag = nullptr;          // "inside both markers" flag of the merge
     bool mk_set[2] = {false, false};
     // CLSVOF, allocated on first use: vof%now (and its spare for the passes),
-    // vof%old, the kept phi of the reconstruction, face velocities vel%x/y/z,
+    // vof%old, the kept phi of the reconstruction, vof%tmp2 (the initial VOF of
+    // the shape error), face velocities vel%x/y/z,
     // CCD normals, the face velocity tables
-    void *vof = nullptr, *vofB = nullptr, *vof_old = nullptr, *vkeep = nullptr;
+    void *vof = nullptr, *vofB = nullptr, *vof_old = nullptr, *vkeep = nullptr, *vof0 = nullptr;
     void *uf = nullptr, *vf = nullptr, *wf = nullptr;
     void *nrm[3] = {nullptr, nullptr, nullptr};
     double *ftab = nullptr;
-    bool vof_set = false, vof_old_set = false, face_set = false;
+    bool vof_set = false, vof_old_set = false, vof0_set = false, face_set = false;
     bool vel_set = false, phi0_set = false;
     lsmp_stats stats{};
     bool profiling = false;

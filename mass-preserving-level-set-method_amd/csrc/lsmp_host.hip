@@ -606,7 +606,8 @@ template <typename T> int velocity_analytic(lsmp_ctx *c, int field, double t, do
 int ensure_clsvof(lsmp_ctx *c)
 {
     const size_t b = c->esz * c->ncell;
-    void **f[] = {&c->vof, &c->vofB, &c->vof_old, &c->vkeep, &c->uf, &c->vf, &c->wf, &c->nrm[0], &c->nrm[1], &c->nrm[2]};
+    void **f[] = {&c->vof,  &c->vofB, &c->vof_old, &c->vkeep,  &c->vof0,  &c->uf,
+                  &c->vf,   &c->wf,   &c->nrm[0],  &c->nrm[1], &c->nrm[2]};
     for (void **p : f)
         if (!*p) {
             LSMP_HIP(hipMalloc(p, b));
@@ -671,12 +672,32 @@ template <typename T> int clsvof_recon(lsmp_ctx *c, int *n0, int *n1)
     return reinit<T>(c, 1, -1.0, 0, true, &err, n1);
 }
 
+// the VOF half of ls_mv (l1 == 0: *a = volume, *b = mass) or the VOF shape
+// error of calculate_interface_loss (l1 != 0: *a = sum |vof0 - vof| / cells)
+template <typename T> int vof_sums(lsmp_ctx *c, int l1, double *a, double *b)
+{
+    const int nb = reduce_blocks(c);
+    KT(c, l1 ? "vof_l1" : "vof_mass_vol",
+       (launch_vof_sums<T>(c->st, (const T *)c->vof, (const T *)c->vof0, c->ncell, c->p.dx * c->p.dy * c->p.dz,
+                           c->p.rho_12, c->p.inverse, l1, c->partial, nb)));
+    double r[3];
+    int rc = reduce_partials(c, nb, r);
+    if (rc) return rc;
+    if (l1) {
+        *a = r[0] / (double)c->nx / (double)c->ny / (double)c->nz;
+    } else {
+        *a = r[0];
+        *b = r[1];
+    }
+    return LSMP_OK;
+}
+
 void free_all(lsmp_ctx *c)
 {
     void *bufs[] = {c->rctl, c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count,
                     c->mk[0], c->mk[1], c->mflag, c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2], c->tiles,
-                    c->vof, c->vofB, c->vof_old, c->vkeep, c->uf, c->vf, c->wf, c->nrm[0], c->nrm[1], c->nrm[2], c->ftab};
+                    c->vof, c->vofB, c->vof_old, c->vkeep, c->vof0, c->uf, c->vf, c->wf, c->nrm[0], c->nrm[1], c->nrm[2], c->ftab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
@@ -1070,6 +1091,30 @@ int lsmp_store_vof_old(lsmp_handle h)
     LSMP_HIP(hipMemcpyAsync(h->vof_old, h->vof, h->ncell * h->esz, hipMemcpyDeviceToDevice, h->st));
     h->vof_old_set = true;
     return LSMP_OK;
+}
+
+int lsmp_store_vof0(lsmp_handle h)
+{
+    if (!h) return LSMP_E_INVALID;
+    if (!h->vof_set) return LSMP_E_STATE;
+    LSMP_HIP(hipMemcpyAsync(h->vof0, h->vof, h->ncell * h->esz, hipMemcpyDeviceToDevice, h->st));
+    LSMP_HIP(hipStreamSynchronize(h->st));
+    h->vof0_set = true;
+    return LSMP_OK;
+}
+
+int lsmp_vof_mass_vol(lsmp_handle h, double *mass, double *vol)
+{
+    if (!h || !mass || !vol) return LSMP_E_INVALID;
+    if (!h->vof_set) return LSMP_E_STATE;
+    return h->fp32 ? vof_sums<float>(h, 0, vol, mass) : vof_sums<double>(h, 0, vol, mass);
+}
+
+int lsmp_vof_interface_l1(lsmp_handle h, double *l1)
+{
+    if (!h || !l1) return LSMP_E_INVALID;
+    if (!h->vof_set || !h->vof0_set) return LSMP_E_STATE;
+    return h->fp32 ? vof_sums<float>(h, 1, l1, nullptr) : vof_sums<double>(h, 1, l1, nullptr);
 }
 
 int lsmp_set_face_velocity(lsmp_handle h, const double *u_box, const double *v_box, const double *w_box)

@@ -155,6 +155,11 @@ void launch_wlic_pass(hipStream_t st, int dir, bool clamp, const T *in, T *out, 
                       const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd);
 template <typename T> void launch_recon_start(hipStream_t st, T *phi, const T *vof, T *keep, size_t n);
 template <typename T> void launch_recon_keep(hipStream_t st, T *phi, const T *keep, size_t n, double w);
+// block partial sums (3 per block) over the VOF: volume and mass (l1 == 0) or
+// sum |vof0 - vof| (l1 != 0)
+template <typename T>
+void launch_vof_sums(hipStream_t st, const T *vof, const T *vof0, size_t n, double dv, double r12, int inverse,
+                     int l1, double *partial, int nblocks);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
 void launch_redis_ctl(hipStream_t st, unsigned long long *ctl, int op);
 

@@ -15,8 +15,8 @@
 ! level_set_rk3_source variant, level_set_rk3_solver.f90:96-102)
 !
 ! The deck has the reference's format (odd lines are comments).  Like the
-! reference's shipped cases it needs threads grid 1 1 1 and ghc = 3; method 1
-! (LS) and 2 (MPLS) are supported, method 3 (CLSVOF) is not.
+! reference's shipped cases it needs threads grid 1 1 1 and ghc = 3; methods 1
+! (LS), 2 (MPLS) and 3 (CLSVOF, its merged branch) are supported.
 program lsmp_driver
 use iso_c_binding
 use lsmp
@@ -35,11 +35,13 @@ integer :: btn_sf, btn_g
 integer :: nx, ny, nz, kb0, kb1
 real(8) :: dx, dy, dz, time, mass, vol, imass, ivol, red_error, l1
 real(8) :: loss_mass_avg, loss_mass_max, loss_vol_avg, loss_vol_max
-real(8), allocatable :: phi(:,:,:), xc(:), yc(:), zc(:)
+real(8), allocatable :: phi(:,:,:), vofb(:,:,:), xc(:), yc(:), zc(:)
+real(8) :: massv, volv, imassv, ivolv, l1v
+real(8) :: loss_mass_avgv, loss_mass_maxv, loss_vol_avgv, loss_vol_maxv
 type(lsmp_params) :: prm
 type(lsmp_stats) :: st
 type(c_ptr) :: h
-integer(c_int) :: npseudo, nmass
+integer(c_int) :: npseudo, nmass, nrec0, nrec1
 integer :: fmass, fvol, fen, fmom, fpos
 
 ! ------------------------------------------------------------ arguments
@@ -108,8 +110,8 @@ read(526,*); read(526,*) wbc(1), wbc(2)
 read(526,*); read(526,*) per(1), per(2), per(3)
 close(526)
 
-if (method /= 1 .and. method /= 2) then
-    write(*,*) "Error >> method ", method, " is not provided by this driver (1 LS, 2 MPLS)"
+if (method < 1 .or. method > 3) then
+    write(*,*) "Error >> method ", method, " is not provided by this driver (1 LS, 2 MPLS, 3 CLSVOF)"
     stop 1
 endif
 if (gx /= 1 .or. gy /= 1 .or. gz /= 1) then
@@ -169,6 +171,7 @@ call lsmp_check(c_lsmp_set_advection_scheme(h, scheme), "lsmp_set_advection_sche
 
 ! ------------------------------------------------------------ initial condition (problem_init.f90:32-103)
 allocate(phi(1-ghc:nx+ghc, 1-ghc:ny+ghc, kb0:kb1))
+if (method == 3) allocate(vofb(1-ghc:nx+ghc, 1-ghc:ny+ghc, kb0:kb1))
 call initial_phi()
 call lsmp_check(c_lsmp_upload_phi(h, phi), "lsmp_upload_phi")
 if (trim(casename) == "disk") then
@@ -177,6 +180,17 @@ if (trim(casename) == "disk") then
 endif
 call lsmp_check(c_lsmp_store_phi0(h), "lsmp_store_phi0")     ! phi%tmp2 (roots.f90:150-166)
 call lsmp_check(c_lsmp_mass_vol(h, imass, ivol), "lsmp_mass_vol")   ! problem_init.f90:140-142
+massv = 0.0d0; volv = 0.0d0; imassv = 0.0d0; ivolv = 0.0d0; l1v = 0.0d0
+if (method == 3) then
+    if (trim(casename) == "disk") then
+        call bc_box(vofb)          ! the disk's VOF was sampled with its phi
+    else
+        call initial_vof()
+    endif
+    call lsmp_check(c_lsmp_upload_vof(h, vofb), "lsmp_upload_vof")
+    call lsmp_check(c_lsmp_store_vof0(h), "lsmp_store_vof0")           ! vof%tmp2
+    call lsmp_check(c_lsmp_vof_mass_vol(h, imassv, ivolv), "lsmp_vof_mass_vol")
+endif
 
 ! ------------------------------------------------------------ outputs (tree.f90:199-217)
 call execute_command_line("mkdir -p out", exitstat=iostat)
@@ -197,6 +211,8 @@ time = 0.0d0
 iter = 0
 loss_mass_avg = 0.0d0; loss_mass_max = 0.0d0
 loss_vol_avg = 0.0d0; loss_vol_max = 0.0d0
+loss_mass_avgv = 0.0d0; loss_mass_maxv = 0.0d0
+loss_vol_avgv = 0.0d0; loss_vol_maxv = 0.0d0
 call lsmp_check(c_lsmp_synchronize(h), "lsmp_synchronize")
 call system_clock(clk0, clkrate)
 do
@@ -205,21 +221,44 @@ do
     iter = iter + 1
     call lsmp_check(c_lsmp_set_velocity_analytic(h, int(field, c_int), time, t2s), "velocity")
 
-    ! interface_solver -> classical_ls / mpls merged branch (interface_solver.f90:15-23, 123-129)
-    call lsmp_check(c_lsmp_rk3_advect(h, dt), "lsmp_rk3_advect")
-    if (mod(iter, t2red) == 0) then
-        call lsmp_check(c_lsmp_reinit(h, 1_c_int, -1.0d0, red_error, npseudo), "lsmp_reinit")
-    endif
-    if (method == 2) then
-        call lsmp_check(c_lsmp_mass_fix(h, imass, mass, nmass), "lsmp_mass_fix")
+    if (method == 3) then
+        ! clsvof merged branch (interface_solver.f90:234-244) after p%switch
+        call lsmp_check(c_lsmp_set_face_velocity_analytic(h, int(field, c_int), time, t2s), "face velocity")
+        call lsmp_check(c_lsmp_store_vof_old(h), "lsmp_store_vof_old")
+        call lsmp_check(c_lsmp_vof_wlic(h, dt), "lsmp_vof_wlic")
+        call lsmp_check(c_lsmp_rk3_advect(h, dt), "lsmp_rk3_advect")
+        if (mod(iter, t2red) == 0) then
+            call lsmp_check(c_lsmp_clsvof_recon(h, nrec0, nrec1), "lsmp_clsvof_recon")
+        endif
+    else
+        ! interface_solver -> classical_ls / mpls merged branch (interface_solver.f90:15-23, 123-129)
+        call lsmp_check(c_lsmp_rk3_advect(h, dt), "lsmp_rk3_advect")
+        if (mod(iter, t2red) == 0) then
+            call lsmp_check(c_lsmp_reinit(h, 1_c_int, -1.0d0, red_error, npseudo), "lsmp_reinit")
+        endif
+        if (method == 2) then
+            call lsmp_check(c_lsmp_mass_fix(h, imass, mass, nmass), "lsmp_mass_fix")
+        endif
     endif
     call lsmp_check(c_lsmp_mass_vol(h, mass, vol), "lsmp_mass_vol")      ! main.f90:35
+    if (method == 3) call lsmp_check(c_lsmp_vof_mass_vol(h, massv, volv), "lsmp_vof_mass_vol")
 
-    ! output (output.f90:7-31); the VOF, marker and NS columns of the
-    ! reference are not transported on this path and are written as zero
+    ! output (output.f90:7-31); the marker and NS columns of the reference
+    ! are not transported on this path and are written as zero, and so is
+    ! the VOF column unless the VOF is transported (CLSVOF)
     if (iter == 1) ivol = vol
-    write(fmass,'(7ES15.4)') time, 100.0d0*(mass-imass)/imass, 0.0d0, 0.0d0, 0.0d0
-    write(fvol,'(7ES15.4)') time, 100.0d0*(vol-ivol)/ivol, 0.0d0, 0.0d0, 0.0d0
+    if (iter == 1) ivolv = volv
+    if (method == 3) then
+        write(fmass,'(7ES15.4)') time, 100.0d0*(mass-imass)/imass, 100.0d0*(massv-imassv)/imassv, 0.0d0, 0.0d0
+        write(fvol,'(7ES15.4)') time, 100.0d0*(vol-ivol)/ivol, 100.0d0*(volv-ivolv)/ivolv, 0.0d0, 0.0d0
+        loss_mass_avgv = loss_mass_avgv + abs(imassv-massv)/imassv
+        loss_vol_avgv = loss_vol_avgv + abs(ivolv-volv)/ivolv
+        loss_mass_maxv = max(loss_mass_maxv, abs(imassv-massv)/imassv)
+        loss_vol_maxv = max(loss_vol_maxv, abs(ivolv-volv)/ivolv)
+    else
+        write(fmass,'(7ES15.4)') time, 100.0d0*(mass-imass)/imass, 0.0d0, 0.0d0, 0.0d0
+        write(fvol,'(7ES15.4)') time, 100.0d0*(vol-ivol)/ivol, 0.0d0, 0.0d0, 0.0d0
+    endif
     loss_mass_avg = loss_mass_avg + abs(imass-mass)/imass
     loss_vol_avg = loss_vol_avg + abs(ivol-vol)/ivol
     loss_mass_max = max(loss_mass_max, abs(imass-mass)/imass)
@@ -246,19 +285,29 @@ call system_clock(clk1)
 
 ! ------------------------------------------------------------ calculate_interface_loss (output.f90:169-231)
 call lsmp_check(c_lsmp_interface_l1(h, l1), "lsmp_interface_l1")
+if (method == 3) call lsmp_check(c_lsmp_vof_interface_l1(h, l1v), "lsmp_vof_interface_l1")
 open(unit=456, file="./Vortex_Results.txt", action='write', position='append')
 if (method == 1) then
     write(456, *) "Method: LS"
-else
+else if (method == 2) then
     write(456, *) "Method: MPLS"
+else
+    write(456, *) "Method: CLSVOF"
 endif
 write(456,*) "Density Ratio:", rho_12
 write(456,*) "Nx :", nx
-write(456,'(A10, 2ES15.4)') "LS Mass:", loss_mass_avg / iter, loss_mass_max
-write(456,'(A10, 2ES15.4)') "LS Vol:", loss_vol_avg / iter, loss_vol_max
-write(456,'(A10, 2ES15.4)') "LS Int:", l1
+if (method == 3) then
+    write(456,'(A10, 2ES15.4)') "VOF Mass:", loss_mass_avgv / iter, loss_mass_maxv
+    write(456,'(A10, 2ES15.4)') "VOF Vol:", loss_vol_avgv / iter, loss_vol_maxv
+    write(456,'(A10, 2ES15.4)') "VOF Int:", l1v
+else
+    write(456,'(A10, 2ES15.4)') "LS Mass:", loss_mass_avg / iter, loss_mass_max
+    write(456,'(A10, 2ES15.4)') "LS Vol:", loss_vol_avg / iter, loss_vol_max
+    write(456,'(A10, 2ES15.4)') "LS Int:", l1
+endif
 close(456)
 write(*,'(A,I8,A,ES24.16,A,ES24.16)') " finished: steps", iter, "  L1", l1, "  max |dM|/M", loss_mass_max
+if (method == 3) write(*,'(A,ES24.16,A,ES24.16)') " VOF: L1", l1v, "  max |dM|/M", loss_mass_maxv
 write(*,'(A,F12.4,A,ES12.4)') " wall ms/step", 1.0d3 * real(clk1 - clk0, 8) / real(clkrate, 8) / iter, &
     "  cell-updates/s", real(nx, 8) * ny * nz * iter / (real(clk1 - clk0, 8) / real(clkrate, 8))
 
@@ -307,6 +356,7 @@ subroutine initial_phi()
             end do
             end do
             phi(i,j,k) = vof * 2.0 - 1.0
+            if (method == 3) vofb(i,j,k) = vof
         case ("vortex2d")
             phi(i,j,k) = -sqrt((xc(i)-0.5)**2+(yc(j)-0.75)**2) + 0.15
         case default
@@ -316,6 +366,59 @@ subroutine initial_phi()
     enddo
     enddo
     call bc_box(phi)
+end subroutine
+
+! The VOF of the circle (Vortex2D) or sphere (Vortex) from 30^3 sub-samples
+! per cell (problem_init.f90:32-59; the 2D case loops over the z samples too).
+! Every inside sample adds the same 1/30^3, so a cell whose centre lies more
+! than 1.5 dx from the surface -- all its samples on one side -- holds the full
+! sum or zero, and only the others are sampled.
+subroutine initial_vof()
+    integer :: i, j, k, ii, jj, kk, sub
+    real(8) :: x, y, z, v, full, d
+    logical :: circle
+    circle = trim(casename) == "vortex2d"
+    sub = 30
+    full = 0.0d0
+    do ii = 1, sub**3
+        full = full + 1.0d0/real(sub,8)**3.0d0
+    enddo
+    do k = 1, nz
+    do j = 1, ny
+    do i = 1, nx
+        if (circle) then
+            d = sqrt((xc(i)-0.5)**2+(yc(j)-0.75)**2)
+        else
+            d = sqrt((xc(i)-0.35)**2+(yc(j)-0.35)**2+(zc(k)-0.35)**2)
+        endif
+        if (abs(d - 0.15d0) >= 1.5d0*dx) then
+            if (d < 0.15d0) then
+                vofb(i,j,k) = full
+            else
+                vofb(i,j,k) = 0.0d0
+            endif
+            cycle
+        endif
+        v = 0.0d0
+        do ii = 1, sub
+        do jj = 1, sub
+        do kk = 1, sub
+            x = 0.5d0*( xc(i)+xc(i-1) ) + real(ii,8)*dx/real(sub,8)
+            y = 0.5d0*( yc(j)+yc(j-1) ) + real(jj,8)*dy/real(sub,8)
+            z = 0.5d0*( zc(k)+zc(k-1) ) + real(kk,8)*dz/real(sub,8)
+            if (circle) then
+                if (sqrt((x-0.5)**2+(y-0.75)**2) <= 0.15) v = v + 1.0d0/real(sub,8)**3.0d0
+            else
+                if (sqrt((x-0.35)**2+(y-0.35)**2+(z-0.35)**2) <= 0.15) v = v + 1.0d0/real(sub,8)**3.0d0
+            endif
+        enddo
+        enddo
+        enddo
+        vofb(i,j,k) = v
+    enddo
+    enddo
+    enddo
+    call bc_box(vofb)
 end subroutine
 
 ! zero-gradient ghost fill, x then y then z (branches.f90:283-310)

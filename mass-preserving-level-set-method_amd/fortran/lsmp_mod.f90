@@ -289,6 +289,20 @@ interface
         import :: c_int, c_ptr
         type(c_ptr), value :: h
     end function
+    integer(c_int) function c_lsmp_store_vof0(h) bind(C, name="lsmp_store_vof0")
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h
+    end function
+    integer(c_int) function c_lsmp_vof_mass_vol(h, mass, vol) bind(C, name="lsmp_vof_mass_vol")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(out) :: mass, vol
+    end function
+    integer(c_int) function c_lsmp_vof_interface_l1(h, l1) bind(C, name="lsmp_vof_interface_l1")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), intent(out) :: l1
+    end function
     integer(c_int) function c_lsmp_set_face_velocity(h, u, v, w) bind(C, name="lsmp_set_face_velocity")
         import :: c_int, c_ptr, c_double
         type(c_ptr), value :: h
@@ -329,6 +343,7 @@ public :: c_lsmp_net_clear_samples, c_lsmp_net_apply, c_lsmp_net_train, c_lsmp_n
 public :: c_lsmp_reinit_learned, c_lsmp_reinit_residual
 public :: c_lsmp_upload_vof, c_lsmp_download_vof, c_lsmp_store_vof_old, c_lsmp_set_face_velocity
 public :: c_lsmp_set_face_velocity_analytic, c_lsmp_vof_wlic, c_lsmp_clsvof_recon
+public :: c_lsmp_store_vof0, c_lsmp_vof_mass_vol, c_lsmp_vof_interface_l1
 public :: lsmp_check, lsmp_error_message
 
 contains

@@ -39,7 +39,8 @@ EXPORTS = [
     "lsmp_net_eval_grad", "lsmp_reinit_learned", "lsmp_reinit_residual", "lsmp_net_add_samples",
     "lsmp_net_clear_samples", "lsmp_net_apply",
     "lsmp_upload_vof", "lsmp_download_vof", "lsmp_store_vof_old", "lsmp_set_face_velocity",
-    "lsmp_set_face_velocity_analytic", "lsmp_vof_wlic", "lsmp_clsvof_recon",
+    "lsmp_set_face_velocity_analytic", "lsmp_vof_wlic", "lsmp_clsvof_recon", "lsmp_store_vof0",
+    "lsmp_vof_mass_vol", "lsmp_vof_interface_l1",
 ]
 NO_STATUS = ("lsmp_finalize", "lsmp_error_string", "lsmp_net_destroy")
 
@@ -131,6 +132,9 @@ def load_library(path=LIB_PATH):
     L.lsmp_set_face_velocity_analytic.argtypes = [H, C.c_int, C.c_double, C.c_double]
     L.lsmp_vof_wlic.argtypes = [H, C.c_double]
     L.lsmp_clsvof_recon.argtypes = [H, _IP, _IP]
+    L.lsmp_store_vof0.argtypes = [H]
+    L.lsmp_vof_mass_vol.argtypes = [H, _DP, _DP]
+    L.lsmp_vof_interface_l1.argtypes = [H, _DP]
     for name in EXPORTS:
         if name not in NO_STATUS:
             getattr(L, name).restype = C.c_int
@@ -354,6 +358,22 @@ class Solver:
     def store_vof_old(self):
         """vof%old <- vof%now (the VOF part of p%switch, tree.f90:383-392)."""
         _check(self.lib.lsmp_store_vof_old(self.h))
+
+    def store_vof0(self):
+        """vof%tmp2 <- vof%now: the initial VOF of vof_interface_l1."""
+        _check(self.lib.lsmp_store_vof0(self.h))
+
+    def vof_mass_vol(self):
+        """ls_mv, VOF part (ns_solver/others.f90:215-247).  Returns (mass, vol)."""
+        m, v = C.c_double(), C.c_double()
+        _check(self.lib.lsmp_vof_mass_vol(self.h, C.byref(m), C.byref(v)))
+        return m.value, v.value
+
+    def vof_interface_l1(self):
+        """calculate_interface_loss, VOF part: sum |vof0 - vof| / cells."""
+        r = C.c_double()
+        _check(self.lib.lsmp_vof_interface_l1(self.h, C.byref(r)))
+        return r.value
 
     def set_face_velocity(self, u, v, w=None):
         """vel%x/y/z%old: index i of a box holds the face i + 1/2."""
