@@ -148,6 +148,13 @@ def test_thinc_flux_matches_quadrature_of_the_profile():
                 alpha = -1.0 if full_left else 1.0          # vof(left) < vof(right) -> +1
                 want = np.sign(u) * thinc_flux_by_quadrature(vfrac, alpha, abs(u) * dt / dx, u > 0, dx)
                 assert abs(got - want) <= 1e-10 * abs(want), (u, vfrac, full_left, got, want)
+                # the check tells the plausible mistakes apart: the swept part taken
+                # at the wrong end of the cell, or the profile's slope reversed
+                wrong_end = np.sign(u) * thinc_flux_by_quadrature(vfrac, alpha, abs(u) * dt / dx, u < 0, dx)
+                wrong_slope = np.sign(u) * thinc_flux_by_quadrature(vfrac, -alpha, abs(u) * dt / dx, u > 0, dx)
+                assert abs(got - wrong_end) > 1e-3 * abs(want)
+                if vfrac != 0.5:
+                    assert abs(got - wrong_slope) > 1e-3 * abs(want)
 
 
 def test_recon_is_the_reference_sequence():
