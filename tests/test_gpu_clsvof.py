@@ -5,8 +5,10 @@ velocities from the prescribed field on the device or given by the host
 (2.2e-16 measured); clsvof_recon's phi to 1e-12 relative (2e-15); three
 merged CLSVOF steps (velocities, vof_old, WLIC, CRWENO5 RK3, reconstruction
 every step) to 1e-12 on phi and vof (4e-15 and 7e-16); the FP32 bulk form to
-1e-5 (2.4e-6).  Interfaces stay away from the walls, where the one documented
-difference (the normal at a ghost cell) cannot arise.
+1e-5 (2.4e-6); the VOF half of ls_mv against the oracle's and the VOF shape
+error against the downloaded fields to 1e-13 relative (only the summation
+order differs).  Interfaces stay away from the walls, where the one
+documented difference (the normal at a ghost cell) cannot arise.
 """
 import sys
 
@@ -136,3 +138,42 @@ def test_clsvof_call_order_and_arguments():
         assert s.lib.lsmp_vof_wlic(s.h, s.params.dt) == -5          # not with the narrow band
     with lsmp.Solver(lsmp.make_params(16, dim=2)) as s:
         assert s.lib.lsmp_set_face_velocity_analytic(s.h, 2, 0.0, 3.0) == -1   # LeVeque is 3D
+
+
+def start(n, dim):
+    lsmp = load_lsmp()
+    o = O.Oracle(O.make_params(n, dim=dim))
+    o.initial_phi(2 if dim == 3 else 1, 1)
+    s = lsmp.Solver(lsmp.make_params(n, dim=dim))
+    s.upload_phi(o.field("PHI"))
+    s.upload_vof(o.field("VOF"))
+    return lsmp, o, s
+
+
+@pytest.mark.parametrize("dim, n", [(3, 20), (2, 40)], ids=["3d", "2d"])
+def test_vof_mass_vol_matches_oracle(dim, n):
+    lsmp, o, s = start(n, dim)
+    say("vof_mass_vol", dim, n)
+    o.ls_mv()
+    m, v = s.vof_mass_vol()
+    say(f"  mass {m!r} / {o.get('massv')!r}, vol {v!r} / {o.get('volv')!r}")
+    assert abs(m - o.get("massv")) <= 1e-13 * o.get("massv")
+    assert abs(v - o.get("volv")) <= 1e-13 * o.get("volv")
+    s.close()
+
+
+def test_vof_interface_l1_matches_the_fields():
+    lsmp, o, s = start(20, 3)
+    say("vof_interface_l1")
+    s.store_vof0()
+    v0 = s.download_vof()[3:-3, 3:-3, 3:-3].copy()
+    t = 0.0
+    for it in range(1, 4):
+        t += s.params.dt
+        s.clsvof_step(t, 3.0, lsmp.VEL_LEVEQUE3D, iteration=it)
+    v = s.download_vof()[3:-3, 3:-3, 3:-3]
+    want = np.sum(np.abs(v0 - v)) / v.size
+    got = s.vof_interface_l1()
+    say(f"  l1 {got!r} / {want!r}")
+    assert want > 0 and abs(got - want) <= 1e-13 * want
+    s.close()
