@@ -140,19 +140,9 @@ def test_clsvof_call_order_and_arguments():
         assert s.lib.lsmp_set_face_velocity_analytic(s.h, 2, 0.0, 3.0) == -1   # LeVeque is 3D
 
 
-def start(n, dim):
-    lsmp = load_lsmp()
-    o = O.Oracle(O.make_params(n, dim=dim))
-    o.initial_phi(2 if dim == 3 else 1, 1)
-    s = lsmp.Solver(lsmp.make_params(n, dim=dim))
-    s.upload_phi(o.field("PHI"))
-    s.upload_vof(o.field("VOF"))
-    return lsmp, o, s
-
-
 @pytest.mark.parametrize("dim, n", [(3, 20), (2, 40)], ids=["3d", "2d"])
 def test_vof_mass_vol_matches_oracle(dim, n):
-    lsmp, o, s = start(n, dim)
+    lsmp, o, s = setup(dict(nx=n, ny=n, nz=n if dim == 3 else 1, dim=dim))
     say("vof_mass_vol", dim, n)
     o.ls_mv()
     m, v = s.vof_mass_vol()
@@ -163,7 +153,7 @@ def test_vof_mass_vol_matches_oracle(dim, n):
 
 
 def test_vof_interface_l1_matches_the_fields():
-    lsmp, o, s = start(20, 3)
+    lsmp, o, s = setup(dict(nx=20, ny=20, nz=20, dim=3))
     say("vof_interface_l1")
     s.store_vof0()
     v0 = s.download_vof()[3:-3, 3:-3, 3:-3].copy()
