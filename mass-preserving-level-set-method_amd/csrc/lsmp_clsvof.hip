@@ -48,8 +48,9 @@ __global__ void k_face_velocity(T *__restrict__ uf, T *__restrict__ vf, T *__res
 }
 
 // One direction's WLIC pass: out = the updated vof of every cell from in
-// (vof_wlic_solver's flux loop and update loop of direction D).
-template <typename T, int D, bool CLAMP>
+// (vof_wlic_solver's flux loop and update loop of direction D).  STENCIL:
+// the cell update that issues its stencil's loads up front (same results).
+template <typename T, int D, bool CLAMP, bool STENCIL>
 __global__ void __launch_bounds__(256) k_wlic_pass(const T *__restrict__ in, T *__restrict__ out,
                                                    const T *__restrict__ old, const T *__restrict__ vel,
                                                    const T *__restrict__ n0, const T *__restrict__ n1,
@@ -60,7 +61,8 @@ __global__ void __launch_bounds__(256) k_wlic_pass(const T *__restrict__ in, T *
     if (q >= (size_t)nx * ny * nz) return;
     int p;
     const WlicLine<T> L = wlic_line<D>(in, vel, n0, n1, n2, q, nx, ny, nz, &p);
-    out[q] = (T)wlic_cell<D, CLAMP>(L, old, p, dt, dd);
+    if (STENCIL) out[q] = (T)wlic_cell_stencil<D, CLAMP>(L, old, p, dt, dd);
+    else out[q] = (T)wlic_cell<D, CLAMP>(L, old, p, dt, dd);
 }
 
 // clsvof_recon, first loop (clsvof.f90:283-292): keep phi, phi <- 2 vof - 1
@@ -134,12 +136,20 @@ void launch_face_velocity(hipStream_t st, T *uf, T *vf, T *wf, const FaceVelArgs
 
 template <typename T>
 void launch_wlic_pass(hipStream_t st, int dir, bool clamp, const T *in, T *out, const T *old, const T *vel,
-                      const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd)
+                      const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd,
+                      bool stencil)
 {
     const size_t n = (size_t)nx * ny * nz;
     const dim3 g(nblocks(n, 256)), b(256);
 #define LSMP_WLIC(D, C)                                                                                        \
-    hipLaunchKernelGGL((k_wlic_pass<T, D, C>), g, b, 0, st, in, out, old, vel, n0, n1, n2, nx, ny, nz, dt, dd)
+    do {                                                                                                       \
+        if (stencil)                                                                                           \
+            hipLaunchKernelGGL((k_wlic_pass<T, D, C, true>), g, b, 0, st, in, out, old, vel, n0, n1, n2, nx, ny, \
+                               nz, dt, dd);                                                                    \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_wlic_pass<T, D, C, false>), g, b, 0, st, in, out, old, vel, n0, n1, n2, nx,    \
+                               ny, nz, dt, dd);                                                                \
+    } while (0)
     if (dir == 0) {
         if (clamp) LSMP_WLIC(0, true);
         else LSMP_WLIC(0, false);
@@ -166,7 +176,7 @@ template <typename T> void launch_recon_keep(hipStream_t st, T *phi, const T *ke
     template void launch_face_velocity<T>(hipStream_t, T *, T *, T *, const FaceVelArgs &, double, double,     \
                                           double, double *);                                                   \
     template void launch_wlic_pass<T>(hipStream_t, int, bool, const T *, T *, const T *, const T *, const T *, \
-                                      const T *, const T *, int, int, int, double, double);                    \
+                                      const T *, const T *, int, int, int, double, double, bool);              \
     template void launch_recon_start<T>(hipStream_t, T *, const T *, T *, size_t);                             \
     template void launch_recon_keep<T>(hipStream_t, T *, const T *, size_t, double);                          \
     template void launch_vof_sums<T>(hipStream_t, const T *, const T *, size_t, double, double, int, int,      \

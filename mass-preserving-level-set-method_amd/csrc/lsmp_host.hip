@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -613,7 +614,11 @@ int ensure_clsvof(lsmp_ctx *c)
             LSMP_HIP(hipMalloc(p, b));
             LSMP_HIP(hipMemsetAsync(*p, 0, b, c->st));
         }
-    if (!c->ftab) LSMP_HIP(hipMalloc((void **)&c->ftab, sizeof(double) * 3 * (size_t)(c->nx + c->ny + c->nz)));
+    if (!c->ftab) {
+        LSMP_HIP(hipMalloc((void **)&c->ftab, sizeof(double) * 3 * (size_t)(c->nx + c->ny + c->nz)));
+        const char *e = std::getenv("LSMP_WLIC_STENCIL");
+        c->wlic_stencil = e && std::atoi(e) != 0;
+    }
     LSMP_HIP(hipStreamSynchronize(c->st));
     return LSMP_OK;
 }
@@ -650,7 +655,7 @@ template <typename T> int vof_wlic(lsmp_ctx *c, double dt)
     for (int d = 0; d < c->dim; ++d) {
         KT(c, "wlic", (launch_wlic_pass<T>(c->st, d, d == c->dim - 1, in, out, (const T *)c->vof_old, vel[d],
                                            (const T *)c->nrm[0], (const T *)c->nrm[1], n2, c->nx, c->ny, c->nz, dt,
-                                           dd[d])));
+                                           dd[d], c->wlic_stencil)));
         std::swap(in, out);
     }
     c->vof = in;

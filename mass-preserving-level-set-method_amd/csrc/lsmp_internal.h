@@ -149,10 +149,11 @@ template <typename T>
 void launch_face_velocity(hipStream_t st, T *uf, T *vf, T *wf, const FaceVelArgs &va, double xend, double yend,
                           double zend, double *tab);
 // one direction's WLIC-THINC pass, in -> out (n2 null in 2D; clamp on the
-// last direction)
+// last direction; stencil: the variant that loads each cell's stencil first)
 template <typename T>
 void launch_wlic_pass(hipStream_t st, int dir, bool clamp, const T *in, T *out, const T *old, const T *vel,
-                      const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd);
+                      const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd,
+                      bool stencil);
 template <typename T> void launch_recon_start(hipStream_t st, T *phi, const T *vof, T *keep, size_t n);
 template <typename T> void launch_recon_keep(hipStream_t st, T *phi, const T *keep, size_t n, double w);
 // block partial sums (3 per block) over the VOF: volume and mass (l1 == 0) or

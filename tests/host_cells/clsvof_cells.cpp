@@ -10,6 +10,14 @@
 
 using namespace lsmp;
 
+// one cell of a WLIC pass: the kernels' update, or the one that loads its
+// stencil up front
+template <int D, bool CLAMP>
+double cell(int stencil, const WlicLine<double> &L, const double *old, int p, double dt, double dd)
+{
+    return stencil ? wlic_cell_stencil<D, CLAMP>(L, old, p, dt, dd) : wlic_cell<D, CLAMP>(L, old, p, dt, dd);
+}
+
 extern "C" {
 
 // face velocities of a prescribed field into interior arrays (x fastest)
@@ -33,10 +41,11 @@ void hc_face_velocity(int nx, int ny, int nz, int field, double xstart, double x
     }
 }
 
-// vof_wlic's passes over an interior vof array, in place (n2 null in 2D)
+// vof_wlic's passes over an interior vof array, in place (n2 null in 2D);
+// stencil selects the cell update that loads its stencil up front
 void hc_vof_wlic(int dim, int nx, int ny, int nz, double dt, double dx, double dy, double dz, double *vof,
                  const double *old, const double *uf, const double *vf, const double *wf, const double *n0,
-                 const double *n1, const double *n2)
+                 const double *n1, const double *n2, int stencil)
 {
     const size_t n = (size_t)nx * ny * nz;
     std::vector<double> in(vof, vof + n), out(n);
@@ -48,13 +57,13 @@ void hc_vof_wlic(int dim, int nx, int ny, int nz, double dt, double dx, double d
             int p;
             if (d == 0) {
                 const WlicLine<double> L = wlic_line<0>(in.data(), vel[0], n0, n1, n2, q, nx, ny, nz, &p);
-                out[q] = clamp ? wlic_cell<0, true>(L, old, p, dt, dd[0]) : wlic_cell<0, false>(L, old, p, dt, dd[0]);
+                out[q] = clamp ? cell<0, true>(stencil, L, old, p, dt, dd[0]) : cell<0, false>(stencil, L, old, p, dt, dd[0]);
             } else if (d == 1) {
                 const WlicLine<double> L = wlic_line<1>(in.data(), vel[1], n0, n1, n2, q, nx, ny, nz, &p);
-                out[q] = clamp ? wlic_cell<1, true>(L, old, p, dt, dd[1]) : wlic_cell<1, false>(L, old, p, dt, dd[1]);
+                out[q] = clamp ? cell<1, true>(stencil, L, old, p, dt, dd[1]) : cell<1, false>(stencil, L, old, p, dt, dd[1]);
             } else {
                 const WlicLine<double> L = wlic_line<2>(in.data(), vel[2], n0, n1, n2, q, nx, ny, nz, &p);
-                out[q] = wlic_cell<2, true>(L, old, p, dt, dd[2]);
+                out[q] = cell<2, true>(stencil, L, old, p, dt, dd[2]);
             }
         }
         std::swap(in, out);

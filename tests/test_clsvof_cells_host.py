@@ -7,7 +7,8 @@ with the oracle's flags and runs the same loops cell by cell; here they must
 reproduce the oracle bit for bit: the face velocities of the three prescribed
 fields, and vof_wlic_solver's three passes (fluxes, divergence term, clamp,
 index clamping at the walls) on a fresh sphere or circle and after a few
-CLSVOF steps.  The kernels add only the thread-to-cell map and the store.
+CLSVOF steps, through both cell updates (the one that loads its stencil up
+front as well).  The kernels add only the thread-to-cell map and the store.
 """
 import ctypes as C
 import os
@@ -29,7 +30,7 @@ def cells():
     subprocess.run(["make", "-s", "-C", CELLS], check=True)
     L = C.CDLL(LIB)
     L.hc_face_velocity.argtypes = [C.c_int] * 4 + [C.c_double] * 11 + [DP] * 3
-    L.hc_vof_wlic.argtypes = [C.c_int] * 4 + [C.c_double] * 4 + [DP] * 8
+    L.hc_vof_wlic.argtypes = [C.c_int] * 4 + [C.c_double] * 4 + [DP] * 8 + [C.c_int]
     return L
 
 
@@ -64,15 +65,16 @@ def test_face_velocities_bitwise(cells, dim, field, shape):
         assert np.array_equal(g, want), (name, np.max(np.abs(g - want)))
 
 
-def run_wlic(cells, o):
-    """One vof_wlic on the host cells from the oracle's current state."""
+def run_wlic(cells, o, stencil=0):
+    """One vof_wlic on the host cells from the oracle's current state
+    (stencil: the cell update that loads its stencil up front)."""
     p = o.p
     o.surface_norms()   # what vof_wlic computes first, from the same phi
     vof = interior(o, "VOF").copy()
     args = [interior(o, n) for n in ("VOF_OLD", "UF", "VF", "WF", "NX", "NY")]
     n2 = interior(o, "NZ") if p.dim == 3 else None
     cells.hc_vof_wlic(p.dim, p.nx, p.ny, p.nz, p.dt, p.dx, p.dy, p.dz, ptr(vof), *(ptr(a) for a in args),
-                      ptr(n2) if n2 is not None else None)
+                      ptr(n2) if n2 is not None else None, stencil)
     return vof
 
 
@@ -89,8 +91,10 @@ def test_wlic_passes_bitwise(cells, dim, shape):
         o.velocity_faces(field, (step + 1) * o.p.dt, 3.0)
         o.field("VOF_OLD")[:] = o.field("VOF")
         got = run_wlic(cells, o)
+        got_stencil = run_wlic(cells, o, stencil=1)
         o.vof_wlic()
         want = interior(o, "VOF")
         cut = np.count_nonzero((want > 1e-10) & (want < 1 - 1e-10))
         assert cut > 50
         assert np.array_equal(got, want), (step, np.max(np.abs(got - want)))
+        assert np.array_equal(got_stencil, want), (step, np.max(np.abs(got_stencil - want)))
