@@ -181,6 +181,7 @@ endif
 call lsmp_check(c_lsmp_store_phi0(h), "lsmp_store_phi0")     ! phi%tmp2 (roots.f90:150-166)
 call lsmp_check(c_lsmp_mass_vol(h, imass, ivol), "lsmp_mass_vol")   ! problem_init.f90:140-142
 massv = 0.0d0; volv = 0.0d0; imassv = 0.0d0; ivolv = 0.0d0; l1v = 0.0d0
+if (trim(casename) /= "disk") red_error = 0.0d0              ! printed before any btn = 1 reinitialisation
 if (method == 3) then
     if (trim(casename) == "disk") then
         call bc_box(vofb)          ! the disk's VOF was sampled with its phi
