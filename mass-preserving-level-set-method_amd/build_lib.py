@@ -51,7 +51,7 @@ def build_driver(verbose=False):
     """The Fortran driver (ISO_C_BINDING) that replaces the reference's
     main.f90 time loop around the library.  Skipped when flang is absent."""
     flang = shutil.which("flang") or shutil.which("amdflang")
-    srcs = [os.path.join(FDIR, f) for f in ("lsmp_mod.f90", "lsmp_driver.f90")]
+    srcs = [os.path.join(FDIR, f) for f in ("lsmp_mod.f90", "lsmp_cases.f90", "lsmp_driver.f90")]
     if not flang or not all(os.path.exists(s) for s in srcs):
         return None
     mod = os.path.join(LIB_DIR, "mod")

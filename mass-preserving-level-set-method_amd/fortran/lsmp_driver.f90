@@ -20,6 +20,7 @@
 program lsmp_driver
 use iso_c_binding
 use lsmp
+use lsmp_cases
 implicit none
 
 character(len=256) :: casename, deck, arg
@@ -150,9 +151,9 @@ ls_wid = ls_wid * dx
 if (t2red < 0) t2red = int(t2s / dt) + 1                      ! tree.f90:352
 
 allocate(xc(0:nx+1), yc(0:ny+1), zc(0:nz+1))
-call coords(xc, nx, xstart, xend, dx)
-call coords(yc, ny, ystart, yend, dy)
-call coords(zc, nz, zstart, zend, dz)
+call case_coords(xc, nx, xstart, xend, dx)
+call case_coords(yc, ny, ystart, yend, dy)
+call case_coords(zc, nz, zstart, zend, dz)
 
 write(*,*) " --- Problem information --- "
 write(*,'(A30)') name
@@ -172,7 +173,11 @@ call lsmp_check(c_lsmp_set_advection_scheme(h, scheme), "lsmp_set_advection_sche
 ! ------------------------------------------------------------ initial condition (problem_init.f90:32-103)
 allocate(phi(1-ghc:nx+ghc, 1-ghc:ny+ghc, kb0:kb1))
 if (method == 3) allocate(vofb(1-ghc:nx+ghc, 1-ghc:ny+ghc, kb0:kb1))
-call initial_phi()
+if (method == 3) then
+    call case_initial_phi(casename, nx, ny, nz, ghc, kb0, kb1, dx, dy, xc, yc, zc, phi, vofb)
+else
+    call case_initial_phi(casename, nx, ny, nz, ghc, kb0, kb1, dx, dy, xc, yc, zc, phi)
+endif
 call lsmp_check(c_lsmp_upload_phi(h, phi), "lsmp_upload_phi")
 if (trim(casename) == "disk") then
     call lsmp_check(c_lsmp_reinit(h, 0_c_int, -1.0d0, red_error, npseudo), "lsmp_reinit(0)")  ! :103
@@ -184,9 +189,9 @@ massv = 0.0d0; volv = 0.0d0; imassv = 0.0d0; ivolv = 0.0d0; l1v = 0.0d0
 if (trim(casename) /= "disk") red_error = 0.0d0              ! printed before any btn = 1 reinitialisation
 if (method == 3) then
     if (trim(casename) == "disk") then
-        call bc_box(vofb)          ! the disk's VOF was sampled with its phi
+        call case_bc_box(vofb, nx, ny, nz, ghc, kb0, kb1)   ! the disk's VOF was sampled with its phi
     else
-        call initial_vof()
+        call case_initial_vof(casename, nx, ny, nz, ghc, kb0, kb1, dx, dy, dz, xc, yc, zc, vofb)
     endif
     call lsmp_check(c_lsmp_upload_vof(h, vofb), "lsmp_upload_vof")
     call lsmp_check(c_lsmp_store_vof0(h), "lsmp_store_vof0")           ! vof%tmp2
@@ -316,141 +321,6 @@ close(fmass); close(fvol); close(fen); close(fmom); close(fpos)
 call c_lsmp_finalize(h)
 
 contains
-
-! cell centres x(i) = start + (i - 0.5) dx with (i - 0.5) in default real,
-! x(0) = start, x(n+1) = end (tree.f90:254-274)
-subroutine coords(x, n, s, e, d)
-    integer, intent(in) :: n
-    real(8), intent(out) :: x(0:n+1)
-    real(8), intent(in) :: s, e, d
-    integer :: i
-    do i = 1, n
-        x(i) = s + (i - 0.5) * d
-    enddo
-    x(0) = s
-    x(n+1) = e
-end subroutine
-
-! problem_init.f90:32-79 for the three cases, then the zero-gradient ghosts
-subroutine initial_phi()
-    integer :: i, j, k, ii, jj, kk, sub
-    real(8) :: x, y, z, vof
-    sub = 30
-    do k = 1, nz
-    do j = 1, ny
-    do i = 1, nx
-        select case (trim(casename))
-        case ("disk")
-            vof = 0.0d0
-            do ii = 1, sub
-            do jj = 1, sub
-            do kk = 1, sub
-                x = 0.5d0*( xc(i)+xc(i-1) ) + real(ii,8)*dx/real(sub,8)
-                y = 0.5d0*( yc(j)+yc(j-1) ) + real(jj,8)*dy/real(sub,8)
-                if (sqrt((x-0.5)**2+(y-0.75)**2) <= 0.15) then
-                    vof = vof + 1.0d0/real(sub,8)**3.0d0
-                end if
-                if (abs(x-0.5) <= 0.025 .and. y < 0.8 .and. (x-0.5)**2+(y-0.75)**2 <= 0.15**2) then
-                    vof = vof - 1.0d0/real(sub,8)**3.0d0
-                end if
-            end do
-            end do
-            end do
-            phi(i,j,k) = vof * 2.0 - 1.0
-            if (method == 3) vofb(i,j,k) = vof
-        case ("vortex2d")
-            phi(i,j,k) = -sqrt((xc(i)-0.5)**2+(yc(j)-0.75)**2) + 0.15
-        case default
-            phi(i,j,k) = - sqrt((xc(i)-0.35)**2+(yc(j)-0.35)**2+(zc(k)-0.35)**2) + 0.15
-        end select
-    enddo
-    enddo
-    enddo
-    call bc_box(phi)
-end subroutine
-
-! The VOF of the circle (Vortex2D) or sphere (Vortex) from 30^3 sub-samples
-! per cell (problem_init.f90:32-59; the 2D case loops over the z samples too).
-! Every inside sample adds the same 1/30^3, so a cell whose centre lies more
-! than 1.5 dx from the surface -- all its samples on one side -- holds the full
-! sum or zero, and only the others are sampled.
-subroutine initial_vof()
-    integer :: i, j, k, ii, jj, kk, sub
-    real(8) :: x, y, z, v, full, d
-    logical :: circle
-    circle = trim(casename) == "vortex2d"
-    sub = 30
-    full = 0.0d0
-    do ii = 1, sub**3
-        full = full + 1.0d0/real(sub,8)**3.0d0
-    enddo
-    do k = 1, nz
-    do j = 1, ny
-    do i = 1, nx
-        if (circle) then
-            d = sqrt((xc(i)-0.5)**2+(yc(j)-0.75)**2)
-        else
-            d = sqrt((xc(i)-0.35)**2+(yc(j)-0.35)**2+(zc(k)-0.35)**2)
-        endif
-        if (abs(d - 0.15d0) >= 1.5d0*dx) then
-            if (d < 0.15d0) then
-                vofb(i,j,k) = full
-            else
-                vofb(i,j,k) = 0.0d0
-            endif
-            cycle
-        endif
-        v = 0.0d0
-        do ii = 1, sub
-        do jj = 1, sub
-        do kk = 1, sub
-            x = 0.5d0*( xc(i)+xc(i-1) ) + real(ii,8)*dx/real(sub,8)
-            y = 0.5d0*( yc(j)+yc(j-1) ) + real(jj,8)*dy/real(sub,8)
-            z = 0.5d0*( zc(k)+zc(k-1) ) + real(kk,8)*dz/real(sub,8)
-            if (circle) then
-                if (sqrt((x-0.5)**2+(y-0.75)**2) <= 0.15) v = v + 1.0d0/real(sub,8)**3.0d0
-            else
-                if (sqrt((x-0.35)**2+(y-0.35)**2+(z-0.35)**2) <= 0.15) v = v + 1.0d0/real(sub,8)**3.0d0
-            endif
-        enddo
-        enddo
-        enddo
-        vofb(i,j,k) = v
-    enddo
-    enddo
-    enddo
-    call bc_box(vofb)
-end subroutine
-
-! zero-gradient ghost fill, x then y then z (branches.f90:283-310)
-subroutine bc_box(dat)
-    real(8), intent(inout) :: dat(1-ghc:nx+ghc, 1-ghc:ny+ghc, kb0:kb1)
-    integer :: i, j, k
-    do k = kb0, kb1
-    do j = 1-ghc, ny+ghc
-    do i = 1, ghc
-        dat(1-i,j,k) = dat(1,j,k)
-        dat(nx+i,j,k) = dat(nx,j,k)
-    enddo
-    enddo
-    enddo
-    do k = kb0, kb1
-    do i = 1-ghc, nx+ghc
-    do j = 1, ghc
-        dat(i,1-j,k) = dat(i,1,k)
-        dat(i,ny+j,k) = dat(i,ny,k)
-    enddo
-    enddo
-    enddo
-    do j = 1-ghc, ny+ghc
-    do i = 1-ghc, nx+ghc
-    do k = 1, ghc
-        dat(i,j,1-k) = dat(i,j,1)
-        dat(i,j,nz+k) = dat(i,j,nz)
-    enddo
-    enddo
-    enddo
-end subroutine
 
 ! print_CPU_info (output.f90:129-143) with device time per phase
 subroutine print_device_time(s)
