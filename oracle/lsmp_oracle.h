@@ -1,6 +1,6 @@
 /*
  * lsmp_oracle.h -- CPU restatement (FP64, scalar C) of the reference's
- * mass-preserving level-set transport step.
+ * mass-preserving level-set transport step and of its CLSVOF step.
  *
  * TEST INFRASTRUCTURE ONLY.  This file and lsmp_oracle.c are the checker for
  * the HIP library: only tests/, __graft_entry__.smoke() and bench.py's
