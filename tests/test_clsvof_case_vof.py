@@ -1,4 +1,4 @@
-"""The CLSVOF case driver's initial VOF (the reference's 30^3 sub-samples per
+"""clsvof.py's initial VOF (the reference's 30^3 sub-samples per
 cell, Vortex/src/setup/problem_init.f90:41-59) equals the oracle's, bit for
 bit, including the rounding of the 27000 partial sums."""
 import os
