@@ -181,6 +181,30 @@ LSMP_HD double wlic_cell_stencil(const WlicLine<T> &L, const T *old, int p, doub
     return v;
 }
 
+// Whether a WLIC step can find cell (i, j, k) cut in any of its passes: not
+// when every cell of its 3 x 3 x 3 neighbourhood (clamped) is empty (vof <=
+// 1e-11) or every one is full (vof >= 1 - 1e-11).  Each pass moves vof by at
+// most one cell along its direction (|u| dt < dd), so after the x and y passes
+// a cell has received vof only from cells up to one step away per axis; and
+// near-empty or near-full neighbourhoods change by less than their 1e-11
+// distance from 0 or 1 times a few, short of the flux's 1e-10 cut threshold.
+template <typename T> LSMP_HD bool wlic_may_cut(const T *vof, int i, int j, int k, int nx, int ny, int nz)
+{
+    double lo = 2.0, hi = -1.0;
+    for (int dk = -1; dk <= 1; ++dk) {
+        const size_t zk = (size_t)clamp_index(k + dk, nz) * (size_t)ny;
+        for (int dj = -1; dj <= 1; ++dj) {
+            const size_t row = ((size_t)clamp_index(j + dj, ny) + zk) * (size_t)nx;
+            for (int di = -1; di <= 1; ++di) {
+                const double v = (double)vof[row + (size_t)clamp_index(i + di, nx)];
+                lo = fmin(lo, v);
+                hi = fmax(hi, v);
+            }
+        }
+    }
+    return hi > 1.0e-11 && lo < 1.0 - 1.0e-11;
+}
+
 // Face velocities of the prescribed fields.  Per axis and cell the table holds
 // sin(pi c), sin(2 pi c) at the cell centre c and sin(pi f) at the face
 // midpoint f = (c + c') / 2, c' the next cell's centre (the wall for the last

@@ -65,6 +65,20 @@ __global__ void __launch_bounds__(256) k_wlic_pass(const T *__restrict__ in, T *
     else out[q] = (T)wlic_cell<D, CLAMP>(L, old, p, dt, dd);
 }
 
+// Flags of the lines through each cell a WLIC step can find cut (wlic_may_cut).
+template <typename T>
+__global__ void k_cut_line_flags(const T *__restrict__ vof, int nx, int ny, int nz, unsigned char *__restrict__ fx,
+                                 unsigned char *__restrict__ fy, unsigned char *__restrict__ fz)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)nx * ny * nz) return;
+    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+    if (!wlic_may_cut(vof, i, j, k, nx, ny, nz)) return;
+    fx[j + (size_t)ny * k] = 1;
+    fy[i + (size_t)nx * k] = 1;
+    if (nz > 1) fz[i + (size_t)nx * j] = 1;
+}
+
 // clsvof_recon, first loop (clsvof.f90:283-292): keep phi, phi <- 2 vof - 1
 template <typename T>
 __global__ void k_recon_start(T *__restrict__ phi, const T *__restrict__ vof, T *__restrict__ keep, size_t n)
@@ -162,6 +176,19 @@ void launch_wlic_pass(hipStream_t st, int dir, bool clamp, const T *in, T *out, 
 #undef LSMP_WLIC
 }
 
+template <typename T>
+void launch_cut_lines(hipStream_t st, int dim, const T *vof, int nx, int ny, int nz, unsigned char *flags,
+                      unsigned *count, int *listx, int *listy, int *listz)
+{
+    const int nfx = ny * nz, nfy = nx * nz, nfz = dim == 3 ? nx * ny : 0;
+    launch_zero_bytes(st, flags, band_flag_bytes(nfx, nfy, nfz));
+    unsigned char *fy = flags + band_flag_offset(1, nfx, nfy), *fz = flags + band_flag_offset(2, nfx, nfy);
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL((k_cut_line_flags<T>), dim3(nblocks(n, 256)), dim3(256), 0, st, vof, nx, ny, nz, flags, fy,
+                       fz);
+    launch_compact_lines(st, dim, flags, nfx, nfy, nfz, listx, listy, listz, count);
+}
+
 template <typename T> void launch_recon_start(hipStream_t st, T *phi, const T *vof, T *keep, size_t n)
 {
     hipLaunchKernelGGL((k_recon_start<T>), dim3(nblocks(n, 256)), dim3(256), 0, st, phi, vof, keep, n);
@@ -178,6 +205,8 @@ template <typename T> void launch_recon_keep(hipStream_t st, T *phi, const T *ke
     template void launch_wlic_pass<T>(hipStream_t, int, bool, const T *, T *, const T *, const T *, const T *, \
                                       const T *, const T *, int, int, int, double, double, bool);              \
     template void launch_recon_start<T>(hipStream_t, T *, const T *, T *, size_t);                             \
+    template void launch_cut_lines<T>(hipStream_t, int, const T *, int, int, int, unsigned char *, unsigned *,  \
+                                      int *, int *, int *);                                                    \
     template void launch_recon_keep<T>(hipStream_t, T *, const T *, size_t, double);                          \
     template void launch_vof_sums<T>(hipStream_t, const T *, const T *, size_t, double, double, int, int,      \
                                      double *, int);

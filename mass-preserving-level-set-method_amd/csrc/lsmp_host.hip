@@ -618,6 +618,8 @@ int ensure_clsvof(lsmp_ctx *c)
         LSMP_HIP(hipMalloc((void **)&c->ftab, sizeof(double) * 3 * (size_t)(c->nx + c->ny + c->nz)));
         const char *e = std::getenv("LSMP_WLIC_STENCIL");
         c->wlic_stencil = e && std::atoi(e) != 0;
+        e = std::getenv("LSMP_CUT_LINES");
+        c->cut_lines = e && std::atoi(e) != 0;
     }
     LSMP_HIP(hipStreamSynchronize(c->st));
     return LSMP_OK;
@@ -644,10 +646,25 @@ template <typename T> int vof_wlic(lsmp_ctx *c, double dt)
 {
     const CcdArgs ca = ccd_args(c);
     static const char *const nname[3] = {"norm_x", "norm_y", "norm_z"};
+    // the fluxes read the normals of cut cells only: with cut_lines the CCD
+    // solves run on the lines that hold one (the same solves, fewer lines)
+    const int *lst[3] = {nullptr, nullptr, nullptr};
+    int nlst[3] = {0, 0, 0};
+    if (c->cut_lines) {
+        KT(c, "cut_lines", (launch_cut_lines<T>(c->st, c->dim, (const T *)c->vof, c->nx, c->ny, c->nz, c->lflag,
+                                                c->lcount, c->llist[0], c->llist[1], c->llist[2])));
+        unsigned *hc = reinterpret_cast<unsigned *>(c->hbuf);
+        LSMP_HIP(hipMemcpyAsync(hc, c->lcount, (size_t)c->dim * sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+        LSMP_HIP(hipStreamSynchronize(c->st));
+        for (int d = 0; d < c->dim; ++d) {
+            lst[d] = c->llist[d];
+            nlst[d] = (int)hc[d];
+        }
+    }
     for (int d = 0; d < c->dim; ++d)
         KT(c, nname[d], (launch_ccd_pass<T>(c->st, d, c->dim, false, (const T *)c->phi, (const T *)c->coef[d],
                                             (T *)c->scr, (T *)c->nrm[d], nullptr, nullptr, nullptr, nullptr, c->nx,
-                                            c->ny, c->nz, ca, nullptr)));
+                                            c->ny, c->nz, ca, nullptr, lst[d], nlst[d])));
     const T *vel[3] = {(const T *)c->uf, (const T *)c->vf, (const T *)c->wf};
     const double dd[3] = {c->p.dx, c->p.dy, c->p.dz};
     const T *n2 = c->dim == 3 ? (const T *)c->nrm[2] : nullptr;

@@ -130,6 +130,11 @@ template <typename T>
 void launch_band_lines(hipStream_t st, int dim, T *phi, int nx, int ny, int nz, double wband,
                        unsigned char *flags, unsigned *count, int *listx, int *listy, int *listz, bool clamp);
 void launch_finalize3(hipStream_t st, const double *partial, int nblk, double imass, double *scal);
+// zero n bytes; the ordered lists and counts of the flagged lines (flags laid
+// out as band_flag_offset; one workgroup per direction)
+void launch_zero_bytes(hipStream_t st, unsigned char *p, size_t n);
+void launch_compact_lines(hipStream_t st, int dim, const unsigned char *flags, int nfx, int nfy, int nfz,
+                          int *listx, int *listy, int *listz, unsigned *count);
 template <typename T>
 void launch_mass_update(hipStream_t st, T *phi, const T *gr, size_t n, T ls_wid, const double *scal);
 template <typename T>
@@ -154,6 +159,11 @@ template <typename T>
 void launch_wlic_pass(hipStream_t st, int dir, bool clamp, const T *in, T *out, const T *old, const T *vel,
                       const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd,
                       bool stencil);
+// flags and ordered lists of the lines (x, y and, in 3D, z) through the cells
+// that a WLIC step can find cut (wlic_may_cut)
+template <typename T>
+void launch_cut_lines(hipStream_t st, int dim, const T *vof, int nx, int ny, int nz, unsigned char *flags,
+                      unsigned *count, int *listx, int *listy, int *listz);
 template <typename T> void launch_recon_start(hipStream_t st, T *phi, const T *vof, T *keep, size_t n);
 template <typename T> void launch_recon_keep(hipStream_t st, T *phi, const T *keep, size_t n, double w);
 // block partial sums (3 per block) over the VOF: volume and mass (l1 == 0) or

@@ -2275,6 +2275,18 @@ void launch_band_lines(hipStream_t st, int dim, T *phi, int nx, int ny, int nz, 
                        count);
 }
 
+void launch_zero_bytes(hipStream_t st, unsigned char *p, size_t n)
+{
+    hipLaunchKernelGGL(k_zero_bytes, dim3(nblk(n, 256)), dim3(256), 0, st, p, n);
+}
+
+void launch_compact_lines(hipStream_t st, int dim, const unsigned char *flags, int nfx, int nfy, int nfz,
+                          int *listx, int *listy, int *listz, unsigned *count)
+{
+    hipLaunchKernelGGL(k_compact_lines, dim3(dim), dim3(1024), 0, st, flags, nfx, nfy, nfz, listx, listy, listz,
+                       count);
+}
+
 template <typename T>
 void launch_mass_update(hipStream_t st, T *phi, const T *gr, size_t n, T ls_wid, const double *scal)
 {

@@ -41,17 +41,18 @@ void hc_face_velocity(int nx, int ny, int nz, int field, double xstart, double x
     }
 }
 
-// vof_wlic's passes over an interior vof array, in place (n2 null in 2D);
-// stencil selects the cell update that loads its stencil up front
-void hc_vof_wlic(int dim, int nx, int ny, int nz, double dt, double dx, double dy, double dz, double *vof,
-                 const double *old, const double *uf, const double *vf, const double *wf, const double *n0,
-                 const double *n1, const double *n2, int stencil)
+// the first npass of vof_wlic's passes over an interior vof array, in place
+// (n2 null in 2D); stencil selects the cell update that loads its stencil up
+// front
+void hc_vof_wlic_n(int dim, int nx, int ny, int nz, double dt, double dx, double dy, double dz, double *vof,
+                   const double *old, const double *uf, const double *vf, const double *wf, const double *n0,
+                   const double *n1, const double *n2, int stencil, int npass)
 {
     const size_t n = (size_t)nx * ny * nz;
     std::vector<double> in(vof, vof + n), out(n);
     const double *vel[3] = {uf, vf, wf};
     const double dd[3] = {dx, dy, dz};
-    for (int d = 0; d < dim; ++d) {
+    for (int d = 0; d < dim && d < npass; ++d) {
         const bool clamp = d == dim - 1;
         for (size_t q = 0; q < n; ++q) {
             int p;
@@ -69,6 +70,29 @@ void hc_vof_wlic(int dim, int nx, int ny, int nz, double dt, double dx, double d
         std::swap(in, out);
     }
     std::memcpy(vof, in.data(), n * sizeof(double));
+}
+
+// vof_wlic's passes over an interior vof array, in place (n2 null in 2D)
+void hc_vof_wlic(int dim, int nx, int ny, int nz, double dt, double dx, double dy, double dz, double *vof,
+                 const double *old, const double *uf, const double *vf, const double *wf, const double *n0,
+                 const double *n1, const double *n2, int stencil)
+{
+    hc_vof_wlic_n(dim, nx, ny, nz, dt, dx, dy, dz, vof, old, uf, vf, wf, n0, n1, n2, stencil, dim);
+}
+
+// k_cut_line_flags on the host: flags of the x lines (ny nz), y lines (nx nz)
+// and z lines (nx ny) through the cells wlic_may_cut selects
+void hc_cut_line_flags(const double *vof, int nx, int ny, int nz, unsigned char *fx, unsigned char *fy,
+                       unsigned char *fz)
+{
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                if (!wlic_may_cut(vof, i, j, k, nx, ny, nz)) continue;
+                fx[j + (size_t)ny * k] = 1;
+                fy[i + (size_t)nx * k] = 1;
+                if (nz > 1) fz[i + (size_t)nx * j] = 1;
+            }
 }
 
 }  // extern "C"

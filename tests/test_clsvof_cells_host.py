@@ -31,6 +31,9 @@ def cells():
     L = C.CDLL(LIB)
     L.hc_face_velocity.argtypes = [C.c_int] * 4 + [C.c_double] * 11 + [DP] * 3
     L.hc_vof_wlic.argtypes = [C.c_int] * 4 + [C.c_double] * 4 + [DP] * 8 + [C.c_int]
+    L.hc_vof_wlic_n.argtypes = [C.c_int] * 4 + [C.c_double] * 4 + [DP] * 8 + [C.c_int] * 2
+    BP = C.POINTER(C.c_ubyte)
+    L.hc_cut_line_flags.argtypes = [DP, C.c_int, C.c_int, C.c_int, BP, BP, BP]
     return L
 
 
@@ -98,3 +101,81 @@ def test_wlic_passes_bitwise(cells, dim, shape):
         assert cut > 50
         assert np.array_equal(got, want), (step, np.max(np.abs(got - want)))
         assert np.array_equal(got_stencil, want), (step, np.max(np.abs(got_stencil - want)))
+
+
+def bptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_ubyte))
+
+
+def pass_inputs(cells, o):
+    """vof as each WLIC pass of one vof_wlic reads it (before the x, y, z passes)."""
+    p = o.p
+    o.surface_norms()
+    args = [interior(o, n) for n in ("VOF_OLD", "UF", "VF", "WF", "NX", "NY")]
+    n2 = interior(o, "NZ") if p.dim == 3 else None
+    out = []
+    for npass in range(p.dim):
+        v = interior(o, "VOF").copy()
+        cells.hc_vof_wlic_n(p.dim, p.nx, p.ny, p.nz, p.dt, p.dx, p.dy, p.dz, ptr(v), *(ptr(a) for a in args),
+                            ptr(n2) if n2 is not None else None, 0, npass)
+        out.append(v)
+    return out
+
+
+def cut_line_flags(cells, o):
+    p = o.p
+    v = interior(o, "VOF")
+    fx = np.zeros((p.nz, p.ny), np.uint8)
+    fy = np.zeros((p.nz, p.nx), np.uint8)
+    fz = np.zeros((p.ny, p.nx), np.uint8)
+    cells.hc_cut_line_flags(ptr(v), p.nx, p.ny, p.nz, bptr(fx), bptr(fy), bptr(fz))
+    return fx, fy, fz
+
+
+def box_case(n):
+    """An axis-aligned box of VOF 1 with faces on cell faces (no cut cell at
+    the start: the first pass makes them) and its signed distance."""
+    o = O.Oracle(O.make_params(n, dim=3))
+    c = (np.arange(n + 6) - 2.5) / n
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    lo, hi = 0.3, 0.6
+    d = np.maximum.reduce([lo - x, x - hi, lo - y, y - hi, lo - z, z - hi])
+    o.field("PHI")[:] = -d
+    o.field("VOF")[:] = (d < 0).astype(float)
+    return o
+
+
+@pytest.mark.parametrize("case", ["sphere", "sphere_moved", "box", "circle_2d"])
+def test_cut_lines_cover_every_cut_cell(cells, case):
+    """The lines that k_cut_line_flags lists (wlic_may_cut on the vof at the
+    step's start) hold every cell that any of the step's passes finds cut, so
+    the normals solved on them are all the fluxes read."""
+    if case == "box":
+        o = box_case(24)
+    else:
+        dim = 2 if case == "circle_2d" else 3
+        o = O.Oracle(O.make_params(40 if dim == 2 else 24, dim=dim))
+        o.initial_phi(2 if dim == 3 else 1, 1)
+        o.store_initial()
+        if case == "sphere_moved":
+            for _ in range(3):
+                o.clsvof_step(2, 3.0, 1)
+    p = o.p
+    field = 2 if p.dim == 3 else 1
+    for t in (0.1, 1.2):   # two velocity fields: the flow reverses in time
+        o.velocity_faces(field, t, 3.0)
+        o.field("VOF_OLD")[:] = o.field("VOF")
+        fx, fy, fz = cut_line_flags(cells, o)
+        ncut = 0
+        for v in pass_inputs(cells, o):
+            cut = (v >= 1e-10) & (v <= 1 - 1e-10)   # not (v > 1 - eps or v < eps)
+            kk, jj, ii = np.nonzero(cut)
+            ncut += kk.size
+            assert np.all(fx[kk, jj] == 1) and np.all(fy[kk, ii] == 1)
+            if p.dim == 3:
+                assert np.all(fz[jj, ii] == 1)
+        assert ncut > 0
+        listed = fx.sum() + fy.sum() + (fz.sum() if p.dim == 3 else 0)
+        total = p.ny * p.nz + p.nx * p.nz + (p.nx * p.ny if p.dim == 3 else 0)
+        assert listed < total   # fewer line solves than every line
+
