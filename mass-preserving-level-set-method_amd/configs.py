@@ -19,8 +19,9 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(HERE, "lib", "lsmp_driver")
 
-# name: (driver case, cells per unit length, stop time, dt, pseudo dt) -- dt and
-# pseudo dt in units of dx, reinitialisation every step, MPLS.
+# name: (driver case, cells per unit length, stop time, dt, pseudo dt[, method,
+# reinitialisation interval]) -- dt and pseudo dt in units of dx; by default
+# MPLS with reinitialisation every step.
 CONFIGS = {
     "disk200": ("disk", 200, 1.0, 0.1, 0.5),         # Zalesak disk, one revolution (t2s = 1)
     "vortex2d256": ("vortex2d", 256, 8.0, 0.1, 0.1),  # single vortex, T = 8
@@ -28,10 +29,13 @@ CONFIGS = {
     "vortex2d1024": ("vortex2d", 1024, 8.0, 0.1, 0.1),
     "vortex128": ("vortex", 128, 3.0, 0.1, 0.1),      # 3D LeVeque deformation, T = 3
     "vortex256": ("vortex", 256, 3.0, 0.1, 0.1),
+    # the reference's own Vortex deck (Vortex/input/default.txt): CLSVOF at 64
+    # cells, advection CFL 0.5, no reconstruction before T (t2red = -1)
+    "vortex64_clsvof": ("vortex", 64, 3.0, 0.5, 0.1, 3, -1),
 }
 
 DECK = """# interface method: 1 level set, 2 mass-preserving level set, 3 CLSVOF
-2
+{method}
 # case name (output file prefix)
 {name}
 # subdomain grid gx gy gz
@@ -53,7 +57,7 @@ DECK = """# interface method: 1 level set, 2 mass-preserving level set, 3 CLSVOF
 # z range
 0.0 1.0
 # stop time, plot interval, reinit interval (steps)
-{t2s} 0.5 1
+{t2s} 0.5 {t2red}
 # dt and pseudo dt, in units of dx
 {dt} {rdt}
 # pressure solver: tolerance exponent, w1, w2, valve
@@ -94,16 +98,18 @@ DECK = """# interface method: 1 level set, 2 mass-preserving level set, 3 CLSVOF
 
 FINISHED = re.compile(r"finished: steps\s+(\d+)\s+L1\s+(\S+)\s+max \|dM\|/M\s+(\S+)")
 WALL = re.compile(r"wall ms/step\s+(\S+)\s+cell-updates/s\s+(\S+)")
+VOF = re.compile(r"VOF: L1\s+(\S+)\s+max \|dM\|/M\s+(\S+)")
 
 
-def run(config, precision="fp64", scheme="crweno", max_steps=0, log=None):
-    case, n, t2s, dt, rdt = CONFIGS[config]
+def run(config, precision="fp64", scheme="crweno", max_steps=0, log=None, driver=DRIVER):
+    case, n, t2s, dt, rdt, method, t2red = (CONFIGS[config] + (2, 1))[:7]
     with tempfile.TemporaryDirectory() as d:
         deck = os.path.join(d, "deck.txt")
         with open(deck, "w") as f:
-            f.write(DECK.format(name=f"{config}_{precision}", n=n, t2s=t2s, dt=dt, rdt=rdt))
+            f.write(DECK.format(name=f"{config}_{precision}", n=n, t2s=t2s, dt=dt, rdt=rdt, method=method,
+                                t2red=t2red))
         # the driver reports every 5 steps (output.f90:14-31); pass on every 1000th step
-        proc = subprocess.Popen([DRIVER, case, deck, str(max_steps), precision, scheme], cwd=d,
+        proc = subprocess.Popen([driver, case, deck, str(max_steps), precision, scheme], cwd=d,
                                 stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
         tail = []
         step_line = re.compile(r"^\S+,\s*(\d+),")
@@ -119,9 +125,12 @@ def run(config, precision="fp64", scheme="crweno", max_steps=0, log=None):
     m, w = FINISHED.search(text), WALL.search(text)
     if rc != 0 or not m or not w:
         raise RuntimeError(f"{config} {precision} failed ({rc}):\n{text[-2000:]}")
-    res = dict(config=config, case=case, n=n, precision=precision, scheme=scheme, steps=int(m.group(1)),
-               l1=float(m.group(2)), max_dM_over_M=float(m.group(3)), ms_per_step=float(w.group(1)),
-               cell_updates_per_s=float(w.group(2)))
+    res = dict(config=config, case=case, n=n, method=method, precision=precision, scheme=scheme,
+               steps=int(m.group(1)), l1=float(m.group(2)), max_dM_over_M=float(m.group(3)),
+               ms_per_step=float(w.group(1)), cell_updates_per_s=float(w.group(2)))
+    v = VOF.search(text)
+    if method == 3 and v:
+        res.update(vof_l1=float(v.group(1)), vof_max_dM_over_M=float(v.group(2)))
     if log:
         log(json.dumps(res))
     return res

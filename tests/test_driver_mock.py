@@ -7,6 +7,7 @@ a Fortran compiler."""
 import os
 import shutil
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -71,3 +72,21 @@ def test_clsvof_order_and_outputs(driver, tmp_path, case, deck):
     name = [f for f in os.listdir(tmp_path / "out") if f.endswith("_MassLoss.plt")][0]
     rows = np.loadtxt(tmp_path / "out" / name, skiprows=1)
     assert rows.shape == (3, 5)
+
+
+@pytest.mark.parametrize("config, method", [("vortex64_clsvof", 3), ("vortex128", 2), ("disk200", 2)])
+def test_config_decks_run_through_the_driver(driver, tmp_path, monkeypatch, config, method):
+    """configs.py's decks parse in the driver; the reference's own Vortex deck
+    (CLSVOF, t2red = -1) runs without a reconstruction."""
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "mass-preserving-level-set-method_amd"))
+    import configs
+    log = tmp_path / "calls.log"
+    monkeypatch.setenv("LSMP_MOCK_LOG", str(log))
+    res = configs.run(config, "fp64", max_steps=3, driver=driver)
+    calls = log.read_text().split()
+    assert res["steps"] == 3 and res["method"] == method
+    if method == 3:
+        assert "vof_l1" in res
+        assert calls.count("lsmp_vof_wlic") == 3 and "lsmp_clsvof_recon" not in calls
+    else:
+        assert calls.count("lsmp_mass_fix") == 3
