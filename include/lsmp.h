@@ -299,10 +299,19 @@ int lsmp_net_eval_grad(lsmp_net_handle net, const float *x, const float *y, int 
  *   LSMP_FALLBACK_CONTINUE -- plain pseudo-steps continue from the current
  *                             field, `sweeps` at a time, until the residual
  *                             passes, the plain stop (max change <= 1e-6) is
- *                             met or the plain pseudo-time budget is spent.
+ *                             met or the plain pseudo-time budget is spent;
+ *   LSMP_FALLBACK_CONTINUE_RELATIVE -- as CONTINUE, and the continuation also
+ *                             stops once a group of `sweeps` pseudo-steps
+ *                             lowers the residual by less than 10 %: the
+ *                             plain method has then reached what it can
+ *                             (thin sheets keep a residual above any fixed
+ *                             tolerance).
+ * net = NULL runs the same sequence without the network pass (the control
+ * that tells the network's share of the result from the sweeps').
  * *residual is the final field's residual, *n_pseudo all pseudo-steps run. */
 #define LSMP_FALLBACK_RESTART 0
 #define LSMP_FALLBACK_CONTINUE 1
+#define LSMP_FALLBACK_CONTINUE_RELATIVE 2
 int lsmp_reinit_learned(lsmp_handle h, lsmp_net_handle net, double band_cells, int sweeps, double residual_tol,
                         double residual_band, int fallback_mode, double *residual, int *fallback, int *n_pseudo);
 /* The band residual mean | |grad phi| - 1 | over |phi| < band_cells dx, with the

@@ -37,11 +37,22 @@ def _run(cmd, cwd=None):
 
 
 def build_library(verbose=False):
+    """Each translation unit compiles in its own hipcc process (in parallel),
+    then one link step makes the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(LIB_DIR, exist_ok=True)
+    obj_dir = os.path.join(LIB_DIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(obj_dir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        outs = list(ex.map(lambda so: _run([HIPCC] + cflags + ["-c", os.path.join(CSRC, so[0]), "-o", so[1]]),
+                           zip(SOURCES, objs)))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + ["-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
-    out = _run(cmd)
+    outs.append(_run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs))
     os.replace(tmp, LIB)
+    out = "\n".join(o for o in outs if o.strip())
     if verbose and out.strip():
         print(out)
     return LIB

@@ -805,7 +805,8 @@ int reinit_learned(lsmp_ctx *c, lsmp_net *net, double band_cells, int sweeps, do
     NET_HIP(hipGetLastError());
     if (mode == LSMP_FALLBACK_RESTART)
         NET_HIP(hipMemcpyAsync(backup, phi, n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
-    int rc = apply<T>(c, net, band_cells, nullptr);
+    // net == nullptr: the control without the network pass
+    int rc = net ? apply<T>(c, net, band_cells, nullptr) : LSMP_OK;
     if (rc) return rc;
     double err = 0.0;
     int np = 0;
@@ -830,15 +831,20 @@ int reinit_learned(lsmp_ctx *c, lsmp_net *net, double band_cells, int sweeps, do
             // more plain pseudo-steps from the current field, `sweeps` at a time,
             // until the residual passes, the plain method's own stop (max change
             // <= 1e-6) is met, or its pseudo-time budget is spent
+            // (CONTINUE_RELATIVE also stops when a group lowers the residual by
+            // less than 10 %: what the plain method can reach is reached)
             int budget = plain_budget(c) - np;
-            while (budget > 0 && !(res <= tol) && err > 1.0e-6) {
+            bool progress = true;
+            while (budget > 0 && !(res <= tol) && err > 1.0e-6 && progress) {
                 int np2 = 0;
                 rc = lsmp_reinit_run(c, 1, -1.0, sweeps < budget ? sweeps : budget, false, &err, &np2);
                 if (rc) return rc;
                 np += np2;
                 budget -= np2;
+                const double prev = res;
                 rc = residual<T>(c, res_band, &res);
                 if (rc) return rc;
+                if (mode == LSMP_FALLBACK_CONTINUE_RELATIVE) progress = res < 0.9 * prev;
             }
         }
     }
@@ -989,8 +995,10 @@ int lsmp_net_eval_grad(lsmp_net_handle net, const float *x, const float *y, int 
 int lsmp_reinit_learned(lsmp_handle h, lsmp_net_handle net, double band_cells, int sweeps, double residual_tol,
                         double residual_band, int fallback_mode, double *residual_out, int *fallback, int *n_pseudo)
 {
-    if (!h || !net || !(band_cells > 0) || sweeps < 1 || !(residual_band > 0)) return LSMP_E_INVALID;
-    if (fallback_mode != LSMP_FALLBACK_RESTART && fallback_mode != LSMP_FALLBACK_CONTINUE) return LSMP_E_INVALID;
+    if (!h || !(band_cells > 0) || sweeps < 1 || !(residual_band > 0)) return LSMP_E_INVALID;
+    if (fallback_mode != LSMP_FALLBACK_RESTART && fallback_mode != LSMP_FALLBACK_CONTINUE &&
+        fallback_mode != LSMP_FALLBACK_CONTINUE_RELATIVE)
+        return LSMP_E_INVALID;
     if (h->dim == 3 && (h->p.dy != h->p.dx || h->p.dz != h->p.dx)) return LSMP_E_INVALID;
     double r = 0.0;
     int fb = 0, np = 0;

@@ -16,7 +16,8 @@ private
 integer(c_int), parameter, public :: LSMP_FP64 = 0, LSMP_FP32 = 1
 integer(c_int), parameter, public :: LSMP_VEL_ROTATION = 0, LSMP_VEL_VORTEX2D = 1, LSMP_VEL_LEVEQUE3D = 2
 integer(c_int), parameter, public :: LSMP_SCHEME_CRWENO = 0, LSMP_SCHEME_UCCD = 1
-integer(c_int), parameter, public :: LSMP_FALLBACK_RESTART = 0, LSMP_FALLBACK_CONTINUE = 1
+integer(c_int), parameter, public :: LSMP_FALLBACK_RESTART = 0, LSMP_FALLBACK_CONTINUE = 1, &
+    LSMP_FALLBACK_CONTINUE_RELATIVE = 2
 
 type, bind(C), public :: lsmp_params
     integer(c_int) :: nx, ny, nz, ghc, dim

@@ -1,12 +1,15 @@
 """The 3D LeVeque deformation run to T (default 3) on the GPU, in any form:
-FP64 plain, FP32 bulk plain, or FP32 bulk with the learned reinitialisation.
+FP64 plain, FP32 bulk plain, FP32 bulk with the learned reinitialisation, or
+its control without the network (fp32-sweeps: the same sweeps, residual test
+and continuation, no network pass).
 
 Case: sphere r = 0.15 at (0.35, 0.35, 0.35) in the unit cube, velocity of
 Vortex/src/setup/velocity.f90 with t2s = T, dt = rdt = 0.1 dx, ls_wid =
 1.5 dx, MPLS every step (Vortex/src/main.f90 loop with t2red = 1).  Prints
 one JSON line: ms/step, cell-updates/s, L1 shape error (output.f90 metric),
 |dM|/M, pseudo-steps per step and fallbacks of the learned form, and the same
-timings over steps 1-50 and over 25 steps around t = T/2.
+timings (with the L1 and fallbacks at the window's end) over steps 1-50 and
+over 25 steps around t = T/2.
 
     python mass-preserving-level-set-method_amd/leveque.py --n 256 --form fp32-learned
 """
@@ -23,7 +26,7 @@ if HERE not in sys.path:
     sys.path.insert(0, HERE)
 import lsmp  # noqa: E402
 
-FORMS = ("fp64-plain", "fp32-plain", "fp32-learned", "fp64-learned")
+FORMS = ("fp64-plain", "fp32-plain", "fp32-learned", "fp64-learned", "fp32-sweeps", "fp64-sweeps")
 
 
 def sphere_box(n):
@@ -41,6 +44,7 @@ def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsm
         log=None, narrow_band=0.0):
     prec = lsmp.FP64 if form.startswith("fp64") else lsmp.FP32
     learned = form.endswith("learned")
+    sweeps_only = form.endswith("sweeps")
     net = None
     if learned:
         net = lsmp.Net()
@@ -67,11 +71,11 @@ def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsm
         for name, (a, b) in windows.items():
             if it + 1 == a:
                 s.synchronize()
-                marks[name] = (time.perf_counter(), pseudo)
+                marks[name] = (time.perf_counter(), pseudo, fallbacks)
         t += dt
         s.set_velocity_analytic(lsmp.VEL_LEVEQUE3D, t, T)
         s.rk3_advect()
-        if learned:
+        if learned or sweeps_only:
             _, fb, k = s.reinit_learned(net, 6.0, sweeps, tol, 1.5, fallback)
             fallbacks += fb
         else:
@@ -81,11 +85,12 @@ def run(n, form, T=3.0, frac=1.0, weights=None, tol=1e-2, sweeps=2, fallback=lsm
         for name, (a, b) in windows.items():
             if it + 1 == b:
                 s.synchronize()
-                w0, p0 = marks[name]
+                w0, p0, f0 = marks[name]
                 el_w = time.perf_counter() - w0
                 wres[name] = dict(steps=b - a + 1, ms_per_step=el_w / (b - a + 1) * 1e3,
                                   cell_updates_per_s=n ** 3 * (b - a + 1) / el_w,
-                                  pseudo_per_step=(pseudo - p0) / (b - a + 1))
+                                  pseudo_per_step=(pseudo - p0) / (b - a + 1), fallbacks=fallbacks - f0,
+                                  l1_at_end=s.interface_l1(), t_end=t)
         if log and (it + 1) % max(1, nsteps // 10) == 0:
             log(f"  {form} {n}^3 step {it + 1}/{nsteps} t={t:.3f}")
     s.synchronize()
@@ -108,10 +113,13 @@ def main(argv=None):
     ap.add_argument("--frac", type=float, default=1.0, help="run this fraction of T")
     ap.add_argument("--weights", default=None)
     ap.add_argument("--tol", type=float, default=1e-2)
+    ap.add_argument("--fallback", choices=["continue", "relative"], default="continue",
+                    help="continuation until the residual passes (continue) or also until it stops falling (relative)")
     ap.add_argument("--narrow-band", type=float, default=0.0,
                     help="advect and reinitialise only where |phi| < this many cells (0: everywhere)")
     args = ap.parse_args(argv)
-    r = run(args.n, args.form, T=args.T, frac=args.frac, weights=args.weights, tol=args.tol,
+    fb = lsmp.FALLBACK_CONTINUE if args.fallback == "continue" else lsmp.FALLBACK_CONTINUE_RELATIVE
+    r = run(args.n, args.form, T=args.T, frac=args.frac, weights=args.weights, tol=args.tol, fallback=fb,
             log=lambda m: print(m, file=sys.stderr, flush=True), narrow_band=args.narrow_band)
     print(json.dumps(r), flush=True)
     return 0

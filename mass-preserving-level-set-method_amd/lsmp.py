@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("LSMP_LIBRARY") or os.path.join(HERE, "lib", "liblsmp.so")
 
 FP64, FP32 = 0, 1
-FALLBACK_RESTART, FALLBACK_CONTINUE = 0, 1
+FALLBACK_RESTART, FALLBACK_CONTINUE, FALLBACK_CONTINUE_RELATIVE = 0, 1, 2
 SCHEME_CRWENO, SCHEME_UCCD = 0, 1
 VEL_ROTATION, VEL_VORTEX2D, VEL_LEVEQUE3D = 0, 1, 2
 
@@ -313,11 +313,14 @@ class Solver:
         """Learned reinitialisation (no reference counterpart): network on the
         band, `sweeps` plain pseudo-steps, plain fallback (restart from the
         input, or continuation) when the residual over |phi| < residual_band
-        dx exceeds residual_tol.  Returns (residual, fallback_taken, n_pseudo)."""
+        dx exceeds residual_tol.  net=None runs the same sequence without the
+        network pass (the sweeps-only control).  Returns (residual,
+        fallback_taken, n_pseudo)."""
         r = C.c_double()
         fb = C.c_int()
         n = C.c_int()
-        _check(self.lib.lsmp_reinit_learned(self.h, net.h, band_cells, sweeps, residual_tol, residual_band,
+        _check(self.lib.lsmp_reinit_learned(self.h, net.h if net is not None else None, band_cells, sweeps,
+                                            residual_tol, residual_band,
                                             int(fallback), C.byref(r), C.byref(fb), C.byref(n)))
         return r.value, bool(fb.value), n.value
 

@@ -98,13 +98,14 @@ def interface_masses(s, cells, ls_wid):
 
 
 def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.01, weights=None, log=print,
-        narrow_band=0.0):
+        narrow_band=0.0, fallback=lsmp.FALLBACK_CONTINUE, tol=1e-2):
     """The case at n^3.  Two interfaces are told apart only while their
     surfaces stay more than 2 ls_wid + dx = 3.5 dx apart (the regions
     phi > -ls_wid then share no face); the default gap 0.01 clears that at
     n >= 350."""
     prec = lsmp.FP64 if form.startswith("fp64") else lsmp.FP32
     learned = form.endswith("learned")
+    sweeps_only = form.endswith("sweeps")
     centres, radii = seeded_spheres(count, gap=gap, seed=seed)
     t0 = time.time()
     box = spheres_box(n, centres, radii)
@@ -119,7 +120,7 @@ def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.
     del box
     s.store_phi0()
     imass, _ = s.mass_vol()
-    m0, comp0, _ = interface_masses(s, point_cells(centres, n), s.params.ls_wid)
+    m0, comp0, all0 = interface_masses(s, point_cells(centres, n), s.params.ls_wid)
     pts = centres.copy()
     dt = s.params.dt
     nsteps = int(round(T / dt * frac))
@@ -131,8 +132,8 @@ def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.
         t += dt
         s.set_velocity_analytic(lsmp.VEL_LEVEQUE3D, t, T)
         s.rk3_advect()
-        if learned:
-            _, fb, k = s.reinit_learned(net, 6.0, 2, 1e-2, 1.5, lsmp.FALLBACK_CONTINUE)
+        if learned or sweeps_only:
+            _, fb, k = s.reinit_learned(net, 6.0, 2, tol, 1.5, fallback)
             fallbacks += fb
         else:
             _, k = s.reinit(1)
@@ -153,13 +154,14 @@ def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.
     rel = np.abs(m1 - m0) / m0
     # components holding no sphere centre: pieces broken off, or islands
     frag = np.setdiff1d(np.arange(all1.size), comp1[comp1 >= 0])
-    out = dict(form=form, narrow_band_cells=narrow_band, n=n, spheres=count, T=T, t_end=t, steps=nsteps, ms_per_step=el / nsteps * 1e3,
+    out = dict(form=form, fallback_mode=int(fallback), narrow_band_cells=narrow_band, n=n, spheres=count, T=T, t_end=t, steps=nsteps, ms_per_step=el / nsteps * 1e3,
                cell_updates_per_s=n ** 3 * nsteps / el, l1=s.interface_l1(), dM_over_M=abs(mass - imass) / imass,
                components_start=int(len(set(comp0.tolist()) - {-1})),
                components_end=int(len(set(comp1.tolist()) - {-1})), components_all_end=int(all1.size),
                spheres_compared=int(alone.sum()),
                interface_dM_over_M_max=float(np.max(rel[alone])) if alone.any() else None,
                interface_dM_over_M_mean=float(np.mean(rel[alone])) if alone.any() else None,
+               mass_in_components_start=float(all0.sum() / imass), components_all_start=int(all0.size),
                mass_in_components_end=float(all1.sum() / mass),
                fragments_end=int(frag.size), fragment_mass_end=float(all1[frag].sum() / mass),
                fragment_mass_max_end=float(all1[frag].max() / mass) if frag.size else 0.0,
@@ -174,15 +176,19 @@ def run(n=512, form="fp32-learned", T=3.0, frac=1.0, count=64, seed=2026, gap=0.
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--form", choices=["fp64-plain", "fp32-plain", "fp32-learned"], default="fp32-learned")
+    ap.add_argument("--form", choices=["fp64-plain", "fp32-plain", "fp32-learned", "fp32-sweeps"],
+                    default="fp32-learned")
+    ap.add_argument("--fallback", choices=["continue", "relative"], default="continue",
+                    help="continuation until the residual passes (continue) or also until it stops falling (relative)")
     ap.add_argument("--T", type=float, default=3.0)
     ap.add_argument("--frac", type=float, default=1.0)
     ap.add_argument("--spheres", type=int, default=64)
     ap.add_argument("--narrow-band", type=float, default=0.0,
                     help="advect and reinitialise only where |phi| < this many cells (0: everywhere)")
     args = ap.parse_args(argv)
+    fb = lsmp.FALLBACK_CONTINUE if args.fallback == "continue" else lsmp.FALLBACK_CONTINUE_RELATIVE
     r = run(args.n, args.form, args.T, args.frac, args.spheres, log=lambda m: print(m, file=sys.stderr, flush=True),
-            narrow_band=args.narrow_band)
+            narrow_band=args.narrow_band, fallback=fb)
     print(json.dumps(r), flush=True)
     return 0
 
