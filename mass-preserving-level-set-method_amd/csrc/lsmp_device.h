@@ -362,6 +362,12 @@ template <int NV> LSMP_DEV void block_sum(double (&v)[NV])
     __syncthreads();
 }
 
+// max of non-negative doubles through their bit patterns (ordered as integers)
+LSMP_DEV void atomic_max_nonneg(unsigned long long *p, double v)
+{
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
 LSMP_DEV double block_max(double v)
 {
     __shared__ double red[16];

@@ -15,12 +15,6 @@ namespace lsmp {
 // ---------------------------------------------------------------------------
 template <typename T> LSMP_DEV double to_d(T v) { return (double)v; }
 
-// max of non-negative doubles through their bit patterns (monotone for >= 0)
-LSMP_DEV void atomic_max_nonneg(unsigned long long *p, double v)
-{
-    atomicMax(p, (unsigned long long)__double_as_longlong(v));
-}
-
 __global__ void k_zero_u64(unsigned long long *p, int n)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
