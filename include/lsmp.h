@@ -113,6 +113,23 @@ int lsmp_reinit(lsmp_handle h, int btn, double timestop, double *red_error, int 
 /* LS mass and volume sums of the current phi (ns_solver/others.f90:172-213). */
 int lsmp_mass_vol(lsmp_handle h, double *mass, double *vol);
 
+/* The sums of lsmp_mass_vol on a caller's device field, allocating nothing.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory or synchronise (code under HIP graph capture, tight serving loops).
+ *   phi     device FP64 interior cells, (k*ny + j)*nx + i (nz = 1 in 2D)
+ *   dx..dz  cell sizes (2D: dz = 1, tree.f90:252); ls_wid, rho_12, inverse
+ *           as in lsmp_params
+ *   out     device double[2] <- (mass, vol), stream-ordered (no host wait)
+ *   stream  hipStream_t, NULL for the default stream
+ * Results equal lsmp_mass_vol's on the same field bit for bit.  Scratch: a
+ * static device array of LSMP_NOALLOC_PARTIALS partial triples (24 KiB) per
+ * GPU, held until the process ends; its slot count caps the reduction's
+ * block count, so every grid fits.  Calls that overlap on one GPU share that
+ * array and race. */
+#define LSMP_NOALLOC_PARTIALS 1024
+int lsmp_mass_vol_noalloc(const double *phi, int nx, int ny, int nz, int dim, double dx, double dy, double dz,
+                          double ls_wid, double rho_12, int inverse, double *out, void *stream);
+
 /* Global mass correction with target imass (mass_preserving_level_set.f90:1-74).
  * mass receives the mass measured at the start of the last iteration (the
  * value the exit test used), n_iter the iterations run. */

@@ -24,7 +24,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-SOURCES = ["lsmp_kernels.hip", "lsmp_host.hip", "lsmp_learn.hip", "lsmp_metrics.hip", "lsmp_clsvof.hip"]
+SOURCES = ["lsmp_kernels.hip", "lsmp_host.hip", "lsmp_learn.hip", "lsmp_metrics.hip", "lsmp_clsvof.hip",
+           "lsmp_noalloc.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -92,6 +92,15 @@ interface
         type(c_ptr), value :: h
         real(c_double), intent(out) :: mass, vol
     end function
+    ! no counterpart in the reference: the sums of lsmp_mass_vol on a device
+    ! field, allocating nothing (for callers that cannot allocate)
+    integer(c_int) function c_lsmp_mass_vol_noalloc(phi, nx, ny, nz, dim, dx, dy, dz, ls_wid, rho_12, inverse, &
+                                                    out, stream) bind(C, name="lsmp_mass_vol_noalloc")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: phi, out, stream
+        integer(c_int), value :: nx, ny, nz, dim, inverse
+        real(c_double), value :: dx, dy, dz, ls_wid, rho_12
+    end function
     integer(c_int) function c_lsmp_mass_fix(h, imass, mass, n_iter) bind(C, name="lsmp_mass_fix")
         import :: c_int, c_ptr, c_double
         type(c_ptr), value :: h
@@ -334,7 +343,7 @@ end interface
 
 public :: c_lsmp_init, c_lsmp_finalize, c_lsmp_upload_phi, c_lsmp_download_phi, c_lsmp_store_phi0
 public :: c_lsmp_set_velocity, c_lsmp_set_velocity_analytic, c_lsmp_rk3_advect, c_lsmp_reinit
-public :: c_lsmp_mass_vol, c_lsmp_mass_fix, c_lsmp_interface_l1, c_lsmp_get_stats, c_lsmp_synchronize
+public :: c_lsmp_mass_vol, c_lsmp_mass_vol_noalloc, c_lsmp_mass_fix, c_lsmp_interface_l1, c_lsmp_get_stats, c_lsmp_synchronize
 public :: c_lsmp_set_advection_scheme, c_lsmp_set_band_lines, c_lsmp_band_line_counts, c_lsmp_set_narrow_band
 public :: c_lsmp_set_marker, c_lsmp_get_marker, c_lsmp_marker_load, c_lsmp_marker_store, c_lsmp_combine_markers
 public :: c_lsmp_set_profiling, c_lsmp_kernel_time, c_lsmp_component_masses

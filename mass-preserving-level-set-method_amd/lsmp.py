@@ -40,7 +40,7 @@ EXPORTS = [
     "lsmp_net_clear_samples", "lsmp_net_apply",
     "lsmp_upload_vof", "lsmp_download_vof", "lsmp_store_vof_old", "lsmp_set_face_velocity",
     "lsmp_set_face_velocity_analytic", "lsmp_vof_wlic", "lsmp_clsvof_recon", "lsmp_store_vof0",
-    "lsmp_vof_mass_vol", "lsmp_vof_interface_l1",
+    "lsmp_vof_mass_vol", "lsmp_vof_interface_l1", "lsmp_mass_vol_noalloc",
 ]
 NO_STATUS = ("lsmp_finalize", "lsmp_error_string", "lsmp_net_destroy")
 
@@ -135,6 +135,8 @@ def load_library(path=LIB_PATH):
     L.lsmp_store_vof0.argtypes = [H]
     L.lsmp_vof_mass_vol.argtypes = [H, _DP, _DP]
     L.lsmp_vof_interface_l1.argtypes = [H, _DP]
+    L.lsmp_mass_vol_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
     for name in EXPORTS:
         if name not in NO_STATUS:
             getattr(L, name).restype = C.c_int
@@ -464,6 +466,14 @@ class Solver:
             npseudo += self.reinit(1)[1]
             merged = True
         return merged, npseudo, nmass
+
+
+def mass_vol_noalloc(phi_ptr, nx, ny, nz, dim, dx, dy, dz, ls_wid, rho_12, inverse, out_ptr, stream=None):
+    """lsmp_mass_vol_noalloc on device pointers: the (mass, vol) of the FP64
+    interior field at phi_ptr into the device doubles at out_ptr, stream
+    ordered.  Allocates nothing."""
+    _check(load_library().lsmp_mass_vol_noalloc(phi_ptr, nx, ny, nz, dim, dx, dy, dz, ls_wid, rho_12, inverse,
+                                                out_ptr, stream))
 
 
 NF, NH = 128, 64
