@@ -131,6 +131,16 @@ LSMP_DEV float rcp_pos(float x)
     return fmaf(r, fmaf(-x, r, 1.0f), r);
 }
 
+// The same with one Newton step in FP64: the hardware estimate is good to
+// about 5e-8, one step to about 2e-15 of 1/x -- enough for the WENO-HJ
+// weights, which multiply a correction term that is itself small.
+LSMP_DEV double rcp_w(double x)
+{
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+LSMP_DEV float rcp_w(float x) { return rcp_pos(x); }
+
 // The same two rows with the weight divisions folded into one reciprocal:
 // with e_k = EPS + b_k and tau = |b3 - b1| the reference's
 // a_k = c_k (1 + tau / e_k) normalise to
@@ -227,7 +237,7 @@ template <typename T> LSMP_DEV T phyn_fast(T a, T b, T c, T d, T sbc, T scd)
     T is2 = T(13) * scd + T(3) * sq(T(3) * c - d);
     T q0 = sq(eps + is0), q1 = sq(eps + is1), q2 = sq(eps + is2);
     T q12 = q1 * q2, q01 = q0 * q1;
-    T r = rcp_pos(q12 + T(6) * (q0 * q2) + T(3) * q01);
+    T r = rcp_w(q12 + T(6) * (q0 * q2) + T(3) * q01);
     // w0/3 = q12 r/3 and (w2 - 1/2)/6 = q01 r/2 - 1/12
     T t0 = q12 * (r * T(1.0 / 3.0));
     T t2 = fma(q01, r * T(0.5), T(-1.0 / 12.0));
@@ -360,6 +370,65 @@ template <int NV> LSMP_DEV void block_sum(double (&v)[NV])
         }
     }
     __syncthreads();
+}
+
+// Neumaier-compensated FP64 summation: s carries the rounded running sum and
+// c the rounding errors of its additions (each recovered exactly), so s + c is
+// the sum to about one rounding whatever the number of terms.
+LSMP_DEV void csum_add(double &s, double &c, double x)
+{
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+// Block-wide compensated sum of NV (s, c) pairs (blockDim.x a multiple of 64,
+// at most 1024): pairs combine by adding their sums with csum_add and their
+// compensations plainly; thread 0 gets s[q] = s + c of the block.
+template <int NV> LSMP_DEV void block_sum_comp(double (&s)[NV], double (&c)[NV])
+{
+    __shared__ double red_s[16][NV], red_c[16][NV];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+        for (int o = 32; o > 0; o >>= 1) {
+            const double s2 = __shfl_xor(s[q], o, 64), c2 = __shfl_xor(c[q], o, 64);
+            c[q] += c2;
+            csum_add(s[q], c[q], s2);
+        }
+    if (lane == 0)
+        for (int q = 0; q < NV; ++q) {
+            red_s[wid][q] = s[q];
+            red_c[wid][q] = c[q];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < NV; ++q) {
+            double ss = red_s[0][q], cc = red_c[0][q];
+            for (int w = 1; w < nw; ++w) {
+                cc += red_c[w][q];
+                csum_add(ss, cc, red_s[w][q]);
+            }
+            s[q] = ss + cc;
+        }
+    }
+    __syncthreads();
+}
+
+// Compensated sum of the per-block partial triples partial[3 b + q], b < nblk,
+// over the whole block (slot b to thread b, b + blockDim.x, ...); the totals
+// land in acc of thread 0.  Fixed order: the result does not depend on timing.
+LSMP_DEV void sum_partials3(const double *__restrict__ partial, int nblk, double (&acc)[3])
+{
+    double cmp[3] = {0.0, 0.0, 0.0};
+    acc[0] = acc[1] = acc[2] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        csum_add(acc[0], cmp[0], partial[3 * b + 0]);
+        csum_add(acc[1], cmp[1], partial[3 * b + 1]);
+        csum_add(acc[2], cmp[2], partial[3 * b + 2]);
+    }
+    block_sum_comp<3>(acc, cmp);
 }
 
 // max of non-negative doubles through their bit patterns (ordered as integers)

@@ -406,6 +406,21 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
                                                   c->tiles)));
         tiles = c->tiles;
     }
+    RedisBand<T> bd;
+    if (c->dim == 3 && btn == 1) {
+        if (!c->rb_list) {
+            if (alloc(&c->rb_s2, c->ncell * c->esz) || alloc(&c->rb_s3, c->ncell * c->esz) ||
+                alloc(&c->rb_gr, c->ncell * c->esz) || alloc((void **)&c->rb_list, c->ncell * sizeof(unsigned)) ||
+                alloc((void **)&c->rb_count, 2 * sizeof(unsigned)))
+                return LSMP_E_HIP;
+            launch_zero_bytes(c->st, (unsigned char *)c->rb_count, 2 * sizeof(unsigned));
+            c->rb_par = 0;
+        }
+        bd.s2 = (T *)c->rb_s2;
+        bd.s3 = (T *)c->rb_s3;
+        bd.gr = (T *)c->rb_gr;
+        bd.list = c->rb_list;
+    }
     launch_redis_ctl(c->st, ctl, 0);
     int launched = 0;
     unsigned long long hc[4] = {0, 0, 0, 0};
@@ -415,9 +430,28 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
             T *cur = (T *)c->phi, *b = (T *)c->phiB, *cc = (T *)c->phiC;
             launch_redis_ctl(c->st, ctl, 1);
             LSMP_LAUNCHED();
+            if (c->dim == 3 && !std::getenv("LSMP_REDIS_OLD")) {
+                // stages 1..3: every cell (lambda = 0), then the band cells (lambda)
+                const T *in[3] = {cur, b, cc};
+                T *outp[3] = {b, cc, b};
+                for (int sg = 0; sg < 3; ++sg) {
+                    if (btn == 1) {
+                        bd.count = c->rb_count + c->rb_par;
+                        bd.next = c->rb_count + (1 - c->rb_par);
+                    }
+                    KT(c, "redis_stage",
+                       (launch_redis3d<T>(c->st, sg + 1, btn, in[sg], outp[sg], cur, S0, ra, ctl, tiles, bd, false)));
+                    if (btn == 1) {
+                        KT(c, "redis_band",
+                           (launch_redis3d<T>(c->st, sg + 1, btn, in[sg], outp[sg], cur, S0, ra, ctl, tiles, bd, true)));
+                        c->rb_par ^= 1;
+                    }
+                }
+            } else {
             KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 1, btn, cur, b, cur, S0, ra, ctl, tiles)));
             KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 2, btn, b, cc, cur, S0, ra, ctl, tiles)));
             KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 3, btn, cc, b, cur, S0, ra, ctl, tiles)));
+            }
             launch_redis_ctl(c->st, ctl, 2);
             // the new phi is in b; the old one becomes a spare
             c->phiB = c->phi;
@@ -719,6 +753,7 @@ void free_all(lsmp_ctx *c)
     void *bufs[] = {c->rctl, c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count,
                     c->mk[0], c->mk[1], c->mflag, c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2], c->tiles,
+                    c->rb_s2, c->rb_s3, c->rb_gr, c->rb_list, c->rb_count,
                     c->vof, c->vofB, c->vof_old, c->vkeep, c->vof0, c->uf, c->vf, c->wf, c->nrm[0], c->nrm[1], c->nrm[2], c->ftab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);

@@ -89,6 +89,20 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
 // narrow band: flags of the 3D reinitialisation workgroups with a cell |phi| < wband
 template <typename T>
 void launch_tube_tiles(hipStream_t st, const T *phi, int nx, int ny, int nz, double wband, unsigned char *tiles);
+// 3D reinitialisation stage (lsmp_redis.hip): band = false launches the pass
+// over every cell with lambda = 0, band = true (btn = 1 only) the pass over the
+// listed band cells with lambda.  The side fields hold s2, s3 and |grad| of the
+// band cells, list their cell indices (room for every cell); count is the
+// list's length for this stage (zero when the stage starts) and next the one
+// for the next stage, which the band pass clears.
+template <typename T> struct RedisBand {
+    T *s2 = nullptr, *s3 = nullptr, *gr = nullptr;
+    unsigned *list = nullptr, *count = nullptr, *next = nullptr;
+};
+template <typename T>
+void launch_redis3d(hipStream_t st, int stage, int btn, const T *phi, T *phi_out, const T *phi0, const T *S0,
+                    const RedisArgs &ra, unsigned long long *ctl, const unsigned char *tiles, const RedisBand<T> &bd,
+                    bool band);
 template <typename T>
 void launch_combine_markers(hipStream_t st, const T *m1, const T *m2, T *phi, size_t n, unsigned *flag);
 template <typename T>

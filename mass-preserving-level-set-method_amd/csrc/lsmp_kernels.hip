@@ -1165,7 +1165,7 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
         S = s;
         SS = ss;
     };
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
     auto emit = [&](int q, T d, T dxa, T dya, T pa) {
         const size_t a = base + (size_t)q * (size_t)estride;
         if (!FINAL) {
@@ -1181,14 +1181,14 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
             const double gg = (double)g;
             const double r12 = ca.rho_12;
             double rho, plam;
-            acc[0] += h * ca.dv;
+            csum_add(acc[0], cmp[0], h * ca.dv);
             if (!ca.inverse) rho = h + r12 * (1.0 - h);
             else rho = r12 * h + (1.0 - h);
-            acc[1] += rho * h * ca.dv;
+            csum_add(acc[1], cmp[1], rho * h * ca.dv);
             plam = dd * dd * gg;
             if (ca.inverse) plam = plam * (2.0 * (r12 - 1.0) * h + 1.0) * ca.dx * ca.dy * ca.dz;
             else plam = plam * (2.0 * (1.0 - r12) * h + r12) * ca.dx * ca.dy * ca.dz;
-            acc[2] += plam;
+            csum_add(acc[2], cmp[2], plam);
         }
     };
 
@@ -1275,7 +1275,7 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
         }
     }
     if (FINAL) {
-        block_sum<3>(acc);
+        block_sum_comp<3>(acc, cmp);
         if (threadIdx.x == 0) {
             partial[3 * blockIdx.x + 0] = acc[0];
             partial[3 * blockIdx.x + 1] = acc[1];
@@ -1443,7 +1443,7 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     const T lo0 = g00 * XS + g01 * XSS + w0, lo1 = g10 * XS + g11 * XSS + w1;   // x at this lane's first row
     const T nx0 = __shfl(lo0, (lane + 1) & 63, 64), nx1 = __shfl(lo1, (lane + 1) & 63, 64);
     // this lane's rows, top down, with the reference's arithmetic from x_{hi+1}
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
     T S = s == 31 ? XS : nx0, SS = s == 31 ? XSS : nx1;
 #pragma unroll
     for (int r = KC - 1; r >= 0; --r) {
@@ -1472,20 +1472,20 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
                     const double gg = (double)g;
                     const double r12 = ca.rho_12;
                     double rho, plam;
-                    acc[0] += h * ca.dv;
+                    csum_add(acc[0], cmp[0], h * ca.dv);
                     if (!ca.inverse) rho = h + r12 * (1.0 - h);
                     else rho = r12 * h + (1.0 - h);
-                    acc[1] += rho * h * ca.dv;
+                    csum_add(acc[1], cmp[1], rho * h * ca.dv);
                     plam = dd * dd * gg;
                     if (ca.inverse) plam = plam * (2.0 * (r12 - 1.0) * h + 1.0) * ca.dx * ca.dy * ca.dz;
                     else plam = plam * (2.0 * (1.0 - r12) * h + r12) * ca.dx * ca.dy * ca.dz;
-                    acc[2] += plam;
+                    csum_add(acc[2], cmp[2], plam);
                 }
             }
         }
     }
     if (FINAL) {
-        block_sum<3>(acc);
+        block_sum_comp<3>(acc, cmp);
         if (threadIdx.x == 0) {
             partial[3 * blockIdx.x + 0] = acc[0];
             partial[3 * blockIdx.x + 1] = acc[1];
@@ -1666,13 +1666,8 @@ __global__ __launch_bounds__(256) void k_uccd_seg(const T *__restrict__ phi, T *
 // scal: [0] vol, [1] mass, [2] denominator, [3] lambda, [4] flag (1 = empty band).
 __global__ void k_finalize3(const double *__restrict__ partial, int nblk, double imass, double *scal)
 {
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        acc[0] += partial[3 * b + 0];
-        acc[1] += partial[3 * b + 1];
-        acc[2] += partial[3 * b + 2];
-    }
-    block_sum<3>(acc);
+    double acc[3];
+    sum_partials3(partial, nblk, acc);
     if (threadIdx.x == 0) {
         scal[0] = acc[0];
         scal[1] = acc[1];
@@ -1825,7 +1820,7 @@ __global__ __launch_bounds__(256) void k_line_hsums(const T *__restrict__ phi, c
                                                     long estride, CcdArgs ca, double *__restrict__ partial)
 {
     const int L = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
     if (L < nlines && !lflag[L]) {
         const size_t base = (size_t)(L / lw) * (size_t)ostride + (size_t)(L % lw) * (size_t)istride;
         const double r12 = ca.rho_12;
@@ -1838,14 +1833,14 @@ __global__ __launch_bounds__(256) void k_line_hsums(const T *__restrict__ phi, c
                 if (q0 + r < n) {
                     double h, dd;
                     ls_funs<double>((double)v[r], ca.ls_wid, h, dd);
-                    acc[0] += h * ca.dv;
+                    csum_add(acc[0], cmp[0], h * ca.dv);
                     const double rho = ca.inverse ? r12 * h + (1.0 - h) : h + r12 * (1.0 - h);
-                    acc[1] += rho * h * ca.dv;
+                    csum_add(acc[1], cmp[1], rho * h * ca.dv);
                 }
             }
         }
     }
-    block_sum<3>(acc);
+    block_sum_comp<3>(acc, cmp);
     if (threadIdx.x == 0) {
         partial[3 * blockIdx.x + 0] = acc[0];
         partial[3 * blockIdx.x + 1] = acc[1];
@@ -1863,18 +1858,18 @@ __global__ void k_zero_bytes(unsigned char *__restrict__ p, size_t n)
 template <typename T>
 __global__ void k_mass_vol(const T *__restrict__ phi, size_t n, CcdArgs ca, double *__restrict__ partial)
 {
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         double h, D;
         ls_funs<double>((double)phi[q], ca.ls_wid, h, D);
         const double r12 = ca.rho_12;
         double rho;
-        acc[0] += h * ca.dv;
+        csum_add(acc[0], cmp[0], h * ca.dv);
         if (!ca.inverse) rho = h + r12 * (1.0 - h);
         else rho = r12 * h + (1.0 - h);
-        acc[1] += rho * h * ca.dv;
+        csum_add(acc[1], cmp[1], rho * h * ca.dv);
     }
-    block_sum<3>(acc);
+    block_sum_comp<3>(acc, cmp);
     if (threadIdx.x == 0) {
         partial[3 * blockIdx.x + 0] = acc[0];
         partial[3 * blockIdx.x + 1] = acc[1];
@@ -1887,13 +1882,13 @@ template <typename T>
 __global__ void k_l1(const T *__restrict__ phi, const T *__restrict__ phi0, size_t n, double ls_wid,
                      double *__restrict__ partial)
 {
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         double src = heavyside<double>((double)phi0[q] / ls_wid);
         double tgt = heavyside<double>((double)phi[q] / ls_wid);
-        acc[0] += fabs(src - tgt);
+        csum_add(acc[0], cmp[0], fabs(src - tgt));
     }
-    block_sum<3>(acc);
+    block_sum_comp<3>(acc, cmp);
     if (threadIdx.x == 0) {
         partial[3 * blockIdx.x + 0] = acc[0];
         partial[3 * blockIdx.x + 1] = 0.0;

@@ -12,6 +12,7 @@
 // D column = lane%16 (i = 0..3).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -90,17 +91,22 @@ template <typename T> LSMP_DEV float grad_norm4(const T *__restrict__ phi, const
 //   q = 126:  |grad phi| - 1 (fourth-order central differences);
 //   q = 127:  (phi_c / h) (1 / max(|grad phi|, 0.1) - 1), the change that
 //             the first-order distance estimate phi / |grad phi| would make.
+// gm: |grad phi| at the cell (grad_norm4), for features 126 and 127.
 template <typename T>
-LSMP_DEV float feature(const T *__restrict__ phi, const Geom &g, int i, int j, int k, T pc, int q)
+LSMP_DEV float feature(const T *__restrict__ phi, const Geom &g, int i, int j, int k, T pc, int q, float gm)
 {
     if (q < 125) {
         const int di = q % 5 - 2, dj = (q / 5) % 5 - 2, dk = q / 25 - 2;
         return (float)(load_clamped(phi, g, i + di, j + dj, k + dk) - pc) * g.inv_h;
     }
     if (q == 125) return (float)pc * g.inv_h;
-    const float gm = grad_norm4(phi, g, i, j, k);
     if (q == 126) return gm - 1.0f;
     return (float)pc * g.inv_h * (1.0f / fmaxf(gm, 0.1f) - 1.0f);
+}
+template <typename T>
+LSMP_DEV float feature(const T *__restrict__ phi, const Geom &g, int i, int j, int k, T pc, int q)
+{
+    return feature(phi, g, i, j, k, pc, q, q >= 126 ? grad_norm4(phi, g, i, j, k) : 1.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -559,7 +565,8 @@ __global__ void k_net_update(float *__restrict__ P, float *__restrict__ m, float
 // Inference on the band cells: phi_out = phi_c + h * net(features) * gate,
 // gate = clamp((|phi_c| - w) / w, 0, 1) with w = ls_wid: the network leaves
 // the interface region |phi| <= ls_wid to the plain sweeps and ramps in over
-// [ls_wid, 2 ls_wid].
+// [ls_wid, 2 ls_wid].  Persistent workgroups: each stages the weights into LDS
+// once and then takes tiles of TILE band cells blockIdx.x, + gridDim.x, ...
 template <typename T>
 __global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                    const int *__restrict__ list, int nlist, Geom g,
@@ -572,34 +579,40 @@ __global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T 
     stage_weights(L, P, nullptr);
     __syncthreads();
     const int t = threadIdx.x, wv = t >> 6, l = t & 63, lr = l & 15, lg = l >> 4;
-    const int c = blockIdx.x * TILE + wv * 16 + lr;
-    const bool valid = c < nlist;
-    int idx = 0, i = 0, j = 0, k = 0;
-    T pc = T(0);
-    if (valid) {
-        idx = list[c];
-        i = idx % g.nx; j = (idx / g.nx) % g.ny; k = idx / (g.nx * g.ny);
-        pc = phi[idx];
-    }
-    bf16x8 xh[4], xl[4];
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-        for (int jj = 0; jj < 8; ++jj) {
-            __bf16 hi, lo;
-            split_bf16(valid ? feature(phi, g, i, j, k, pc, 32 * ks + 8 * lg + jj) : 0.0f, hi, lo);
-            xh[ks][jj] = hi;
-            xl[ks][jj] = lo;
+    const int ntiles = (nlist + TILE - 1) / TILE;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int c = tile * TILE + wv * 16 + lr;
+        const bool valid = c < nlist;
+        int idx = 0, i = 0, j = 0, k = 0;
+        T pc = T(0);
+        if (valid) {
+            idx = list[c];
+            i = idx % g.nx; j = (idx / g.nx) % g.ny; k = idx / (g.nx * g.ny);
+            pc = phi[idx];
         }
-    f32x4 z1[4], z2[4];
-    float o[4];
-    net_forward(L, xh, xl, sh1[wv], sl1[wv], z1, z2, o);
-    if (lr == 0)
-        for (int q = 0; q < 4; ++q) sout[wv][4 * lg + q] = o[q];
-    __syncthreads();
-    if (valid && lg == 0) {
-        const T gate = fmin(fmax((fabs(pc) - w) / w, T(0)), T(1));
-        phi_out[idx] = pc + h * (T)sout[wv][lr] * gate;
+        // |grad phi| once per lane (features 126 and 127 of the lanes lg = 3)
+        const float gm = valid && lg == 3 ? grad_norm4(phi, g, i, j, k) : 1.0f;
+        bf16x8 xh[4], xl[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) {
+                __bf16 hi, lo;
+                split_bf16(valid ? feature(phi, g, i, j, k, pc, 32 * ks + 8 * lg + jj, gm) : 0.0f, hi, lo);
+                xh[ks][jj] = hi;
+                xl[ks][jj] = lo;
+            }
+        f32x4 z1[4], z2[4];
+        float o[4];
+        net_forward(L, xh, xl, sh1[wv], sl1[wv], z1, z2, o);
+        if (lr == 0)
+            for (int q = 0; q < 4; ++q) sout[wv][4 * lg + q] = o[q];
+        __syncthreads();
+        if (valid && lg == 0) {
+            const T gate = fmin(fmax((fabs(pc) - w) / w, T(0)), T(1));
+            phi_out[idx] = pc + h * (T)sout[wv][lr] * gate;
+        }
+        __syncthreads();   // sout and the wave's H1 tiles are reused by the next tile
     }
 }
 
@@ -769,8 +782,12 @@ template <typename T> int apply(lsmp_ctx *c, lsmp_net *net, double band_cells, i
     int rc = build_band<T>(c, phi, band_cells * c->p.dx, &nb);
     if (rc) return rc;
     if (nb) {
-        hipLaunchKernelGGL((k_net_infer<T>), dim3(cdiv(nb, TILE)), dim3(256), 0, c->st, phi, out, c->band_list,
-                           (int)nb, geom(c), net->P, (T)c->p.dx, (T)c->p.ls_wid);
+        // persistent workgroups, two per CU (the weights fill a third of the LDS)
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        const unsigned nt = cdiv(nb, TILE), grid = std::min<unsigned>(nt, 2u * (unsigned)std::max(ncu, 1));
+        hipLaunchKernelGGL((k_net_infer<T>), dim3(grid), dim3(256), 0, c->st, phi, out, c->band_list, (int)nb,
+                           geom(c), net->P, (T)c->p.dx, (T)c->p.ls_wid);
         NET_HIP(hipGetLastError());
     }
     c->phiB = c->phi;

@@ -47,18 +47,13 @@ int scratch_address(double **out)
     return LSMP_OK;
 }
 
-// The fixed-order final reduction of k_finalize3 (1024 threads, slot b to
-// thread b, then the block sum), writing (mass, vol) to the caller's buffer.
+// The final reduction of k_finalize3 (1024 threads, the same compensated sum
+// in the same order), writing (mass, vol) to the caller's buffer.
 __global__ __launch_bounds__(1024) void k_mass_vol_out(const double *__restrict__ partial, int nblk,
                                                        double *__restrict__ out)
 {
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        acc[0] += partial[3 * b + 0];
-        acc[1] += partial[3 * b + 1];
-        acc[2] += partial[3 * b + 2];
-    }
-    block_sum<3>(acc);
+    double acc[3];
+    sum_partials3(partial, nblk, acc);
     if (threadIdx.x == 0) {
         out[0] = acc[1];
         out[1] = acc[0];

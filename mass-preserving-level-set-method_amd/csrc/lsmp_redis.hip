@@ -1,0 +1,394 @@
+// lsmp_redis.hip -- the 3D reinitialisation stage (level_set_rk3_redis.f90:176-264
+// with the gradient :266-369, lambda :371-454 and ls_funs ns_solver/others.f90:1-45),
+// as two kernels per RK stage.
+//
+// k_redis3d updates every cell of the domain with lambda = 0.  That is the
+// final value wherever delta = 0 (lambda is multiplied by delta), i.e. outside
+// the band |phi| < ls_wid.  For the cells inside it, it also stores s2 =
+// delta S0 (|grad| - 1), s3 = |grad| delta^2 and |grad| in side fields and
+// lists the cell: first in the workgroup's list in LDS, which the workgroup
+// appends to the device list with one atomic at its end.  k_redis3d_band
+// then takes the listed cells, one per thread over the whole device, forms
+// lambda of each from the 3x3x3 box of s2, s3 (centre weight 51) and writes
+// the cell's final value with the same expression.  Off the band s2 and s3 are
+// zero, so the box sums read only band cells.
+//
+// k_redis3d: a workgroup of R3X*R3Y threads owns an interior tile in (i, j),
+// one cell per thread, and marches over KC planes in k.  Each thread keeps
+// its column phi(k-3..k+3) in registers; the current plane, with a 4-cell
+// halo for the x and y stencils, goes to one of two LDS buffers (one barrier
+// per plane).  z direction: the WENO-HJ correction terms of D+ at plane p and
+// of D- at plane p+1 are built from the same four second differences
+// s(p-1..p+2) and share their smoothness indicators, their squares and the
+// pairwise products of those (in reversed roles), so each plane evaluates
+// one face p+1/2 and carries its D- term and the second differences to the
+// next plane.  x and y evaluate both terms per cell (weno_hj_pm_fast).
+//
+// The RK stage is written in Shu-Osher form, algebraically identical to the
+// reference's increments: phi1 = phi0 - dtau l(phi0);
+// phi2 = 3/4 phi0 + 1/4 (phi1 - dtau l(phi1)); phi3 = 1/3 phi0 + 2/3 (phi2 - dtau l(phi2)).
+// Stage 3 also reduces max |phi3 - phi0| (:46-62).
+#include <hip/hip_runtime.h>
+
+#include "lsmp_device.h"
+#include "lsmp_internal.h"
+
+namespace lsmp {
+
+namespace {
+
+constexpr int NT = R3X * R3Y;                   // one thread per tile cell
+constexpr int NHALO = P3X * P3Y - R3X * R3Y;    // LDS plane cells outside the tile
+static_assert(NHALO <= 2 * NT, "at most two halo cells per thread");
+static_assert(NT % 64 == 0, "whole waves");
+
+// LDS position (row, col) of halo cell number t of the (R3X + 8) x (R3Y + 8)
+// plane: four rows on top, four at the bottom, four columns left and right.
+LSMP_DEV void halo_cell(int t, int &r, int &c)
+{
+    if (t < 4 * P3X) { r = t / P3X; c = t % P3X; }
+    else if (t < 8 * P3X) { r = P3Y - 4 + (t - 4 * P3X) / P3X; c = (t - 4 * P3X) % P3X; }
+    else { const int u = t - 8 * P3X; r = 4 + u / 8; c = u % 8 < 4 ? u % 8 : P3X - 8 + u % 8; }
+}
+
+// delta != 0 only where ls_funs_fast takes its middle branch
+template <typename T> LSMP_DEV bool in_band_x(T x)
+{
+    return !(x > T(1) - T(1.0e-12)) && !(x < T(-1) + T(1.0e-12));
+}
+
+// Face m+1/2 of a line with second differences sa = s(m-1), sb = s(m),
+// sc = s(m+1), sd = s(m+2): plus = phyn(sd, sc, sb, sa) (the term of D+ at m)
+// and minus = phyn(sa, sb, sc, sd) (the term of D- at m+1), each as
+// phyn_fast evaluates it.  IS0 of one is IS2 of the other, IS1 is common,
+// and so are the products of their squares.
+template <typename T> LSMP_DEV void weno_hj_face(T sa, T sb, T sc, T sd, T &plus, T &minus)
+{
+    const T eps = T(1.0e-6);
+    const T isA = T(13) * sq(sd - sc) + T(3) * sq(sd - T(3) * sc);   // IS0 of plus, IS2 of minus
+    const T isJ = T(13) * sq(sc - sb) + T(3) * sq(sc + sb);          // IS1 of both
+    const T isB = T(13) * sq(sb - sa) + T(3) * sq(T(3) * sb - sa);   // IS2 of plus, IS0 of minus
+    const T qA = sq(eps + isA), qJ = sq(eps + isJ), qB = sq(eps + isB);
+    const T pJB = qJ * qB, pAJ = qA * qJ, pAB = qA * qB;
+    // plus: q0 = qA, q1 = qJ, q2 = qB
+    const T rp = rcp_w(pJB + T(6) * pAB + T(3) * pAJ);
+    const T tp0 = pJB * (rp * T(1.0 / 3.0));
+    const T tp2 = fma(pAJ, rp * T(0.5), T(-1.0 / 12.0));
+    plus = fma(tp0, sd - T(2) * sc + sb, tp2 * (sc - T(2) * sb + sa));
+    // minus: q0 = qB, q1 = qJ, q2 = qA
+    const T rm = rcp_w(pAJ + T(6) * pAB + T(3) * pJB);
+    const T tm0 = pAJ * (rm * T(1.0 / 3.0));
+    const T tm2 = fma(pJB, rm * T(0.5), T(-1.0 / 12.0));
+    minus = fma(tm0, sa - T(2) * sb + sc, tm2 * (sb - T(2) * sc + sd));
+}
+
+// Godunov |grad phi| (level_set_rk3_redis.f90:341-360): with sigma the sign of
+// S0, max(-min(D+, 0), max(D-, 0)) is max(-sigma D+, sigma D-, 0) per direction
+// (and max(max(D+, 0), -min(D-, 0)) the same with sigma = -1).
+template <typename T> LSMP_DEV T godunov_sig(T xp, T xm, T yp, T ym, T zp, T zm, T s0)
+{
+    const T sg = s0 >= T(0) ? T(1) : T(-1);
+    const T gx = fmax(fmax(-(sg * xp), sg * xm), T(0));
+    const T gy = fmax(fmax(-(sg * yp), sg * ym), T(0));
+    const T gz = fmax(fmax(-(sg * zp), sg * zm), T(0));
+    return sqrt(sq(gx) + sq(gy) + sq(gz));
+}
+
+// The stage's new value from the pseudo-time RHS l (Shu-Osher form).
+template <typename T, int STAGE> LSMP_DEV T rk_stage(T phi_c, T p0, T l, T rdt)
+{
+    if (STAGE == 1) return phi_c - rdt * l;
+    if (STAGE == 2) return T(0.75) * p0 + T(0.25) * (phi_c - rdt * l);
+    return p0 * T(1.0 / 3.0) + T(2.0 / 3.0) * (phi_c - rdt * l);
+}
+
+template <typename T> LSMP_DEV T load_at(const T *plane_base, unsigned byte_off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(plane_base) + byte_off);
+}
+
+}  // namespace
+
+template <typename T, int STAGE, int BTN, int KC>
+__global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__restrict__ phi_out,
+                                                const T *__restrict__ phi0, const T *__restrict__ S0, RedisArgs ra,
+                                                unsigned long long *ctl, const unsigned char *__restrict__ tiles,
+                                                T *__restrict__ bs2, T *__restrict__ bs3, T *__restrict__ bgr,
+                                                unsigned *__restrict__ blist, unsigned *__restrict__ bcount)
+{
+    __shared__ T sphi[2][P3Y][P3X];
+    // the workgroup's band cells, (plane - k0) * NT + thread, and their count
+    __shared__ unsigned short lcell[BTN == 1 ? KC * NT : 1];
+    __shared__ unsigned lcount, lbase;
+    static_assert(KC * NT <= 65536, "local band-cell index in 16 bits");
+    if (ctl[1]) return;   // the pseudo-time loop has stopped (k_redis_ctl)
+    const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
+    const size_t tile_id = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    // narrow band: a workgroup with no cell of the tube keeps its cells.  Stage
+    // 1 writes the step's output buffer and stage 2 the one stage 3 reads, so
+    // both copy; stage 3 writes the buffer stage 1 filled, which holds them.
+    if (tiles && !tiles[tile_id]) {
+        if (STAGE == 3) return;
+        const int ci = blockIdx.x * R3X + (int)threadIdx.x % R3X, cj = blockIdx.y * R3Y + (int)threadIdx.x / R3X;
+        if (ci >= nx || cj >= ny) return;
+        const size_t c0 = (size_t)cj * nx + ci, pl = (size_t)nx * ny;
+        for (int k = blockIdx.z * KC; k < min((int)(blockIdx.z + 1) * KC, nz); ++k)
+            phi_out[(size_t)k * pl + c0] = phi[(size_t)k * pl + c0];
+        return;
+    }
+    const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy, inv12dz = (T)ra.inv12dz;
+    const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
+    const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * R3X, j0 = blockIdx.y * R3Y, k0 = blockIdx.z * KC;
+    const int k1 = min(k0 + KC, nz);
+    const int tx = t % R3X, ty = t / R3X;
+    // cells past the domain edge hold the boundary cell (zero-gradient ghosts)
+    const int gi = min(i0 + tx, nx - 1), gj = min(j0 + ty, ny - 1);
+    const bool inner = i0 + tx < nx && j0 + ty < ny;
+    const int pc = gi - (i0 - 4), pr = gj - (j0 - 4);
+    const bool halo = t < NHALO, halo2 = t + NT < NHALO;
+    int hr = 0, hc = 0, hr2 = 0, hc2 = 0;
+    if (halo) halo_cell(t, hr, hc);
+    if (halo2) halo_cell(t + NT, hr2, hc2);
+    // byte offsets within a plane (a plane holds fewer than 2^28 cells); the
+    // plane bases are uniform over the workgroup
+    const size_t plane = (size_t)nx * ny;
+    const unsigned col = ((unsigned)gj * nx + gi) * (unsigned)sizeof(T);
+    const unsigned hcol = ((unsigned)clampi(j0 - 4 + hr, 0, ny - 1) * nx + clampi(i0 - 4 + hc, 0, nx - 1)) *
+                          (unsigned)sizeof(T);
+    const unsigned hcol2 = ((unsigned)clampi(j0 - 4 + hr2, 0, ny - 1) * nx + clampi(i0 - 4 + hc2, 0, nx - 1)) *
+                           (unsigned)sizeof(T);
+    auto base = [&](const T *f, int p) { return f + (size_t)clampi(p, 0, nz - 1) * plane; };
+
+    T w[7];   // phi(p-3 .. p+3) of this cell's column
+#pragma unroll
+    for (int d = 0; d < 7; ++d) w[d] = load_at(base(phi, k0 - 3 + d), col);
+    // z: second differences s(k0-1), s(k0), s(k0+1) and the D- term of the
+    // first plane (its face k0 - 1/2)
+    T za = w[0] - T(2) * w[1] + w[2];   // s(k0 - 2)
+    T zb = w[1] - T(2) * w[2] + w[3];   // s(k0 - 1)
+    T zc = w[2] - T(2) * w[3] + w[4];   // s(k0)
+    T zminus;
+    {
+        T unused;
+        const T zd = w[3] - T(2) * w[4] + w[5];   // s(k0 + 1)
+        weno_hj_face(za, zb, zc, zd, unused, zminus);
+        za = zb; zb = zc; zc = zd;
+    }
+    T halo_cur = halo ? load_at(base(phi, k0), hcol) : T(0);
+    T halo2_cur = halo2 ? load_at(base(phi, k0), hcol2) : T(0);
+    T s0_cur = load_at(base(S0, k0), col);
+    T p0_cur = STAGE > 1 ? load_at(base(phi0, k0), col) : T(0);
+    if (BTN == 1 && t == 0) lcount = 0;   // read after the loop's first barrier
+    double lerr = 0.0;
+    for (int pl = k0; pl < k1; ++pl) {
+        // the plane index is the same in every lane: plane addresses are scalar
+        const int p = __builtin_amdgcn_readfirstlane(pl);
+        // next plane's inputs, in flight while this one is computed
+        const T w_next = load_at(base(phi, p + 4), col);
+        const T halo_next = halo ? load_at(base(phi, p + 1), hcol) : T(0);
+        const T halo2_next = halo2 ? load_at(base(phi, p + 1), hcol2) : T(0);
+        const T s0_next = load_at(base(S0, p + 1), col);
+        const T p0_next = STAGE > 1 ? load_at(base(phi0, p + 1), col) : T(0);
+
+        T(*sp)[P3X] = sphi[p & 1];
+        sp[ty + 4][tx + 4] = w[3];
+        if (halo) sp[hr][hc] = halo_cur;
+        if (halo2) sp[hr2][hc2] = halo2_cur;
+
+        // z face p + 1/2: its D+ term is this plane's, its D- term the next one's
+        const T zs_new = w[4] - T(2) * w[5] + w[6];   // s(p + 2)
+        T zplus, zminus_next;
+        weno_hj_face(za, zb, zc, zs_new, zplus, zminus_next);
+        const T az = inv12dz * (-(w[2] - w[1]) + T(7) * (w[3] - w[2]) + T(7) * (w[4] - w[3]) - (w[5] - w[4]));
+        const T zs1 = az + invdz * zplus, zs2 = az - invdz * zminus;
+        __syncthreads();
+        T xs1, xs2, ys1, ys2;
+        weno_hj_pm_fast(sp[pr][pc - 3], sp[pr][pc - 2], sp[pr][pc - 1], w[3], sp[pr][pc + 1], sp[pr][pc + 2],
+                        sp[pr][pc + 3], inv12dx, invdx, xs1, xs2);
+        weno_hj_pm_fast(sp[pr - 3][pc], sp[pr - 2][pc], sp[pr - 1][pc], w[3], sp[pr + 1][pc], sp[pr + 2][pc],
+                        sp[pr + 3][pc], inv12dy, invdy, ys1, ys2);
+        const T grad = godunov_sig(xs1, xs2, ys1, ys2, zs1, zs2, s0_cur);
+        // lambda = 0: final off the band, replaced by k_redis3d_band on it
+        const T l = s0_cur * grad - s0_cur;
+        const T pn = rk_stage<T, STAGE>(w[3], p0_cur, l, rdt);
+        const bool band = BTN == 1 && inner && in_band_x(w[3] * inv_w);
+        if (inner) {
+            const size_t off = (size_t)p * plane;
+            *reinterpret_cast<T *>(reinterpret_cast<char *>(phi_out + off) + col) = pn;
+            if (band) {
+                T H, del;
+                ls_funs_fast(w[3], inv_w, H, del);
+                const size_t c = off + col / sizeof(T);
+                bs2[c] = del * s0_cur * (grad - T(1));
+                bs3[c] = grad * del * del;
+                bgr[c] = grad;
+            } else if (STAGE == 3) {
+                lerr = fmax(lerr, (double)fabs(pn - p0_cur));
+            }
+        }
+        if (BTN == 1) {
+            // list this plane's band cells in LDS: one LDS atomic per wave
+            const unsigned long long vote = __ballot(band);
+            if (vote) {
+                const int lane = t & 63, lead = __ffsll((long long)vote) - 1;
+                unsigned at = 0;
+                if (lane == lead) at = atomicAdd(&lcount, (unsigned)__popcll(vote));
+                at = __shfl(at, lead, 64);
+                if (band)
+                    lcell[at + (unsigned)__popcll(vote & ((1ull << lane) - 1ull))] =
+                        (unsigned short)((p - k0) * NT + t);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 6; ++d) w[d] = w[d + 1];
+        w[6] = w_next;
+        za = zb; zb = zc; zc = zs_new;
+        zminus = zminus_next;
+        halo_cur = halo_next;
+        halo2_cur = halo2_next;
+        s0_cur = s0_next;
+        p0_cur = p0_next;
+    }
+    if (BTN == 1) {
+        // the workgroup's list onto the device list: one atomic
+        __syncthreads();
+        const unsigned nl = lcount;
+        if (nl) {
+            if (t == 0) lbase = atomicAdd(bcount, nl);
+            __syncthreads();
+            const unsigned at = lbase;
+            for (unsigned e = t; e < nl; e += NT) {
+                const unsigned lc = lcell[e], lt = lc % NT;
+                blist[at + e] = (unsigned)((size_t)(k0 + (int)(lc / NT)) * plane) +
+                                (unsigned)(j0 + (int)(lt / R3X)) * (unsigned)nx + (unsigned)(i0 + (int)(lt % R3X));
+            }
+        }
+    }
+    if (STAGE == 3) {
+        double m = block_max(lerr);
+        if (t == 0 && m > 0.0) atomic_max_nonneg(ctl, m);
+    }
+}
+
+// The listed band cells: lambda from the 3x3x3 box sums of s2, s3 (centre
+// weight 51, :409-448) in the reference's order -- in-plane sums row by row,
+// then (plane k-1 + plane k) + plane k+1 -- and the cell's value.  Ghost cells
+// of the box are the clamped interior cells (the zero-gradient fill of s2,
+// s3).  One thread per listed cell over a fixed grid.  Consecutive stages
+// alternate between two list lengths: this stage's (bcount) and the next
+// one's (bnext), which this pass clears -- the previous stage's pass has read
+// it, and the next stage's listing starts after this pass ends.
+template <typename T, int STAGE>
+__global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, T *__restrict__ phi_out,
+                                                     const T *__restrict__ phi0, const T *__restrict__ S0,
+                                                     RedisArgs ra, unsigned long long *ctl,
+                                                     const T *__restrict__ bs2, const T *__restrict__ bs3,
+                                                     const T *__restrict__ bgr, const unsigned *__restrict__ blist,
+                                                     const unsigned *bcount, unsigned *bnext)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *bnext = 0;
+    if (ctl[1]) return;   // stopped: the stage kernel listed nothing
+    const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
+    const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
+    const size_t plane = (size_t)nx * ny;
+    const unsigned ncell = bcount[0];
+    double lerr = 0.0;
+    for (unsigned e = blockIdx.x * NT + threadIdx.x; e < ncell; e += gridDim.x * NT) {
+        const unsigned c = blist[e];   // below 2^32: the list holds 32-bit indices
+        const unsigned rem = c % (unsigned)plane;
+        const int k = (int)(c / (unsigned)plane), cj = (int)(rem / (unsigned)nx), ci = (int)(rem % (unsigned)nx);
+        const T phc = phi[c];
+        const T s0 = S0[c], grad = bgr[c], c2 = bs2[c], c3 = bs3[c];
+        const T p0 = STAGE > 1 ? phi0[c] : T(0);
+        // the box: phi (band membership) and s2, s3 of every cell, all loads
+        // in flight together; cells off the band contribute nothing (their s2,
+        // s3 are zero up to sign), whatever the side fields hold there
+        T pv[27], v2[27], v3[27];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii) {
+                    const int m = (kk * 3 + jj) * 3 + ii;
+                    const size_t q = (size_t)clampi(k - 1 + kk, 0, nz - 1) * plane +
+                                     (size_t)clampi(cj - 1 + jj, 0, ny - 1) * nx + clampi(ci - 1 + ii, 0, nx - 1);
+                    pv[m] = phi[q];
+                    v2[m] = bs2[q];
+                    v3[m] = bs3[q];
+                }
+        T H, del;
+        ls_funs_fast(phc, inv_w, H, del);
+        T sum2[3], sum3[3];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            T a = T(0), b = T(0);
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const int m = kk * 9 + r;
+                const bool inb = in_band_x(pv[m] * inv_w);
+                a = a + (inb ? v2[m] : T(0));
+                b = b + (inb ? v3[m] : T(0));
+            }
+            sum2[kk] = a;
+            sum3[kk] = b;
+        }
+        const T la = T(51) * c2 + ((sum2[0] + sum2[1]) + sum2[2]);
+        const T lb = T(51) * c3 + ((sum3[0] + sum3[1]) + sum3[2]);
+        T lam = T(0);
+        if (fabs(lb) > T(1.0e-12)) lam = la / lb * del;
+        const T l = (s0 - lam) * grad - s0;
+        const T pn = rk_stage<T, STAGE>(phc, p0, l, rdt);
+        phi_out[c] = pn;
+        if (STAGE == 3) lerr = fmax(lerr, (double)fabs(pn - p0));
+    }
+    if (STAGE == 3) {
+        double m = block_max(lerr);
+        if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(ctl, m);
+    }
+}
+
+static inline unsigned nb_of(int n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// workgroups of the band pass: two per CU of the device, each thread a band
+// cell (about 131 k at once; a band that long is a large interface)
+constexpr unsigned BAND_GRID = 512;
+
+template <typename T>
+void launch_redis3d(hipStream_t st, int stage, int btn, const T *phi, T *phi_out, const T *phi0, const T *S0,
+                    const RedisArgs &ra, unsigned long long *ctl, const unsigned char *tiles, const RedisBand<T> &bd,
+                    bool band)
+{
+    // the narrow band (tiles) marches RKC_NB planes per workgroup
+    const int kc = tiles ? RKC_NB : RKC;
+    const dim3 g(nb_of(ra.nx, R3X), nb_of(ra.ny, R3Y), nb_of(ra.nz, kc));
+#define LSMP_R3(S, B, KCV)                                                                                     \
+    do {                                                                                                       \
+        if (!band)                                                                                             \
+            hipLaunchKernelGGL((k_redis3d<T, S, B, KCV>), g, dim3(NT), 0, st, phi, phi_out, phi0, S0, ra, ctl, \
+                               tiles, bd.s2, bd.s3, bd.gr, bd.list, bd.count);                                 \
+        else if (B == 1)                                                                                       \
+            hipLaunchKernelGGL((k_redis3d_band<T, S>), dim3(BAND_GRID), dim3(NT), 0, st, phi, phi_out, phi0,   \
+                               S0, ra, ctl, bd.s2, bd.s3, bd.gr, bd.list, bd.count, bd.next);                  \
+    } while (0)
+#define LSMP_R3K(S, B)                                                                                         \
+    do {                                                                                                       \
+        if (tiles) LSMP_R3(S, B, RKC_NB);                                                                      \
+        else LSMP_R3(S, B, RKC);                                                                               \
+    } while (0)
+    if (btn) { if (stage == 1) LSMP_R3K(1, 1); else if (stage == 2) LSMP_R3K(2, 1); else LSMP_R3K(3, 1); }
+    else { if (stage == 1) LSMP_R3K(1, 0); else if (stage == 2) LSMP_R3K(2, 0); else LSMP_R3K(3, 0); }
+#undef LSMP_R3K
+#undef LSMP_R3
+}
+
+template void launch_redis3d<double>(hipStream_t, int, int, const double *, double *, const double *, const double *,
+                                     const RedisArgs &, unsigned long long *, const unsigned char *,
+                                     const RedisBand<double> &, bool);
+template void launch_redis3d<float>(hipStream_t, int, int, const float *, float *, const float *, const float *,
+                                    const RedisArgs &, unsigned long long *, const unsigned char *,
+                                    const RedisBand<float> &, bool);
+
+}  // namespace lsmp

@@ -9,6 +9,12 @@
  *
  * Loops follow the reference's k, j, i (i fastest) order so that the
  * sequential sums reproduce an OpenMP build run on one thread.
+ *
+ * Built with -fopenmp (the CPU baseline's build, never the checker's), the
+ * loops whose iterations are independent -- pointwise updates, line solves,
+ * ghost fills -- and the max reduction of the pseudo-time stop run on every
+ * host thread; the sums stay sequential, so both builds give the same
+ * results.
  */
 #include "lsmp_oracle.h"
 
@@ -17,6 +23,18 @@
 #include <string.h>
 
 #define GHC 3
+
+#ifdef _OPENMP
+#include <omp.h>
+#define ORC_OMP(x) _Pragma(x)
+static int orc_threads(void) { return omp_get_max_threads(); }
+static int orc_tid(void) { return omp_get_thread_num(); }
+#else
+#define ORC_OMP(x)
+static int orc_threads(void) { return 1; }
+static int orc_tid(void) { return 0; }
+#endif
+#define ORC_PAR ORC_OMP("omp parallel for collapse(2) schedule(static)")
 
 struct orc_state {
     orc_params p;
@@ -120,18 +138,21 @@ void orc_bc(orc_state *s, double *dat)
 {
     const int nx = s->p.nx, ny = s->p.ny;
     const int ks = s->ks, ke = s->ke;
+    ORC_PAR
     for (int k = ks - GHC; k <= ke + GHC; ++k)
         for (int j = 1 - GHC; j <= ny + GHC; ++j)
             for (int i = 1; i <= GHC; ++i) {
                 AT(dat, 1 - i, j, k) = AT(dat, 1, j, k);
                 AT(dat, nx + i, j, k) = AT(dat, nx, j, k);
             }
+    ORC_PAR
     for (int k = ks - GHC; k <= ke + GHC; ++k)
         for (int i = 1 - GHC; i <= nx + GHC; ++i)
             for (int j = 1; j <= GHC; ++j) {
                 AT(dat, i, 1 - j, k) = AT(dat, i, 1, k);
                 AT(dat, i, ny + j, k) = AT(dat, i, ny, k);
             }
+    ORC_PAR
     for (int j = 1 - GHC; j <= ny + GHC; ++j)
         for (int i = 1 - GHC; i <= nx + GHC; ++i)
             for (int k = 1; k <= GHC; ++k) {
@@ -150,6 +171,7 @@ void orc_ls_funs(orc_state *s)
     const double pi = acos(-1.0);
     const double w = s->p.ls_wid;
     double *phi = s->f[ORC_PHI], *H = s->f[ORC_HEAVY], *D = s->f[ORC_DELTA], *SG = s->f[ORC_SIGN];
+    ORC_OMP("omp parallel for schedule(static)")
     for (long q = 0; q < s->nbox; ++q) {
         double x = phi[q] / w;
         double heavy, hp;
@@ -350,6 +372,7 @@ void orc_rk3_source(orc_state *s)
     double *ssy2 = s->f[ORC_SSY2], *ssz1 = s->f[ORC_SSZ1], *ssz2 = s->f[ORC_SSZ2];
     double *tmp = s->f[ORC_PHI_TMP];
 
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -368,10 +391,13 @@ void orc_rk3_source(orc_state *s)
 
     const int nmax = (nx > ny ? nx : ny) > s->p.nz ? (nx > ny ? nx : ny) : s->p.nz;
     const int len = nmax + 2 * GHC;
-    double *lf = (double *)malloc(sizeof(double) * len), *lg = (double *)malloc(sizeof(double) * len);
-    double *lfp = (double *)malloc(sizeof(double) * len), *lgm = (double *)malloc(sizeof(double) * len);
+    double *scr = (double *)malloc(sizeof(double) * 4 * len * orc_threads());
+    ORC_OMP("omp parallel")
+    {
+    double *lf = scr + 4 * len * orc_tid(), *lg = lf + len, *lfp = lg + len, *lgm = lfp + len;
 
     /* x lines (:,j,k) */
+    ORC_OMP("omp for collapse(2) schedule(static)")
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j) {
             for (int i = 1 - GHC; i <= nx + GHC; ++i) {
@@ -384,6 +410,7 @@ void orc_rk3_source(orc_state *s)
             }
         }
     /* y lines (i,:,k) */
+    ORC_OMP("omp for collapse(2) schedule(static)")
     for (int k = ks; k <= ke; ++k)
         for (int i = 1; i <= nx; ++i) {
             for (int j = 1 - GHC; j <= ny + GHC; ++j) {
@@ -397,6 +424,7 @@ void orc_rk3_source(orc_state *s)
         }
     /* z lines (i,j,:), 3D only */
     if (three_d) {
+        ORC_OMP("omp for collapse(2) schedule(static)")
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
                 for (int k = ks - GHC; k <= ke + GHC; ++k) {
@@ -409,7 +437,8 @@ void orc_rk3_source(orc_state *s)
                 }
             }
     }
-    free(lf); free(lg); free(lfp); free(lgm);
+    }
+    free(scr);
 
     memcpy(sx1, ssx1, sizeof(double) * s->nbox);
     memcpy(sx2, ssx2, sizeof(double) * s->nbox);
@@ -419,6 +448,7 @@ void orc_rk3_source(orc_state *s)
     memcpy(sz2, ssz2, sizeof(double) * s->nbox);
 
     const double dx = s->p.dx, dy = s->p.dy, dz = s->p.dz;
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -448,6 +478,7 @@ void orc_rk3_advect(orc_state *s)
     double *l1 = s->f[ORC_L1], *l2 = s->f[ORC_L2], *l3 = s->f[ORC_L3];
 
     rk3_rhs(s);
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -458,6 +489,7 @@ void orc_rk3_advect(orc_state *s)
     orc_bc(s, phi);
 
     rk3_rhs(s);
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -468,6 +500,7 @@ void orc_rk3_advect(orc_state *s)
     orc_bc(s, phi);
 
     rk3_rhs(s);
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -508,6 +541,7 @@ void orc_redis_gradient(orc_state *s)
 
     orc_bc(s, P);
 #define PH(a, b, c) AT(P, (a), (b), (c))
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -589,6 +623,7 @@ void orc_redis_lambda(orc_state *s, int btn)
         return;
     }
     orc_ls_funs(s);
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -600,6 +635,7 @@ void orc_redis_lambda(orc_state *s, int btn)
     orc_bc(s, s3);
     const double wc = three_d ? 51.0 : 17.0;
     const int kr = three_d ? 1 : 0;
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -627,6 +663,7 @@ static void orc_redis_solver(orc_state *s, int btn)
 
     orc_redis_gradient(s);
     orc_redis_lambda(s, btn);
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -638,6 +675,7 @@ static void orc_redis_solver(orc_state *s, int btn)
 
     orc_redis_gradient(s);
     orc_redis_lambda(s, btn);
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -649,6 +687,7 @@ static void orc_redis_solver(orc_state *s, int btn)
 
     orc_redis_gradient(s);
     orc_redis_lambda(s, btn);
+    ORC_PAR
     for (int k = ks; k <= ke; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -702,11 +741,13 @@ int orc_redis(orc_state *s, int btn, double T)
     for (;;) {
         iter = iter + 1;
         time = time + s->p.rdt;
+        ORC_PAR
         for (int k = ks; k <= ke; ++k)
             for (int j = 1; j <= ny; ++j)
                 for (int i = 1; i <= nx; ++i) AT(tmp, i, j, k) = AT(phi, i, j, k);
         orc_redis_solver(s, btn);
         double error = 0.0;
+        ORC_OMP("omp parallel for collapse(2) reduction(max:error) schedule(static)")
         for (int k = ks; k <= ke; ++k)
             for (int j = 1; j <= ny; ++j)
                 for (int i = 1; i <= nx; ++i)
@@ -985,9 +1026,12 @@ void orc_surface_norms(orc_state *s)
     ccd_build(&my, s->nyb, dx);
     ccd_build(&mz, s->nzb, dx);
     int len = s->nxb; if (s->nyb > len) len = s->nyb; if (s->nzb > len) len = s->nzb;
-    double *f = (double *)malloc(sizeof(double) * len);
-    double *fx = (double *)malloc(sizeof(double) * len), *fxx = (double *)malloc(sizeof(double) * len);
+    double *scr = (double *)malloc(sizeof(double) * 3 * len * orc_threads());
+    ORC_OMP("omp parallel")
+    {
+    double *f = scr + 3 * len * orc_tid(), *fx = f + len, *fxx = fx + len;
 
+    ORC_OMP("omp for collapse(2) schedule(static)")
     for (int k = ks - GHC; k <= ke + GHC; ++k)
         for (int j = 1 - GHC; j <= ny + GHC; ++j) {
             for (int i = 1 - GHC; i <= nx + GHC; ++i) f[i - 1 + GHC] = AT(P, i, j, k);
@@ -996,6 +1040,7 @@ void orc_surface_norms(orc_state *s)
                 AT(NX, i, j, k) = fx[i - 1 + GHC]; AT(NXX, i, j, k) = fxx[i - 1 + GHC];
             }
         }
+    ORC_OMP("omp for collapse(2) schedule(static)")
     for (int k = ks - GHC; k <= ke + GHC; ++k)
         for (int i = 1 - GHC; i <= nx + GHC; ++i) {
             for (int j = 1 - GHC; j <= ny + GHC; ++j) f[j - 1 + GHC] = AT(P, i, j, k);
@@ -1004,6 +1049,7 @@ void orc_surface_norms(orc_state *s)
                 AT(NY, i, j, k) = fx[j - 1 + GHC]; AT(NYY, i, j, k) = fxx[j - 1 + GHC];
             }
         }
+    ORC_OMP("omp for collapse(2) schedule(static)")
     for (int j = 1 - GHC; j <= ny + GHC; ++j)
         for (int i = 1 - GHC; i <= nx + GHC; ++i) {
             for (int k = ks - GHC; k <= ke + GHC; ++k) f[k - ks + GHC] = AT(P, i, j, k);
@@ -1012,7 +1058,8 @@ void orc_surface_norms(orc_state *s)
                 AT(NZ, i, j, k) = fx[k - ks + GHC]; AT(NZZ, i, j, k) = fxx[k - ks + GHC];
             }
         }
-    free(f); free(fx); free(fxx);
+    }
+    free(scr);
     ccd_free(&mx); ccd_free(&my); ccd_free(&mz);
     orc_bc(s, NX); orc_bc(s, NXX);
     orc_bc(s, NY); orc_bc(s, NYY);
@@ -1047,6 +1094,7 @@ int orc_mass_fix(orc_state *s)
                     lam = lam + plam;
                 }
         lam = (s->imass - s->mass) / lam;
+        ORC_PAR
         for (int k = ks; k <= ke; ++k)
             for (int j = 1; j <= ny; ++j)
                 for (int i = 1; i <= nx; ++i)

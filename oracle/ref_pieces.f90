@@ -7,10 +7,17 @@
 !   Vortex/src/others/TDMA.f90          solve_tridiagonal            (:1-36)
 !   Vortex/lib/solver/matrix_solver.f90 TWIN_DEC / TWIN_BKS          (:6-78)
 !   Vortex/lib/solver/ccd_solvers.f90   ccd_root init / solve "ccd"  (:382-427)
-! The rest of the reference cannot be built in this image (flang rejects the
-! REAL subscript at lib/data/mutligrid_root.f90:32-34 and LAPACK dgesv, used
-! at lib/data/tree_multigrid.f90:159,227, is absent), so the hot-path units
-! that live inside `program main` are not reachable from here.
+!   Vortex/src/interface_solver/level_set_rk3_solver.f90
+!                                       wenojs_flux_split,
+!                                       crweno_flux_split            (:288-414)
+!   Vortex/src/interface_solver/level_set_rk3_redis.f90
+!                                       phyn                         (:456-471)
+! The last two files also hold units that `use all`; only the argument-only
+! units above are compiled from them (whole, unmodified).  The rest of the
+! reference cannot be built in this image (flang rejects the REAL subscript
+! at lib/data/mutligrid_root.f90:32-34 and LAPACK dgesv, used at
+! lib/data/tree_multigrid.f90:159,227, is absent), so the hot-path units
+! that work on module `all` are not reachable from here.
 
 module ref_pieces
 use iso_c_binding
@@ -22,6 +29,13 @@ interface
         integer :: M, N
         real(8), dimension(m:n) :: A, B, C, S, X
     end subroutine
+    subroutine crweno_flux_split(f, g, fp, gm, is, ie, ghc)
+        integer :: is, ie, ghc
+        real(8), dimension(is-ghc:ie+ghc) :: f, g, fp, gm
+    end subroutine
+    function phyn(a, b, c, d)
+        real(8) :: a, b, c, d, phyn
+    end function
 end interface
 
 contains
@@ -44,6 +58,22 @@ subroutine ref_ccd_line(n, dx, f, fx, fxx) bind(C, name="ref_ccd_line")
     call r%init(1, n, dx, dt)
     call r%solve("ccd", f, fx, fxx)
 end subroutine
+
+! One CRWENO5 line as level_set_rk3_source_weno calls it
+! (level_set_rk3_solver.f90:149-151): split fluxes f (negative) and g
+! (positive) on (1-3 : n+3); fp and gm receive the face fluxes.
+subroutine ref_crweno_flux_split(f, g, fp, gm, n) bind(C, name="ref_crweno_flux_split")
+    integer(c_int), value :: n
+    real(c_double), dimension(1-3:n+3) :: f, g, fp, gm
+    call crweno_flux_split(f, g, fp, gm, 1, n, 3)
+end subroutine
+
+! The WENO-HJ correction term (level_set_rk3_redis.f90:456-471).
+function ref_phyn(a, b, c, d) bind(C, name="ref_phyn")
+    real(c_double), value :: a, b, c, d
+    real(c_double) :: ref_phyn
+    ref_phyn = phyn(a, b, c, d)
+end function
 
 ! Upwind CCD first derivative (root_solve "uccd", ccd_solvers.f90:428-438).
 subroutine ref_uccd_line(n, dx, f, u, fx) bind(C, name="ref_uccd_line")

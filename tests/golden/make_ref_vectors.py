@@ -2,10 +2,12 @@
 
 Runs the reference's solve_tridiagonal (Vortex/src/others/TDMA.f90:1-36), its
 CCD line solver (Vortex/lib/solver/ccd_solvers.f90:382-427 with
-matrix_solver.f90:6-78) and its upwind CCD solver ("uccd", ccd_solvers.f90:
-165-214, 312-345, 428-438), compiled from the reference sources by
-`make -C oracle ref`, on seeded inputs, and stores inputs and outputs in
-ref_vectors.npz next to this script.
+matrix_solver.f90:6-78), its upwind CCD solver ("uccd", ccd_solvers.f90:
+165-214, 312-345, 428-438), its CRWENO5 line routine with the WENO-JS face
+closures (Vortex/src/interface_solver/level_set_rk3_solver.f90:288-414) and
+its WENO-HJ term phyn (level_set_rk3_redis.f90:456-471), compiled from the
+reference sources by `make -C oracle ref`, on seeded inputs, and stores
+inputs and outputs in ref_vectors.npz next to this script.
 
     make -C oracle ref && python tests/golden/make_ref_vectors.py
 """
@@ -60,6 +62,28 @@ def main():
         fx = np.zeros(n)
         R.ref_uccd_line(n, dx, O._dp(f), O._dp(u), O._dp(fx))
         out[f"uccd{n}_f"], out[f"uccd{n}_u"], out[f"uccd{n}_dx"], out[f"uccd{n}_fx"] = f, u, dx, fx
+    # CRWENO5: split fluxes of smooth, discontinuous and noisy lines; odd,
+    # 2-unknown and 256^3-sized lines
+    for tag, n in (("smooth", 64), ("step", 63), ("noise", 255), ("tiny", 3)):
+        x = (np.arange(n + 6) - 2.5) / n
+        if tag == "smooth":
+            u = 1.3 * np.sin(2 * np.pi * x) * np.cos(np.pi * x)
+        elif tag == "step":
+            u = np.where(x > 0.37, 1.0, -0.4) + 0.01 * rng.normal(size=n + 6)
+        else:
+            u = rng.normal(size=n + 6)
+        ph = 0.15 - np.abs(x - 0.45) + 0.05 * rng.normal(size=n + 6)
+        g = 0.5 * (u + np.abs(u)) * ph
+        f = 0.5 * (u - np.abs(u)) * ph
+        fp, gm = rng.normal(size=n + 6), rng.normal(size=n + 6)
+        out[f"crw_{tag}_f"], out[f"crw_{tag}_g"] = f, g
+        out[f"crw_{tag}_fp_in"], out[f"crw_{tag}_gm_in"] = fp.copy(), gm.copy()
+        R.ref_crweno_flux_split(O._dp(f.copy()), O._dp(g.copy()), O._dp(fp), O._dp(gm), n)
+        out[f"crw_{tag}_fp"], out[f"crw_{tag}_gm"] = fp, gm
+    # phyn on second differences over six decades of scale
+    v = rng.normal(size=(4096, 4)) * rng.choice([1e-7, 1e-4, 1e-2, 1.0, 1e2], size=(4096, 1))
+    out["phyn_in"] = v
+    out["phyn_out"] = np.array([R.ref_phyn(*r) for r in v])
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
     print("wrote", os.path.join(HERE, "ref_vectors.npz"), len(out), "arrays")
 

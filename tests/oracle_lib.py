@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liblsmp_oracle.so")
+# LSMP_ORACLE_SO names another build of the same source (the OpenMP one of the
+# CPU baseline)
+ORACLE_SO = os.environ.get("LSMP_ORACLE_SO") or os.path.join(ROOT, "oracle", "_build", "liblsmp_oracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref_pieces.so")
 
 DP = C.POINTER(C.c_double)
@@ -100,7 +102,8 @@ _ref = None
 
 
 def ref():
-    """The reference's TDMA / CCD units, or None when they were not built."""
+    """The reference's TDMA, CCD, CRWENO5 and WENO-HJ units, or None when
+    they were not built."""
     global _ref
     if _ref is None:
         if not os.path.exists(REF_SO):
@@ -110,6 +113,9 @@ def ref():
         L.ref_ccd_line.argtypes = [C.c_int, C.c_double, DP, DP, DP]
         L.ref_uccd_line.argtypes = [C.c_int, C.c_double, DP, DP, DP]
         L.ref_ccd_tables.argtypes = [C.c_int, C.c_double, DP]
+        L.ref_crweno_flux_split.argtypes = [DP, DP, DP, DP, C.c_int]
+        L.ref_phyn.argtypes = [C.c_double] * 4
+        L.ref_phyn.restype = C.c_double
         _ref = L
     return _ref
 

@@ -63,6 +63,54 @@ def test_thomas_live_reference_unit():
         assert np.array_equal(X1, X2)
 
 
+# ------------------------------------------ CRWENO5 + WENO-JS closures (level_set_rk3_solver.f90:288-414)
+@pytest.mark.parametrize("tag", ["smooth", "step", "noise", "tiny"])
+def test_crweno_bitexact_vs_reference_vectors(gold, tag):
+    """The reference's own crweno_flux_split, optimal weights 0.2089141306,
+    0.4999999998, 0.2910858692 and WENO-JS face closures included."""
+    f, g = gold[f"crw_{tag}_f"], gold[f"crw_{tag}_g"]
+    fp, gm = gold[f"crw_{tag}_fp_in"].copy(), gold[f"crw_{tag}_gm_in"].copy()
+    n = f.size - 6
+    O.lib().orc_crweno_flux_split(dp(f.copy()), dp(g.copy()), dp(fp), dp(gm), n)
+    assert np.array_equal(fp, gold[f"crw_{tag}_fp"])
+    assert np.array_equal(gm, gold[f"crw_{tag}_gm"])
+
+
+def test_crweno_live_reference_unit():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+    rng = np.random.default_rng(11)
+    for n in (2, 5, 17, 64, 300):
+        for _ in range(3):
+            u = rng.normal(size=n + 6) * rng.choice([1e-3, 1.0, 50.0])
+            ph = rng.normal(size=n + 6)
+            g, f = 0.5 * (u + np.abs(u)) * ph, 0.5 * (u - np.abs(u)) * ph
+            fp1, gm1 = rng.normal(size=n + 6), rng.normal(size=n + 6)
+            fp2, gm2 = fp1.copy(), gm1.copy()
+            O.lib().orc_crweno_flux_split(dp(f.copy()), dp(g.copy()), dp(fp1), dp(gm1), n)
+            R.ref_crweno_flux_split(dp(f.copy()), dp(g.copy()), dp(fp2), dp(gm2), n)
+            assert np.array_equal(fp1, fp2) and np.array_equal(gm1, gm2)
+
+
+# ------------------------------------------------ WENO-HJ term phyn (level_set_rk3_redis.f90:456-471)
+def test_phyn_bitexact_vs_reference_vectors(gold):
+    v, want = gold["phyn_in"], gold["phyn_out"]
+    got = np.array([O.lib().orc_phyn(*r) for r in v])
+    assert np.array_equal(got, want)
+
+
+def test_phyn_live_reference_unit():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+    rng = np.random.default_rng(13)
+    v = rng.normal(size=(2000, 4)) * rng.choice([1e-8, 1e-3, 1.0, 1e3], size=(2000, 1))
+    got = np.array([O.lib().orc_phyn(*r) for r in v])
+    want = np.array([R.ref_phyn(*r) for r in v])
+    assert np.array_equal(got, want)
+
+
 # ---------------------------------------------------------------- CCD (ccd_solvers.f90, matrix_solver.f90)
 @pytest.mark.parametrize("n", [7, 38, 262])
 def test_ccd_bitexact_vs_reference_vectors(gold, n):
