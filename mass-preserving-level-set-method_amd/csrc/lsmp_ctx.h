@@ -44,10 +44,10 @@ struct lsmp_ctx {
     int last_lines[3] = {-1, -1, -1};  // listed lines of the last mass correction
     double nb_cells = 0.0;             // narrow band half-width in cells (0: off)
     unsigned char *tiles = nullptr;    // narrow band: 3D reinitialisation workgroups holding a tube cell
-    // 3D reinitialisation, band cells: s2, s3, |grad| side fields, the list of
-    // band cells and its two lengths, used by turns (allocated on the first
-    // btn = 1 reinitialisation); rb_count[rb_par] is zero between stages
-    void *rb_s2 = nullptr, *rb_s3 = nullptr, *rb_gr = nullptr;
+    // 3D reinitialisation, band cells: |grad| side field, the list of band
+    // cells and its two lengths, used by turns (allocated on the first btn = 1
+    // reinitialisation); rb_count[rb_par] is zero between stages
+    void *rb_gr = nullptr;
     unsigned *rb_list = nullptr, *rb_count = nullptr;
     int rb_par = 0;
     int *band_list = nullptr;        // narrow-band cell indices (learned reinitialisation)

@@ -409,15 +409,12 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
     RedisBand<T> bd;
     if (c->dim == 3 && btn == 1) {
         if (!c->rb_list) {
-            if (alloc(&c->rb_s2, c->ncell * c->esz) || alloc(&c->rb_s3, c->ncell * c->esz) ||
-                alloc(&c->rb_gr, c->ncell * c->esz) || alloc((void **)&c->rb_list, c->ncell * sizeof(unsigned)) ||
+            if (alloc(&c->rb_gr, c->ncell * c->esz) || alloc((void **)&c->rb_list, c->ncell * sizeof(unsigned)) ||
                 alloc((void **)&c->rb_count, 2 * sizeof(unsigned)))
                 return LSMP_E_HIP;
             launch_zero_bytes(c->st, (unsigned char *)c->rb_count, 2 * sizeof(unsigned));
             c->rb_par = 0;
         }
-        bd.s2 = (T *)c->rb_s2;
-        bd.s3 = (T *)c->rb_s3;
         bd.gr = (T *)c->rb_gr;
         bd.list = c->rb_list;
     }
@@ -753,7 +750,7 @@ void free_all(lsmp_ctx *c)
     void *bufs[] = {c->rctl, c->ucoef[0], c->ucoef[1], c->ucoef[2], c->vtab, c->phi, c->phiB, c->phiC, c->u, c->v, c->w, c->R, c->L1, c->L2, c->S0, c->phi0,
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count,
                     c->mk[0], c->mk[1], c->mflag, c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2], c->tiles,
-                    c->rb_s2, c->rb_s3, c->rb_gr, c->rb_list, c->rb_count,
+                    c->rb_gr, c->rb_list, c->rb_count,
                     c->vof, c->vofB, c->vof_old, c->vkeep, c->vof0, c->uf, c->vf, c->wf, c->nrm[0], c->nrm[1], c->nrm[2], c->ftab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);

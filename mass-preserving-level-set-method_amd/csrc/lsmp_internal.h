@@ -91,12 +91,12 @@ template <typename T>
 void launch_tube_tiles(hipStream_t st, const T *phi, int nx, int ny, int nz, double wband, unsigned char *tiles);
 // 3D reinitialisation stage (lsmp_redis.hip): band = false launches the pass
 // over every cell with lambda = 0, band = true (btn = 1 only) the pass over the
-// listed band cells with lambda.  The side fields hold s2, s3 and |grad| of the
-// band cells, list their cell indices (room for every cell); count is the
-// list's length for this stage (zero when the stage starts) and next the one
-// for the next stage, which the band pass clears.
+// listed band cells with lambda.  gr holds |grad| of the band cells, list
+// their cell indices (room for every cell); count is the list's length for
+// this stage (zero when the stage starts) and next the one for the next
+// stage, which the band pass clears.
 template <typename T> struct RedisBand {
-    T *s2 = nullptr, *s3 = nullptr, *gr = nullptr;
+    T *gr = nullptr;
     unsigned *list = nullptr, *count = nullptr, *next = nullptr;
 };
 template <typename T>

@@ -4,25 +4,28 @@
 //
 // k_redis3d updates every cell of the domain with lambda = 0.  That is the
 // final value wherever delta = 0 (lambda is multiplied by delta), i.e. outside
-// the band |phi| < ls_wid.  For the cells inside it, it also stores s2 =
-// delta S0 (|grad| - 1), s3 = |grad| delta^2 and |grad| in side fields and
-// lists the cell: first in the workgroup's list in LDS, which the workgroup
-// appends to the device list with one atomic at its end.  k_redis3d_band
-// then takes the listed cells, one per thread over the whole device, forms
-// lambda of each from the 3x3x3 box of s2, s3 (centre weight 51) and writes
-// the cell's final value with the same expression.  Off the band s2 and s3 are
-// zero, so the box sums read only band cells.
+// the band |phi| < ls_wid.  For the cells inside it, it also stores |grad| in
+// a side field and lists the cell: first in the workgroup's list in LDS,
+// which the workgroup appends to the device list with one atomic at its end.
+// k_redis3d_band then takes the listed cells, one per thread over the whole
+// device, forms s2 = delta S0 (|grad| - 1) and s3 = |grad| delta^2 of each
+// band cell of the 3x3x3 box around it, lambda from their sums (centre weight
+// 51) and the cell's final value with the same expression.  Off the band s2
+// and s3 are zero, so the box sums take band cells only.
 //
 // k_redis3d: a workgroup of R3X*R3Y threads owns an interior tile in (i, j),
 // one cell per thread, and marches over KC planes in k.  Each thread keeps
 // its column phi(k-3..k+3) in registers; the current plane, with a 4-cell
-// halo for the x and y stencils, goes to one of two LDS buffers (one barrier
-// per plane).  z direction: the WENO-HJ correction terms of D+ at plane p and
-// of D- at plane p+1 are built from the same four second differences
-// s(p-1..p+2) and share their smoothness indicators, their squares and the
-// pairwise products of those (in reversed roles), so each plane evaluates
-// one face p+1/2 and carries its D- term and the second differences to the
-// next plane.  x and y evaluate both terms per cell (weno_hj_pm_fast).
+// halo for the x and y stencils, goes to LDS.  The WENO-HJ correction terms
+// of D+ at m and of D- at m+1 are built from the same four second
+// differences s(m-1..m+2) and share their smoothness indicators, their
+// squares and the pairwise products of those (in reversed roles), so each
+// direction evaluates one face m+1/2 per cell.  z: each plane evaluates face
+// p+1/2 and carries its D- term and the second differences to the next
+// plane.  x and y: each thread evaluates the faces i+1/2 and j+1/2 of its
+// cell and hands the D- terms to its neighbours through LDS; wave 0 also
+// evaluates the tile's first faces i0-1/2 and j0-1/2.  Two barriers per
+// plane: the plane is in LDS, the D- terms are.
 //
 // The RK stage is written in Shu-Osher form, algebraically identical to the
 // reference's increments: phi1 = phi0 - dtau l(phi0);
@@ -60,26 +63,40 @@ template <typename T> LSMP_DEV bool in_band_x(T x)
 // Face m+1/2 of a line with second differences sa = s(m-1), sb = s(m),
 // sc = s(m+1), sd = s(m+2): plus = phyn(sd, sc, sb, sa) (the term of D+ at m)
 // and minus = phyn(sa, sb, sc, sd) (the term of D- at m+1), each as
-// phyn_fast evaluates it.  IS0 of one is IS2 of the other, IS1 is common,
-// and so are the products of their squares.
+// phyn_fast evaluates it up to rounding.  IS0 of one is IS2 of the other,
+// IS1 is common, and so are the products of their squares and the two
+// third differences the terms weigh.  The indicators are the expanded
+// quadratic forms 13 (a - b)^2 + 3 (a - 3b)^2 = 4 (4a^2 - 11ab + 10b^2) and
+// 13 (a - b)^2 + 3 (a + b)^2 = 4 (4a^2 - 5ab + 4b^2) over shared squares.
+// FP64 takes both reciprocals from one (the product of the two denominators
+// stays within [1e-48, 1e28]).
 template <typename T> LSMP_DEV void weno_hj_face(T sa, T sb, T sc, T sd, T &plus, T &minus)
 {
     const T eps = T(1.0e-6);
-    const T isA = T(13) * sq(sd - sc) + T(3) * sq(sd - T(3) * sc);   // IS0 of plus, IS2 of minus
-    const T isJ = T(13) * sq(sc - sb) + T(3) * sq(sc + sb);          // IS1 of both
-    const T isB = T(13) * sq(sb - sa) + T(3) * sq(T(3) * sb - sa);   // IS2 of plus, IS0 of minus
-    const T qA = sq(eps + isA), qJ = sq(eps + isJ), qB = sq(eps + isB);
+    const T bb = sb * sb, cc = sc * sc;
+    const T isA = fma(sd, fma(T(4), sd, T(-11) * sc), T(10) * cc);   // IS0 of plus, IS2 of minus, over 4
+    const T isJ = fma(sc, fma(T(4), sc, T(-5) * sb), T(4) * bb);     // IS1 of both, over 4
+    const T isB = fma(sa, fma(T(4), sa, T(-11) * sb), T(10) * bb);   // IS2 of plus, IS0 of minus, over 4
+    const T qA = sq(fma(T(4), isA, eps)), qJ = sq(fma(T(4), isJ, eps)), qB = sq(fma(T(4), isB, eps));
     const T pJB = qJ * qB, pAJ = qA * qJ, pAB = qA * qB;
-    // plus: q0 = qA, q1 = qJ, q2 = qB
-    const T rp = rcp_w(pJB + T(6) * pAB + T(3) * pAJ);
+    // plus: q0 = qA, q1 = qJ, q2 = qB; minus: q0 = qB, q1 = qJ, q2 = qA
+    const T dp = pJB + T(6) * pAB + T(3) * pAJ, dm = pAJ + T(6) * pAB + T(3) * pJB;
+    T rp, rm;
+    if constexpr (sizeof(T) == 8) {
+        const T r = rcp_w(dp * dm);
+        rp = dm * r;
+        rm = dp * r;
+    } else {
+        rp = rcp_w(dp);
+        rm = rcp_w(dm);
+    }
+    const T e1 = sd - T(2) * sc + sb, e2 = sc - T(2) * sb + sa;
     const T tp0 = pJB * (rp * T(1.0 / 3.0));
     const T tp2 = fma(pAJ, rp * T(0.5), T(-1.0 / 12.0));
-    plus = fma(tp0, sd - T(2) * sc + sb, tp2 * (sc - T(2) * sb + sa));
-    // minus: q0 = qB, q1 = qJ, q2 = qA
-    const T rm = rcp_w(pAJ + T(6) * pAB + T(3) * pJB);
+    plus = fma(tp0, e1, tp2 * e2);
     const T tm0 = pAJ * (rm * T(1.0 / 3.0));
     const T tm2 = fma(pJB, rm * T(0.5), T(-1.0 / 12.0));
-    minus = fma(tm0, sa - T(2) * sb + sc, tm2 * (sb - T(2) * sc + sd));
+    minus = fma(tm0, e2, tm2 * e1);
 }
 
 // Godunov |grad phi| (level_set_rk3_redis.f90:341-360): with sigma the sign of
@@ -91,7 +108,7 @@ template <typename T> LSMP_DEV T godunov_sig(T xp, T xm, T yp, T ym, T zp, T zm,
     const T gx = fmax(fmax(-(sg * xp), sg * xm), T(0));
     const T gy = fmax(fmax(-(sg * yp), sg * ym), T(0));
     const T gz = fmax(fmax(-(sg * zp), sg * zm), T(0));
-    return sqrt(sq(gx) + sq(gy) + sq(gz));
+    return sqrt_w(sq(gx) + sq(gy) + sq(gz));
 }
 
 // The stage's new value from the pseudo-time RHS l (Shu-Osher form).
@@ -113,10 +130,14 @@ template <typename T, int STAGE, int BTN, int KC>
 __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                 const T *__restrict__ phi0, const T *__restrict__ S0, RedisArgs ra,
                                                 unsigned long long *ctl, const unsigned char *__restrict__ tiles,
-                                                T *__restrict__ bs2, T *__restrict__ bs3, T *__restrict__ bgr,
-                                                unsigned *__restrict__ blist, unsigned *__restrict__ bcount)
+                                                T *__restrict__ bgr, unsigned *__restrict__ blist,
+                                                unsigned *__restrict__ bcount)
 {
-    __shared__ T sphi[2][P3Y][P3X];
+    __shared__ T sphi[P3Y][P3X];
+    // D- terms of the x and y faces: column tx (row ty) holds the one of cell tx (ty)
+    __shared__ T mxs[R3Y][R3X + 1];
+    __shared__ T mys[R3Y + 1][R3X];
+    static_assert(R3X + R3Y <= 64, "the first faces in wave 0");
     // the workgroup's band cells, (plane - k0) * NT + thread, and their count
     __shared__ unsigned short lcell[BTN == 1 ? KC * NT : 1];
     __shared__ unsigned lcount, lbase;
@@ -146,7 +167,6 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     // cells past the domain edge hold the boundary cell (zero-gradient ghosts)
     const int gi = min(i0 + tx, nx - 1), gj = min(j0 + ty, ny - 1);
     const bool inner = i0 + tx < nx && j0 + ty < ny;
-    const int pc = gi - (i0 - 4), pr = gj - (j0 - 4);
     const bool halo = t < NHALO, halo2 = t + NT < NHALO;
     int hr = 0, hc = 0, hr2 = 0, hc2 = 0;
     if (halo) halo_cell(t, hr, hc);
@@ -160,6 +180,14 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     const unsigned hcol2 = ((unsigned)clampi(j0 - 4 + hr2, 0, ny - 1) * nx + clampi(i0 - 4 + hc2, 0, nx - 1)) *
                            (unsigned)sizeof(T);
     auto base = [&](const T *f, int p) { return f + (size_t)clampi(p, 0, nz - 1) * plane; };
+
+    // the faces i+1/2 and j+1/2 read the LDS plane at the thread's own tile
+    // position: past the domain edge it holds the boundary cell, as the halo
+    // does.  Wave 0's lanes 0..R3Y-1 also take the x face i0-1/2 of row t,
+    // lanes R3Y..R3Y+R3X-1 the y face j0-1/2 of column t-R3Y.
+    const bool first_x = t < R3Y, first_y = t >= R3Y && t < R3Y + R3X;
+    const int f_off = first_x ? (t + 4) * P3X + 1 : (first_y ? P3X + 4 + (t - R3Y) : 0);
+    const int f_str = first_x ? 1 : P3X;
 
     T w[7];   // phi(p-3 .. p+3) of this cell's column
 #pragma unroll
@@ -178,8 +206,6 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     }
     T halo_cur = halo ? load_at(base(phi, k0), hcol) : T(0);
     T halo2_cur = halo2 ? load_at(base(phi, k0), hcol2) : T(0);
-    T s0_cur = load_at(base(S0, k0), col);
-    T p0_cur = STAGE > 1 ? load_at(base(phi0, k0), col) : T(0);
     if (BTN == 1 && t == 0) lcount = 0;   // read after the loop's first barrier
     double lerr = 0.0;
     for (int pl = k0; pl < k1; ++pl) {
@@ -189,13 +215,14 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
         const T w_next = load_at(base(phi, p + 4), col);
         const T halo_next = halo ? load_at(base(phi, p + 1), hcol) : T(0);
         const T halo2_next = halo2 ? load_at(base(phi, p + 1), hcol2) : T(0);
-        const T s0_next = load_at(base(S0, p + 1), col);
-        const T p0_next = STAGE > 1 ? load_at(base(phi0, p + 1), col) : T(0);
+        // this plane's S0 and phi0, used after the two barriers
+        const T s0_cur = load_at(base(S0, p), col);
+        const T p0_cur = STAGE > 1 ? load_at(base(phi0, p), col) : T(0);
 
-        T(*sp)[P3X] = sphi[p & 1];
-        sp[ty + 4][tx + 4] = w[3];
-        if (halo) sp[hr][hc] = halo_cur;
-        if (halo2) sp[hr2][hc2] = halo2_cur;
+        // the previous plane's readers have passed its second barrier
+        sphi[ty + 4][tx + 4] = w[3];
+        if (halo) sphi[hr][hc] = halo_cur;
+        if (halo2) sphi[hr2][hc2] = halo2_cur;
 
         // z face p + 1/2: its D+ term is this plane's, its D- term the next one's
         const T zs_new = w[4] - T(2) * w[5] + w[6];   // s(p + 2)
@@ -204,11 +231,42 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
         const T az = inv12dz * (-(w[2] - w[1]) + T(7) * (w[3] - w[2]) + T(7) * (w[4] - w[3]) - (w[5] - w[4]));
         const T zs1 = az + invdz * zplus, zs2 = az - invdz * zminus;
         __syncthreads();
-        T xs1, xs2, ys1, ys2;
-        weno_hj_pm_fast(sp[pr][pc - 3], sp[pr][pc - 2], sp[pr][pc - 1], w[3], sp[pr][pc + 1], sp[pr][pc + 2],
-                        sp[pr][pc + 3], inv12dx, invdx, xs1, xs2);
-        weno_hj_pm_fast(sp[pr - 3][pc], sp[pr - 2][pc], sp[pr - 1][pc], w[3], sp[pr + 1][pc], sp[pr + 2][pc],
-                        sp[pr + 3][pc], inv12dy, invdy, ys1, ys2);
+        // x face i+1/2: D+ term of this cell, D- term of cell i+1
+        T xs1, ax, ys1, ay;
+        {
+            const T *r = &sphi[ty + 4][tx + 4];
+            const T m2 = r[-2], m1 = r[-1], c = w[3], p1 = r[1], p2 = r[2], p3 = r[3];
+            ax = inv12dx * (-(m1 - m2) + T(7) * (c - m1) + T(7) * (p1 - c) - (p2 - p1));
+            T plus, minus;
+            weno_hj_face(m2 - T(2) * m1 + c, m1 - T(2) * c + p1, c - T(2) * p1 + p2, p1 - T(2) * p2 + p3, plus,
+                         minus);
+            xs1 = ax + invdx * plus;
+            mxs[ty][tx + 1] = minus;
+        }
+        // y face j+1/2
+        {
+            const T *r = &sphi[ty + 4][tx + 4];
+            const T m2 = r[-2 * P3X], m1 = r[-P3X], c = w[3], p1 = r[P3X], p2 = r[2 * P3X], p3 = r[3 * P3X];
+            ay = inv12dy * (-(m1 - m2) + T(7) * (c - m1) + T(7) * (p1 - c) - (p2 - p1));
+            T plus, minus;
+            weno_hj_face(m2 - T(2) * m1 + c, m1 - T(2) * c + p1, c - T(2) * p1 + p2, p1 - T(2) * p2 + p3, plus,
+                         minus);
+            ys1 = ay + invdy * plus;
+            mys[ty + 1][tx] = minus;
+        }
+        if (first_x || first_y) {
+            const T *f = &sphi[0][0] + f_off;
+            const T v0 = f[0], v1 = f[f_str], v2 = f[2 * f_str], v3 = f[3 * f_str], v4 = f[4 * f_str],
+                    v5 = f[5 * f_str];
+            T plus, minus;
+            weno_hj_face(v0 - T(2) * v1 + v2, v1 - T(2) * v2 + v3, v2 - T(2) * v3 + v4, v3 - T(2) * v4 + v5, plus,
+                         minus);
+            if (first_x) mxs[t][0] = minus;
+            else mys[0][t - R3Y] = minus;
+        }
+        __syncthreads();
+        const T xs2 = ax - invdx * mxs[ty][tx];
+        const T ys2 = ay - invdy * mys[ty][tx];
         const T grad = godunov_sig(xs1, xs2, ys1, ys2, zs1, zs2, s0_cur);
         // lambda = 0: final off the band, replaced by k_redis3d_band on it
         const T l = s0_cur * grad - s0_cur;
@@ -218,12 +276,7 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
             const size_t off = (size_t)p * plane;
             *reinterpret_cast<T *>(reinterpret_cast<char *>(phi_out + off) + col) = pn;
             if (band) {
-                T H, del;
-                ls_funs_fast(w[3], inv_w, H, del);
-                const size_t c = off + col / sizeof(T);
-                bs2[c] = del * s0_cur * (grad - T(1));
-                bs3[c] = grad * del * del;
-                bgr[c] = grad;
+                bgr[off + col / sizeof(T)] = grad;
             } else if (STAGE == 3) {
                 lerr = fmax(lerr, (double)fabs(pn - p0_cur));
             }
@@ -248,8 +301,6 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
         zminus = zminus_next;
         halo_cur = halo_next;
         halo2_cur = halo2_next;
-        s0_cur = s0_next;
-        p0_cur = p0_next;
     }
     if (BTN == 1) {
         // the workgroup's list onto the device list: one atomic
@@ -272,19 +323,26 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     }
 }
 
+// delta of ls_funs_fast for a cell of the band (its middle branch)
+template <typename T> LSMP_DEV T band_delta(T phi, T inv_w)
+{
+    const T pi = T(3.141592653589793238462643383279502884);
+    return T(0.5) * (T(1) + cos(pi * (phi * inv_w))) * inv_w;
+}
+
 // The listed band cells: lambda from the 3x3x3 box sums of s2, s3 (centre
 // weight 51, :409-448) in the reference's order -- in-plane sums row by row,
 // then (plane k-1 + plane k) + plane k+1 -- and the cell's value.  Ghost cells
 // of the box are the clamped interior cells (the zero-gradient fill of s2,
-// s3).  One thread per listed cell over a fixed grid.  Consecutive stages
-// alternate between two list lengths: this stage's (bcount) and the next
-// one's (bnext), which this pass clears -- the previous stage's pass has read
-// it, and the next stage's listing starts after this pass ends.
+// s3).  Four lanes per listed cell: lanes 0..2 sum box planes k-1..k+1, lane
+// 3 loads the cell itself and finishes it.  Consecutive stages alternate
+// between two list lengths: this stage's (bcount) and the next one's (bnext),
+// which this pass clears -- the previous stage's pass has read it, and the
+// next stage's listing starts after this pass ends.
 template <typename T, int STAGE>
 __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                      const T *__restrict__ phi0, const T *__restrict__ S0,
                                                      RedisArgs ra, unsigned long long *ctl,
-                                                     const T *__restrict__ bs2, const T *__restrict__ bs3,
                                                      const T *__restrict__ bgr, const unsigned *__restrict__ blist,
                                                      const unsigned *bcount, unsigned *bnext)
 {
@@ -292,57 +350,65 @@ __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, 
     if (ctl[1]) return;   // stopped: the stage kernel listed nothing
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
     const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
-    const size_t plane = (size_t)nx * ny;
+    const unsigned plane = (unsigned)nx * (unsigned)ny;
     const unsigned ncell = bcount[0];
+    const int q = threadIdx.x & 3, g0 = (threadIdx.x & 63) & ~3;   // role, first lane of the four
+    constexpr unsigned CPB = NT / 4;                                 // cells per workgroup and pass
     double lerr = 0.0;
-    for (unsigned e = blockIdx.x * NT + threadIdx.x; e < ncell; e += gridDim.x * NT) {
-        const unsigned c = blist[e];   // below 2^32: the list holds 32-bit indices
-        const unsigned rem = c % (unsigned)plane;
-        const int k = (int)(c / (unsigned)plane), cj = (int)(rem / (unsigned)nx), ci = (int)(rem % (unsigned)nx);
-        const T phc = phi[c];
-        const T s0 = S0[c], grad = bgr[c], c2 = bs2[c], c3 = bs3[c];
-        const T p0 = STAGE > 1 ? phi0[c] : T(0);
-        // the box: phi (band membership) and s2, s3 of every cell, all loads
-        // in flight together; cells off the band contribute nothing (their s2,
-        // s3 are zero up to sign), whatever the side fields hold there
-        T pv[27], v2[27], v3[27];
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk)
+    // the pass count is the same for every lane of the workgroup (shuffles)
+    for (unsigned e0 = blockIdx.x * CPB; e0 < ncell; e0 += gridDim.x * CPB) {
+        const unsigned e = e0 + threadIdx.x / 4;
+        const bool live = e < ncell;
+        const unsigned c = blist[live ? e : e0];   // below 2^32: the list holds 32-bit indices
+        const unsigned rem = c % plane;
+        const int k = (int)(c / plane), cj = (int)(rem / (unsigned)nx), ci = (int)(rem % (unsigned)nx);
+        T part2 = T(0), part3 = T(0);
+        T phc = T(0), s0 = T(0), grad = T(0), p0 = T(0);
+        if (q < 3) {
+            // box plane k-1+q: phi (band membership, delta), S0 and |grad| of
+            // its nine cells, all loads in flight together; cells off the band
+            // contribute nothing (their s2, s3 are zero up to sign), whatever
+            // |grad| holds there
+            const unsigned pb = (unsigned)clampi(k - 1 + q, 0, nz - 1) * plane;
+            T pv[9], sv[9], gv[9];
 #pragma unroll
             for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
                 for (int ii = 0; ii < 3; ++ii) {
-                    const int m = (kk * 3 + jj) * 3 + ii;
-                    const size_t q = (size_t)clampi(k - 1 + kk, 0, nz - 1) * plane +
-                                     (size_t)clampi(cj - 1 + jj, 0, ny - 1) * nx + clampi(ci - 1 + ii, 0, nx - 1);
-                    pv[m] = phi[q];
-                    v2[m] = bs2[q];
-                    v3[m] = bs3[q];
+                    const unsigned at = pb + (unsigned)clampi(cj - 1 + jj, 0, ny - 1) * (unsigned)nx +
+                                        (unsigned)clampi(ci - 1 + ii, 0, nx - 1);
+                    pv[jj * 3 + ii] = phi[at];
+                    sv[jj * 3 + ii] = S0[at];
+                    gv[jj * 3 + ii] = bgr[at];
                 }
-        T H, del;
-        ls_funs_fast(phc, inv_w, H, del);
-        T sum2[3], sum3[3];
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk) {
-            T a = T(0), b = T(0);
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
-                const int m = kk * 9 + r;
-                const bool inb = in_band_x(pv[m] * inv_w);
-                a = a + (inb ? v2[m] : T(0));
-                b = b + (inb ? v3[m] : T(0));
+                const bool inb = in_band_x(pv[r] * inv_w);
+                const T dn = band_delta(pv[r], inv_w);
+                const T s2 = dn * sv[r] * (gv[r] - T(1)), s3 = gv[r] * dn * dn;
+                part2 = part2 + (inb ? s2 : T(0));
+                part3 = part3 + (inb ? s3 : T(0));
             }
-            sum2[kk] = a;
-            sum3[kk] = b;
+        } else {
+            phc = phi[c];
+            s0 = S0[c];
+            grad = bgr[c];
+            if (STAGE > 1) p0 = phi0[c];
         }
-        const T la = T(51) * c2 + ((sum2[0] + sum2[1]) + sum2[2]);
-        const T lb = T(51) * c3 + ((sum3[0] + sum3[1]) + sum3[2]);
-        T lam = T(0);
-        if (fabs(lb) > T(1.0e-12)) lam = la / lb * del;
-        const T l = (s0 - lam) * grad - s0;
-        const T pn = rk_stage<T, STAGE>(phc, p0, l, rdt);
-        phi_out[c] = pn;
-        if (STAGE == 3) lerr = fmax(lerr, (double)fabs(pn - p0));
+        const T a0 = __shfl(part2, g0, 64), a1 = __shfl(part2, g0 + 1, 64), a2 = __shfl(part2, g0 + 2, 64);
+        const T b0 = __shfl(part3, g0, 64), b1 = __shfl(part3, g0 + 1, 64), b2 = __shfl(part3, g0 + 2, 64);
+        if (q == 3 && live) {
+            const T del = band_delta(phc, inv_w);
+            const T c2 = del * s0 * (grad - T(1)), c3 = grad * del * del;
+            const T la = T(51) * c2 + ((a0 + a1) + a2);
+            const T lb = T(51) * c3 + ((b0 + b1) + b2);
+            T lam = T(0);
+            if (fabs(lb) > T(1.0e-12)) lam = la / lb * del;
+            const T l = (s0 - lam) * grad - s0;
+            const T pn = rk_stage<T, STAGE>(phc, p0, l, rdt);
+            phi_out[c] = pn;
+            if (STAGE == 3) lerr = fmax(lerr, (double)fabs(pn - p0));
+        }
     }
     if (STAGE == 3) {
         double m = block_max(lerr);
@@ -352,9 +418,9 @@ __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, 
 
 static inline unsigned nb_of(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// workgroups of the band pass: two per CU of the device, each thread a band
-// cell (about 131 k at once; a band that long is a large interface)
-constexpr unsigned BAND_GRID = 512;
+// workgroups of the band pass: four per CU of the device, four lanes a band
+// cell (65536 cells a pass; a longer list takes more passes)
+constexpr unsigned BAND_GRID = 1024;
 
 template <typename T>
 void launch_redis3d(hipStream_t st, int stage, int btn, const T *phi, T *phi_out, const T *phi0, const T *S0,
@@ -368,10 +434,10 @@ void launch_redis3d(hipStream_t st, int stage, int btn, const T *phi, T *phi_out
     do {                                                                                                       \
         if (!band)                                                                                             \
             hipLaunchKernelGGL((k_redis3d<T, S, B, KCV>), g, dim3(NT), 0, st, phi, phi_out, phi0, S0, ra, ctl, \
-                               tiles, bd.s2, bd.s3, bd.gr, bd.list, bd.count);                                 \
+                               tiles, bd.gr, bd.list, bd.count);                                               \
         else if (B == 1)                                                                                       \
             hipLaunchKernelGGL((k_redis3d_band<T, S>), dim3(BAND_GRID), dim3(NT), 0, st, phi, phi_out, phi0,   \
-                               S0, ra, ctl, bd.s2, bd.s3, bd.gr, bd.list, bd.count, bd.next);                  \
+                               S0, ra, ctl, bd.gr, bd.list, bd.count, bd.next);                                \
     } while (0)
 #define LSMP_R3K(S, B)                                                                                         \
     do {                                                                                                       \
