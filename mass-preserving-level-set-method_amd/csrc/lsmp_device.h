@@ -17,12 +17,12 @@ LSMP_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi :
 // own L2.  This renumbers workgroup b so that XCD x takes a contiguous range
 // (the first nb % 8 XCDs one workgroup more): neighbouring strided lines, which
 // share cache lines, then run on the same XCD at about the same time.
-LSMP_DEV int xcd_block()
+LSMP_DEV int xcd_renumber(int b, int nb)
 {
-    const int b = (int)blockIdx.x, nb = (int)gridDim.x;
     const int x = b % 8, i = b / 8, q = nb / 8, r = nb % 8;
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
+LSMP_DEV int xcd_block() { return xcd_renumber((int)blockIdx.x, (int)gridDim.x); }
 
 // Pointwise flux split g = 1/2 (u + |u|) phi, f = 1/2 (u - |u|) phi
 // (level_set_rk3_solver.f90:118-125).

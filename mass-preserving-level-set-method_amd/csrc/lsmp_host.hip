@@ -425,9 +425,7 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
         const int k = std::min(CHUNK, budget - launched);
         for (int q = 0; q < k; ++q) {
             T *cur = (T *)c->phi, *b = (T *)c->phiB, *cc = (T *)c->phiC;
-            launch_redis_ctl(c->st, ctl, 1);
-            LSMP_LAUNCHED();
-            if (c->dim == 3 && !std::getenv("LSMP_REDIS_OLD")) {
+            if (c->dim == 3) {
                 // stages 1..3: every cell (lambda = 0), then the band cells (lambda)
                 const T *in[3] = {cur, b, cc};
                 T *outp[3] = {b, cc, b};
@@ -445,9 +443,9 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
                     }
                 }
             } else {
-            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 1, btn, cur, b, cur, S0, ra, ctl, tiles)));
-            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 2, btn, b, cc, cur, S0, ra, ctl, tiles)));
-            KT(c, "redis_stage", (launch_redis_stage<T>(c->st, c->dim, 3, btn, cc, b, cur, S0, ra, ctl, tiles)));
+                KT(c, "redis_stage", (launch_redis_stage2d<T>(c->st, 1, btn, cur, b, cur, S0, ra, ctl)));
+                KT(c, "redis_stage", (launch_redis_stage2d<T>(c->st, 2, btn, b, cc, cur, S0, ra, ctl)));
+                KT(c, "redis_stage", (launch_redis_stage2d<T>(c->st, 3, btn, cc, b, cur, S0, ra, ctl)));
             }
             launch_redis_ctl(c->st, ctl, 2);
             // the new phi is in b; the old one becomes a spare

@@ -32,14 +32,12 @@ constexpr int RPX = RTX + 8, RPY = RTY + 8;
 constexpr int RNT = 640;
 static_assert(REX * REY <= RNT, "reinit tile needs one thread per ring cell");
 
-// 3D reinitialisation tile: R3X x R3Y interior cells, one thread each; RKC
-// planes per workgroup in z.  32 x 8 (4 waves) measured fastest at 256^3:
-// FP64 311 us per stage against 342 us for 32 x 16.
+// 3D reinitialisation tile: R3X x R3Y interior cells, one thread each (a
+// 4-wave workgroup); RKC planes per workgroup in z.
 #ifndef LSMP_R3Y
 #define LSMP_R3Y 8
 #endif
 constexpr int R3X = 32, R3Y = LSMP_R3Y;
-constexpr int E3X = R3X + 2, E3Y = R3Y + 2;
 constexpr int P3X = R3X + 8, P3Y = R3Y + 8;
 constexpr int RKC = 32;
 // the narrow band's workgroups march RKC_NB planes: a workgroup without a tube
@@ -106,9 +104,8 @@ void launch_redis3d(hipStream_t st, int stage, int btn, const T *phi, T *phi_out
 template <typename T>
 void launch_combine_markers(hipStream_t st, const T *m1, const T *m2, T *phi, size_t n, unsigned *flag);
 template <typename T>
-void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *phi, T *phi_out, const T *phi0,
-                        const T *S0, const RedisArgs &ra, unsigned long long *ctl,
-                        const unsigned char *tiles = nullptr);
+void launch_redis_stage2d(hipStream_t st, int stage, int btn, const T *phi, T *phi_out, const T *phi0, const T *S0,
+                          const RedisArgs &ra, unsigned long long *ctl);
 template <typename T> void launch_sign_init(hipStream_t st, const T *phi, T *S0, size_t n, T ls_wid, int raw);
 template <typename T>
 void launch_grad_max(hipStream_t st, int dim, const T *phi, const T *S0, const RedisArgs &ra,

@@ -699,44 +699,10 @@ __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ 
 
 
 // ---------------------------------------------------------------------------
-// Reinitialisation stage, 3D (level_set_rk3_redis.f90:176-264 with the
-// gradient :266-369, lambda :371-454 and ls_funs ns_solver/others.f90:1-45).
-//
-// A workgroup of R3X*R3Y threads owns an interior tile R3X x R3Y in (i, j),
-// one cell per thread, and marches over RKC planes in k.  Each thread keeps
-// its column phi(k-3..k+3) in registers (the z stencil); the current plane,
-// with a 4-cell halo for the x and y stencils, is published to LDS (the
-// halo by threads 0..RHALO-1, one cell each, loaded a plane ahead).  grad,
-// s2 = delta S0 (|grad| - 1) and s3 = |grad| delta^2 are needed on the
-// tile + 1 ring for the 3x3x3 lambda quadrature (centre weight 51); the
-// in-plane 3x3 sums go through LDS and each thread keeps three planes of them
-// in registers, so the quadrature never re-reads a plane.  s2 and s3 vanish
-// where delta = 0, so a cell that does not update (the ring, and the two
-// extra planes k0-1 and k1) needs its gradient only inside the band: the
-// ring cells are a second pass of threads 0..RRING3-1 that does work only
-// for in-band cells, reading their z column from the cache.  With every
-// thread on a tile cell a 32 x 8 tile is a 4-wave workgroup; at about 160
-// VGPRs (FP64) three fit on a CU.
-// The RK stage is written in Shu-Osher form, algebraically identical to the
-// reference's increments: phi1 = phi0 - dtau l(phi0);
-// phi2 = 3/4 phi0 + 1/4 (phi1 - dtau l(phi1)); phi3 = 1/3 phi0 + 2/3 (phi2 - dtau l(phi2)).
-// Stage 3 also reduces max |phi3 - phi0| (:46-62).
+// Reinitialisation stage, 2D (level_set_rk3_redis.f90:176-264 with the
+// gradient :266-369, lambda :371-454 and ls_funs ns_solver/others.f90:1-45);
+// the 3D stage is in lsmp_redis.hip.
 // ---------------------------------------------------------------------------
-constexpr int RNT3 = R3X * R3Y;                   // one thread per tile cell
-constexpr int RHALO = P3X * P3Y - R3X * R3Y;      // LDS plane cells outside the tile
-constexpr int RRING3 = 2 * E3X + 2 * R3Y;         // ring of the tile + 1 ring
-static_assert(RHALO <= 2 * RNT3 && RRING3 <= RNT3, "at most two halo cells and one ring cell per thread");
-
-// LDS position (row, col) of halo cell number t of a (TX + 8) x (TY + 8)
-// plane: four rows on top, four at the bottom, four columns left and right
-// of the TX x TY tile.
-template <int TX, int TY> LSMP_DEV void halo_pos(int t, int &r, int &c)
-{
-    constexpr int PX = TX + 8, PY = TY + 8;
-    if (t < 4 * PX) { r = t / PX; c = t % PX; }
-    else if (t < 8 * PX) { r = PY - 4 + (t - 4 * PX) / PX; c = (t - 4 * PX) % PX; }
-    else { const int u = t - 8 * PX; r = 4 + u / 8; c = u % 8 < 4 ? u % 8 : PX - 8 + u % 8; }
-}
 
 // Extended-tile position (row ey, column ex) of ring cell number u of a
 // TX x TY tile + 1 ring: the top and bottom rows, then the left and right
@@ -762,184 +728,6 @@ LSMP_DEV void ext_pos(int t, int &ey, int &ex)
 template <typename T> LSMP_DEV bool in_band(T x)
 {
     return !(x > T(1) - T(1.0e-12)) && !(x < T(-1) + T(1.0e-12));
-}
-
-template <typename T, int STAGE, int BTN, int KC>
-__global__ __launch_bounds__(RNT3) void k_redis_stage3d(const T *__restrict__ phi, T *__restrict__ phi_out,
-                                                        const T *__restrict__ phi0, const T *__restrict__ S0,
-                                                        RedisArgs ra, unsigned long long *ctl,
-                                                        const unsigned char *__restrict__ tiles)
-{
-    __shared__ T sphi[P3Y][P3X];
-    __shared__ T ss2[E3Y][E3X];
-    __shared__ T ss3[E3Y][E3X];
-    if (ctl[1]) return;   // the pseudo-time loop has stopped (k_redis_ctl)
-    const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
-    // narrow band: a workgroup with no cell of the tube keeps its cells.  Stage
-    // 1 writes the step's output buffer and stage 2 the one stage 3 reads, so
-    // both copy; stage 3 writes the buffer stage 1 filled, which holds them.
-    if (tiles && !tiles[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x]) {
-        if (STAGE == 3) return;
-        const int ci = blockIdx.x * R3X + (int)threadIdx.x % R3X, cj = blockIdx.y * R3Y + (int)threadIdx.x / R3X;
-        if (ci >= nx || cj >= ny) return;
-        const size_t c0 = (size_t)cj * nx + ci, pl = (size_t)nx * ny;
-        for (int k = blockIdx.z * KC; k < min((int)(blockIdx.z + 1) * KC, nz); ++k)
-            phi_out[(size_t)k * pl + c0] = phi[(size_t)k * pl + c0];
-        return;
-    }
-    const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy, inv12dz = (T)ra.inv12dz;
-    const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
-    const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
-    const int t = threadIdx.x;
-    const int i0 = blockIdx.x * R3X, j0 = blockIdx.y * R3Y, k0 = blockIdx.z * KC;
-    const int k1 = min(k0 + KC, nz);
-    const int tx = t % R3X, ty = t / R3X;
-    const int ex = tx + 1, ey = ty + 1;
-    // cells past the domain edge hold the boundary cell (zero-gradient ghosts)
-    const int gi = min(i0 + tx, nx - 1), gj = min(j0 + ty, ny - 1);
-    const bool inner = i0 + tx < nx && j0 + ty < ny;
-    const int pc = gi - (i0 - 4), pr = gj - (j0 - 4);
-    // halo cells t and t + RNT3 of the LDS plane are loaded by this thread
-    const bool halo = t < RHALO, halo2 = t + RNT3 < RHALO;
-    int hr = 0, hc = 0, hr2 = 0, hc2 = 0;
-    if (halo) halo_pos<R3X, R3Y>(t, hr, hc);
-    if (halo2) halo_pos<R3X, R3Y>(t + RNT3, hr2, hc2);
-    const bool rg = BTN == 1 && t < RRING3;
-    int qy = 0, qx = 0;
-    if (rg) ext_ring_pos<R3X, R3Y>(t, qy, qx);
-    const int qi = clampi(i0 - 1 + qx, 0, nx - 1), qj = clampi(j0 - 1 + qy, 0, ny - 1);
-    const int qc = qi - (i0 - 4), qr = qj - (j0 - 4);
-    const size_t plane = (size_t)nx * ny;
-    const size_t col = (size_t)gj * nx + gi;
-    const size_t hcol = (size_t)clampi(j0 - 4 + hr, 0, ny - 1) * nx + clampi(i0 - 4 + hc, 0, nx - 1);
-    const size_t hcol2 = (size_t)clampi(j0 - 4 + hr2, 0, ny - 1) * nx + clampi(i0 - 4 + hc2, 0, nx - 1);
-    const size_t qcol = (size_t)qj * nx + qi;
-    auto P = [&](int p) -> size_t { return (size_t)clampi(p, 0, nz - 1) * plane; };
-
-    T w[7];
-#pragma unroll
-    for (int d = 0; d < 7; ++d) w[d] = phi[P(k0 - 4 + d) + col];
-    T halo_cur = halo ? phi[P(k0 - 1) + hcol] : T(0);
-    T halo2_cur = halo2 ? phi[P(k0 - 1) + hcol2] : T(0);
-    T s0_cur = S0[P(k0 - 1) + col];
-    T p0_cur = T(0);
-    T grad_q = T(0), del_q = T(0), s0_q = T(0), phi_q = T(0), s2_q = T(0), s3_q = T(0);
-    // in-plane 3x3 sums of s2 and s3 around this cell for planes p-2, p-1, p
-    T b2m = T(0), b2c = T(0), b2p = T(0), b3m = T(0), b3c = T(0), b3p = T(0);
-    double lerr = 0.0;
-    for (int p = k0 - 1; p <= k1; ++p) {
-        // next iteration's inputs, in flight while this plane is computed
-        const T w_next = phi[P(p + 4) + col];
-        const T halo_next = halo ? phi[P(p + 1) + hcol] : T(0);
-        const T halo2_next = halo2 ? phi[P(p + 1) + hcol2] : T(0);
-        const T s0_next = S0[P(p + 1) + col];
-        const T p0_next = (STAGE > 1 && inner && p >= k0 && p < k1) ? phi0[(size_t)p * plane + col] : T(0);
-
-        sphi[ty + 4][tx + 4] = w[3];
-        if (halo) sphi[hr][hc] = halo_cur;
-        if (halo2) sphi[hr2][hc2] = halo2_cur;
-        __syncthreads();
-        const bool live = p >= 0 && p < nz;
-        T grad_p = T(0), del_p = T(0), s2_p = T(0), s3_p = T(0);
-        const T s0_p = s0_cur, phi_p = w[3];
-        const bool centre = inner && p >= k0 && p < k1;
-        if (live && (centre || (BTN == 1 && in_band(w[3] * inv_w)))) {
-            T xs1, xs2, ys1, ys2, zs1, zs2;
-            weno_hj_pm_fast(sphi[pr][pc - 3], sphi[pr][pc - 2], sphi[pr][pc - 1], w[3], sphi[pr][pc + 1],
-                            sphi[pr][pc + 2], sphi[pr][pc + 3], inv12dx, invdx, xs1, xs2);
-            weno_hj_pm_fast(sphi[pr - 3][pc], sphi[pr - 2][pc], sphi[pr - 1][pc], w[3], sphi[pr + 1][pc],
-                            sphi[pr + 2][pc], sphi[pr + 3][pc], inv12dy, invdy, ys1, ys2);
-            weno_hj_pm_fast(w[0], w[1], w[2], w[3], w[4], w[5], w[6], inv12dz, invdz, zs1, zs2);
-            grad_p = godunov<T, 3>(xs1, xs2, ys1, ys2, zs1, zs2, s0_p);
-            if (BTN == 1) {
-                T H;
-                ls_funs_fast(w[3], inv_w, H, del_p);
-                s2_p = del_p * s0_p * (grad_p - T(1));
-                s3_p = grad_p * del_p * del_p;
-            }
-        }
-        if (BTN == 1 && live) {
-            ss2[ey][ex] = s2_p;
-            ss3[ey][ex] = s3_p;
-        }
-        if (rg && live) {
-            // ring cell of the extended tile: s2, s3 only, and only in the band
-            const T rphi = sphi[qr][qc];
-            T r2 = T(0), r3 = T(0);
-            if (in_band(rphi * inv_w)) {
-                T z[7];
-#pragma unroll
-                for (int d = 0; d < 7; ++d) z[d] = phi[P(p - 3 + d) + qcol];
-                const T rs0 = S0[(size_t)p * plane + qcol];
-                T xs1, xs2, ys1, ys2, zs1, zs2;
-                weno_hj_pm_fast(sphi[qr][qc - 3], sphi[qr][qc - 2], sphi[qr][qc - 1], rphi, sphi[qr][qc + 1],
-                                sphi[qr][qc + 2], sphi[qr][qc + 3], inv12dx, invdx, xs1, xs2);
-                weno_hj_pm_fast(sphi[qr - 3][qc], sphi[qr - 2][qc], sphi[qr - 1][qc], rphi, sphi[qr + 1][qc],
-                                sphi[qr + 2][qc], sphi[qr + 3][qc], inv12dy, invdy, ys1, ys2);
-                weno_hj_pm_fast(z[0], z[1], z[2], z[3], z[4], z[5], z[6], inv12dz, invdz, zs1, zs2);
-                const T g = godunov<T, 3>(xs1, xs2, ys1, ys2, zs1, zs2, rs0);
-                T H, del;
-                ls_funs_fast(rphi, inv_w, H, del);
-                r2 = del * rs0 * (g - T(1));
-                r3 = g * del * del;
-            }
-            ss2[qy][qx] = r2;
-            ss3[qy][qx] = r3;
-        }
-        __syncthreads();
-        if (BTN == 1 && inner && live) {
-            // 3x3 in-plane part of the lambda quadrature; the z direction is
-            // summed from the planes kept in registers
-            T a = T(0), b = T(0);
-#pragma unroll
-            for (int jj = -1; jj <= 1; ++jj)
-#pragma unroll
-                for (int ii = -1; ii <= 1; ++ii) {
-                    a = a + ss2[ey + jj][ex + ii];
-                    b = b + ss3[ey + jj][ex + ii];
-                }
-            b2m = b2c; b2c = b2p; b2p = a;
-            b3m = b3c; b3c = b3p; b3p = b;
-        } else if (p >= nz) {
-            // plane nz is the ghost copy of plane nz-1
-            b2m = b2c; b2c = b2p;
-            b3m = b3c; b3c = b3p;
-        }
-        const int q = p - 1;
-        if (inner && q >= k0 && q < k1) {
-            T lam = T(0);
-            if (BTN == 1) {
-                const T lo2 = q == 0 ? b2c : b2m, lo3 = q == 0 ? b3c : b3m;   // plane -1 = plane 0
-                const T a = T(51) * s2_q + ((lo2 + b2c) + b2p);
-                const T b = T(51) * s3_q + ((lo3 + b3c) + b3p);
-                if (fabs(b) > T(1.0e-12)) lam = a / b * del_q;
-            }
-            const T l = (s0_q - lam) * grad_q - s0_q;
-            const size_t gidx = (size_t)q * plane + col;
-            T pn;
-            if (STAGE == 1) {
-                pn = phi_q - rdt * l;
-            } else if (STAGE == 2) {
-                pn = T(0.75) * p0_cur + T(0.25) * (phi_q - rdt * l);
-            } else {
-                pn = p0_cur * T(1.0 / 3.0) + T(2.0 / 3.0) * (phi_q - rdt * l);
-                lerr = fmax(lerr, (double)fabs(pn - p0_cur));
-            }
-            phi_out[gidx] = pn;
-        }
-#pragma unroll
-        for (int d = 0; d < 6; ++d) w[d] = w[d + 1];
-        w[6] = w_next;
-        halo_cur = halo_next;
-        halo2_cur = halo2_next;
-        s0_cur = s0_next;
-        p0_cur = p0_next;
-        grad_q = grad_p; del_q = del_p; s0_q = s0_p; phi_q = phi_p; s2_q = s2_p; s3_q = s3_p;
-    }
-    if (STAGE == 3) {
-        double m = block_max(lerr);
-        if (t == 0) atomic_max_nonneg(ctl, m);
-    }
 }
 
 // 2D stage: one tile, no marching (Disk/ and Vortex2D/ forms: |grad| without
@@ -1048,22 +836,22 @@ __global__ void k_sign_init(const T *__restrict__ phi, T *__restrict__ S0, size_
 // Device-side control of the pseudo-time loop (level_set_rk3_redis.f90:17-23, 68):
 // ctl[0] max |dphi| of the running pseudo-step (bits of a non-negative double),
 // ctl[1] stop flag, ctl[2] pseudo-steps run, ctl[3] max |dphi| of the last one.
-// op 0 resets, op 1 opens a pseudo-step, op 2 closes it and stops the loop once
-// the step's max |dphi| <= 1e-6 (a NaN never stops it, as in the reference);
-// the time limit is the host's count of launched steps.  Stage kernels return
-// at once when the flag is set, so steps can be queued ahead without host
-// round trips and the step count stays the reference's.
+// op 0 resets, op 2 closes a pseudo-step -- keeps its max |dphi|, clears the
+// running one for the next step and stops the loop once the step's max |dphi|
+// <= 1e-6 (a NaN never stops it, as in the reference); the time limit is the
+// host's count of launched steps.  Stage kernels return at once when the flag
+// is set, so steps can be queued ahead without host round trips and the step
+// count stays the reference's.
 __global__ void k_redis_ctl(unsigned long long *ctl, int op)
 {
     if (threadIdx.x != 0) return;
     if (op == 0) {
         ctl[0] = 0; ctl[1] = 0; ctl[2] = 0; ctl[3] = 0;
-    } else if (op == 1) {
-        if (!ctl[1]) ctl[0] = 0;
     } else if (!ctl[1]) {
         ctl[2] += 1;
         ctl[3] = ctl[0];
-        const double e = __longlong_as_double((long long)ctl[0]);
+        ctl[0] = 0;
+        const double e = __longlong_as_double((long long)ctl[3]);
         if (e <= 1.0e-6) ctl[1] = 1;
     }
 }
@@ -2064,32 +1852,14 @@ void launch_tube_tiles(hipStream_t st, const T *phi, int nx, int ny, int nz, dou
 }
 
 template <typename T>
-void launch_redis_stage(hipStream_t st, int dim, int stage, int btn, const T *phi, T *phi_out, const T *phi0,
-                        const T *S0, const RedisArgs &ra, unsigned long long *ctl, const unsigned char *tiles)
+void launch_redis_stage2d(hipStream_t st, int stage, int btn, const T *phi, T *phi_out, const T *phi0, const T *S0,
+                          const RedisArgs &ra, unsigned long long *ctl)
 {
-    if (dim == 3) {
-        // the narrow band (tiles) marches RKC_NB planes per workgroup
-        const dim3 g(nblk(ra.nx, R3X), nblk(ra.ny, R3Y), nblk(ra.nz, RKC)),
-            gt(nblk(ra.nx, R3X), nblk(ra.ny, R3Y), nblk(ra.nz, RKC_NB));
-#define LSMP_R3(S, B)                                                                                          \
-    do {                                                                                                       \
-        if (tiles)                                                                                             \
-            hipLaunchKernelGGL((k_redis_stage3d<T, S, B, RKC_NB>), gt, dim3(RNT3), 0, st, phi, phi_out, phi0,   \
-                               S0, ra, ctl, tiles);                                                            \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_redis_stage3d<T, S, B, RKC>), g, dim3(RNT3), 0, st, phi, phi_out, phi0, S0,   \
-                               ra, ctl, tiles);                                                                \
-    } while (0)
-        if (btn) { if (stage == 1) LSMP_R3(1, 1); else if (stage == 2) LSMP_R3(2, 1); else LSMP_R3(3, 1); }
-        else { if (stage == 1) LSMP_R3(1, 0); else if (stage == 2) LSMP_R3(2, 0); else LSMP_R3(3, 0); }
-#undef LSMP_R3
-    } else {
-        const dim3 g(nblk(ra.nx, RTX), nblk(ra.ny, RTY), 1);
+    const dim3 g(nblk(ra.nx, RTX), nblk(ra.ny, RTY), 1);
 #define LSMP_R2(S, B) hipLaunchKernelGGL((k_redis_stage2d<T, S, B>), g, dim3(RNT), 0, st, phi, phi_out, phi0, S0, ra, ctl)
-        if (btn) { if (stage == 1) LSMP_R2(1, 1); else if (stage == 2) LSMP_R2(2, 1); else LSMP_R2(3, 1); }
-        else { if (stage == 1) LSMP_R2(1, 0); else if (stage == 2) LSMP_R2(2, 0); else LSMP_R2(3, 0); }
+    if (btn) { if (stage == 1) LSMP_R2(1, 1); else if (stage == 2) LSMP_R2(2, 1); else LSMP_R2(3, 1); }
+    else { if (stage == 1) LSMP_R2(1, 0); else if (stage == 2) LSMP_R2(2, 0); else LSMP_R2(3, 0); }
 #undef LSMP_R2
-    }
 }
 
 template <typename T>
@@ -2353,8 +2123,8 @@ void launch_zero_u64(hipStream_t st, unsigned long long *p, int n)
     template void launch_adv_pass<T>(hipStream_t, int, int, int, T *, const T *, T *, T *, T *, T *, int, int, int, \
                                      T, T, const int *, int, const TubeMask &);                                \
     template void launch_combine_markers<T>(hipStream_t, const T *, const T *, T *, size_t, unsigned *);         \
-    template void launch_redis_stage<T>(hipStream_t, int, int, int, const T *, T *, const T *, const T *,          \
-                                        const RedisArgs &, unsigned long long *, const unsigned char *);       \
+    template void launch_redis_stage2d<T>(hipStream_t, int, int, const T *, T *, const T *, const T *,             \
+                                          const RedisArgs &, unsigned long long *);                            \
     template void launch_tube_tiles<T>(hipStream_t, const T *, int, int, int, double, unsigned char *);          \
     template void launch_sign_init<T>(hipStream_t, const T *, T *, size_t, T, int);                            \
     template void launch_grad_max<T>(hipStream_t, int, const T *, const T *, const RedisArgs &,                \

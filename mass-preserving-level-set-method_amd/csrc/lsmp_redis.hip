@@ -144,16 +144,21 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     static_assert(KC * NT <= 65536, "local band-cell index in 16 bits");
     if (ctl[1]) return;   // the pseudo-time loop has stopped (k_redis_ctl)
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
-    const size_t tile_id = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    // tiles in x-fastest order, renumbered so that each XCD takes a contiguous
+    // run of them: the x and y halos of a tile are then mostly in its XCD's L2
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int tile_id = xcd_renumber((int)((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x),
+                                     gx * gy * (int)gridDim.z);
+    const int bx = tile_id % gx, by = (tile_id / gx) % gy, bz = tile_id / (gx * gy);
     // narrow band: a workgroup with no cell of the tube keeps its cells.  Stage
     // 1 writes the step's output buffer and stage 2 the one stage 3 reads, so
     // both copy; stage 3 writes the buffer stage 1 filled, which holds them.
     if (tiles && !tiles[tile_id]) {
         if (STAGE == 3) return;
-        const int ci = blockIdx.x * R3X + (int)threadIdx.x % R3X, cj = blockIdx.y * R3Y + (int)threadIdx.x / R3X;
+        const int ci = bx * R3X + (int)threadIdx.x % R3X, cj = by * R3Y + (int)threadIdx.x / R3X;
         if (ci >= nx || cj >= ny) return;
         const size_t c0 = (size_t)cj * nx + ci, pl = (size_t)nx * ny;
-        for (int k = blockIdx.z * KC; k < min((int)(blockIdx.z + 1) * KC, nz); ++k)
+        for (int k = bz * KC; k < min((bz + 1) * KC, nz); ++k)
             phi_out[(size_t)k * pl + c0] = phi[(size_t)k * pl + c0];
         return;
     }
@@ -161,7 +166,7 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
     const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
     const int t = threadIdx.x;
-    const int i0 = blockIdx.x * R3X, j0 = blockIdx.y * R3Y, k0 = blockIdx.z * KC;
+    const int i0 = bx * R3X, j0 = by * R3Y, k0 = bz * KC;
     const int k1 = min(k0 + KC, nz);
     const int tx = t % R3X, ty = t / R3X;
     // cells past the domain edge hold the boundary cell (zero-gradient ghosts)
@@ -323,11 +328,27 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     }
 }
 
-// delta of ls_funs_fast for a cell of the band (its middle branch)
+// delta of ls_funs for a cell of the band (its middle branch, |x| < 1 with
+// x = phi / ls_wid): 0.5 (1 + cos(pi x)) / ls_wid = cos(theta)^2 / ls_wid with
+// theta = pi x / 2 in (-pi/2, pi/2), cos from its Taylor series in theta^2 to
+// the term in theta^20 (remainder below 2e-17 there).  Within 2.3e-16 of
+// 0.5 (1 + cos(pi x)) evaluated with a correctly rounded cosine.
 template <typename T> LSMP_DEV T band_delta(T phi, T inv_w)
 {
-    const T pi = T(3.141592653589793238462643383279502884);
-    return T(0.5) * (T(1) + cos(pi * (phi * inv_w))) * inv_w;
+    const T th = (phi * inv_w) * T(1.570796326794896619231321691639751442);
+    const T t = th * th;
+    T c = T(1.0 / 2432902008176640000.0);     // 1/20!
+    c = fma(c, t, T(-1.0 / 6402373705728000.0));   // -1/18!
+    c = fma(c, t, T(1.0 / 20922789888000.0));      // 1/16!
+    c = fma(c, t, T(-1.0 / 87178291200.0));        // -1/14!
+    c = fma(c, t, T(1.0 / 479001600.0));           // 1/12!
+    c = fma(c, t, T(-1.0 / 3628800.0));            // -1/10!
+    c = fma(c, t, T(1.0 / 40320.0));               // 1/8!
+    c = fma(c, t, T(-1.0 / 720.0));                // -1/6!
+    c = fma(c, t, T(1.0 / 24.0));                  // 1/4!
+    c = fma(c, t, T(-0.5));                        // -1/2!
+    c = fma(c, t, T(1));
+    return c * c * inv_w;
 }
 
 // The listed band cells: lambda from the 3x3x3 box sums of s2, s3 (centre
