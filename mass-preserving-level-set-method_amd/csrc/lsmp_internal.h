@@ -62,6 +62,13 @@ struct VelArgs {
 // Narrow band: the RK stage of the last direction updates a cell only when its
 // x line and, in 3D, its y line cross the tube (fx, fy: the band line flags of
 // those directions); fx == nullptr updates every cell.
+// rows per lane at most in the partitioned line solves (8 up to 257 cells,
+// 16 up to 513)
+constexpr int PKX = 16;
+// lines of an advection pass from which the tiled solve (lsmp_adv_tile.hip)
+// takes over from the paired, twisted and partitioned ones
+constexpr int TILE_MIN_LINES = 4096;
+
 struct TubeMask {
     const unsigned char *fx = nullptr, *fy = nullptr;
     int nx = 0, ny = 0;
@@ -97,6 +104,10 @@ template <typename T> struct RedisBand {
     T *gr = nullptr;
     unsigned *list = nullptr, *count = nullptr, *next = nullptr;
 };
+template <typename T>
+void launch_adv_tile(hipStream_t st, int mode, int stage, T *phi, const T *vel, T *R, T *L1, T *L2, int nlines, int n,
+                     int lw, long ostride, long istride, long estride, T dd, T dt, const int *lines,
+                     const TubeMask &tm);
 template <typename T>
 void launch_redis3d(hipStream_t st, int stage, int btn, const T *phi, T *phi_out, const T *phi0, const T *S0,
                     const RedisArgs &ra, unsigned long long *ctl, const unsigned char *tiles, const RedisBand<T> &bd,

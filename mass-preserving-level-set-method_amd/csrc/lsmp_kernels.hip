@@ -514,7 +514,6 @@ __global__ __launch_bounds__(256) void k_adv_pair_x(const T *__restrict__ phi, c
 // row of the mirrored stencil with A and C exchanged); the solve differs from
 // one Thomas sweep only in rounding.  In MODE 2 the in-place update is safe:
 // a line's cells are read by its own wave only, all before the reduction.
-constexpr int PKX = 16;   // rows per lane at most (8 up to 257 cells, 16 up to 513)
 
 template <typename T, int MODE, int STAGE, int KR, bool TUBE>
 __global__ __launch_bounds__(256) void k_adv_part(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
@@ -1769,6 +1768,12 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
     // as fast (FP64) or faster (FP32 182 against 220 us).  Listed lines (the
     // narrow band) take it up to PART_MAX_LISTED_ADV strided lines.
     int nlines = lines ? nlist : nall;
+    // many lines: the tiled solve reads and writes each field once, coalesced
+    if (n >= 34 && n <= 32 * PKX + 1 && nlines >= TILE_MIN_LINES) {
+        launch_adv_tile<T>(st, mode, stage, phi, vel, R, L1, L2, nlines, n, lw, ostride, istride, estride, dd, dt,
+                           lines, tm);
+        return;
+    }
     const bool part = n >= 34 && n <= 32 * PKX + 1 &&
                       nlines <= (dir == 0 ? PART_MAX_LINES_X : lines ? PART_MAX_LISTED_ADV : PART_MAX_LINES_STRIDED);
     if (mode == 0 && !part && lines) {   // the paired x kernel takes every line

@@ -78,6 +78,11 @@ def ids(c):
 # the mass correction's line lists and flags, in 3D
 CASES_MANY_LINES = [dict(nx=72, ny=66, nz=68, dim=3)]
 
+# advection passes of 4096 lines or more take the tiled line solve: lines of
+# 64..72 cells in 3D, x lines of 300 cells (16 rows per lane), and 2D y lines
+# of 40 cells whose last workgroup of 8 lines holds 4
+CASES_TILED = CASES_MANY_LINES + [dict(nx=300, ny=64, nz=64, dim=3), dict(nx=4100, ny=40, dim=2)]
+
 
 @pytest.mark.parametrize("case", CASES, ids=ids)
 def test_upload_download_roundtrip(case):
@@ -89,7 +94,7 @@ def test_upload_download_roundtrip(case):
     s.close()
 
 
-@pytest.mark.parametrize("case", CASES, ids=ids)
+@pytest.mark.parametrize("case", CASES + CASES_TILED, ids=ids)
 def test_advect_parity(case):
     o, s = pair(**case)
     say("advect", case)
