@@ -36,12 +36,19 @@ constexpr int OFF_W3 = OFF_B2 + NH;          // W3[k], k < NH
 constexpr int OFF_B3 = OFF_W3 + NH;
 constexpr int NPARAM = OFF_B3 + 1;           // 12481
 constexpr int TILE = 64;                     // samples per workgroup (4 waves x 16)
+// LDS row pitches (in bf16) of the MFMA operand tiles: 16 bytes past a
+// multiple of the bank width, so that the 16 rows a lane group reads at once
+// fall on different banks
+constexpr int KP1 = NF + 8, KP2 = NH + 8, KPT = TILE + 8;
 
 LSMP_DEV f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
-LSMP_DEV float silu(float z) { return z / (1.0f + expf(-z)); }
+// the logistic function from the hardware 2^x and reciprocal (each within
+// about an ulp): SiLU to a few ulp, far finer than the BF16 products it feeds
+LSMP_DEV float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z)); }
+LSMP_DEV float silu(float z) { return z * sigm(z); }
 LSMP_DEV float dsilu(float z)
 {
-    const float s = 1.0f / (1.0f + expf(-z));
+    const float s = sigm(z);
     return s * (1.0f + z * (1.0f - s));
 }
 LSMP_DEV float group16_sum(float v)   // sum over the 16 lanes sharing lane/16
@@ -269,16 +276,16 @@ LSMP_DEV void split_bf16(float v, __bf16 &hi, __bf16 &lo)
 }
 
 struct NetLds {
-    __attribute__((aligned(16))) __bf16 w1h[NH][NF];   // W1 transposed, [n][k], hi part
-    __attribute__((aligned(16))) __bf16 w1l[NH][NF];   // lo part
-    __attribute__((aligned(16))) __bf16 w2h[NH][NH];   // W2 transposed, [n2][n1]
-    __attribute__((aligned(16))) __bf16 w2l[NH][NH];
+    __attribute__((aligned(16))) __bf16 w1h[NH][KP1];   // W1 transposed, [n][k], hi part
+    __attribute__((aligned(16))) __bf16 w1l[NH][KP1];   // lo part
+    __attribute__((aligned(16))) __bf16 w2h[NH][KP2];   // W2 transposed, [n2][n1]
+    __attribute__((aligned(16))) __bf16 w2l[NH][KP2];
     float b1[NH], b2[NH], w3[NH];
     float b3;
 };
 
 // w2: optional [n1][n2] copy of the hi part of W2 for the backward pass.
-LSMP_DEV void stage_weights(NetLds &L, const float *__restrict__ P, __bf16 (*w2)[NH])
+LSMP_DEV void stage_weights(NetLds &L, const float *__restrict__ P, __bf16 (*w2)[KP2])
 {
     for (int idx = threadIdx.x; idx < NF * NH; idx += blockDim.x) {
         const int k = idx / NH, n = idx % NH;
@@ -301,8 +308,8 @@ LSMP_DEV void stage_weights(NetLds &L, const float *__restrict__ P, __bf16 (*w2)
 // (hi and lo parts).  Returns z1, z2 (pre-activations, D layout) and the
 // outputs o[i] of rows 4*(lane/16)+i; sh/sl are this wave's 16x64 LDS tiles
 // for the H1 relayout (hi and lo parts; sh is kept for the backward pass).
-LSMP_DEV void net_forward(const NetLds &L, const bf16x8 (&xh)[4], const bf16x8 (&xl)[4], __bf16 (*sh)[NH],
-                          __bf16 (*sl)[NH], f32x4 (&z1)[4], f32x4 (&z2)[4], float (&o)[4])
+LSMP_DEV void net_forward(const NetLds &L, const bf16x8 (&xh)[4], const bf16x8 (&xl)[4], __bf16 (*sh)[KP2],
+                          __bf16 (*sl)[KP2], f32x4 (&z1)[4], f32x4 (&z2)[4], float (&o)[4])
 {
     const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
 #pragma unroll
@@ -395,14 +402,14 @@ __global__ __launch_bounds__(256) void k_net_grad(const float *__restrict__ P, c
                                                   double *__restrict__ lpart)
 {
     __shared__ NetLds L;
-    __shared__ __attribute__((aligned(16))) __bf16 w2[NH][NH];        // [n1][n2]
-    __shared__ __attribute__((aligned(16))) __bf16 sh1[4][16][NH];     // per wave [row][n1], hi part
-    __shared__ __attribute__((aligned(16))) __bf16 sl1[4][16][NH];     // lo part
-    __shared__ __attribute__((aligned(16))) __bf16 sdz2[4][16][NH];    // per wave [row][n2]
-    __shared__ __attribute__((aligned(16))) __bf16 h1t[NH][TILE];      // [n1][sample]
-    __shared__ __attribute__((aligned(16))) __bf16 dz2t[NH][TILE];     // [n2][sample]
-    __shared__ __attribute__((aligned(16))) __bf16 dz1t[NH][TILE];     // [n1][sample]
-    __shared__ __attribute__((aligned(16))) __bf16 xt[NF][TILE];       // [feature][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 w2[NH][KP2];        // [n1][n2]
+    __shared__ __attribute__((aligned(16))) __bf16 sh1[4][16][KP2];     // per wave [row][n1], hi part
+    __shared__ __attribute__((aligned(16))) __bf16 sl1[4][16][KP2];     // lo part
+    __shared__ __attribute__((aligned(16))) __bf16 sdz2[4][16][KP2];    // per wave [row][n2]
+    __shared__ __attribute__((aligned(16))) __bf16 h1t[NH][KPT];        // [n1][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 dz2t[NH][KPT];       // [n2][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 dz1t[NH][KPT];       // [n1][sample]
+    __shared__ __attribute__((aligned(16))) __bf16 xt[NF][KPT];         // [feature][sample]
     __shared__ float red[4][3 * NH + 2];
     stage_weights(L, P, w2);
     __syncthreads();
@@ -573,9 +580,17 @@ __global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T 
                                                    const float *__restrict__ P, T h, T w)
 {
     __shared__ NetLds L;
-    __shared__ __attribute__((aligned(16))) __bf16 sh1[4][16][NH];
-    __shared__ __attribute__((aligned(16))) __bf16 sl1[4][16][NH];
+    __shared__ __attribute__((aligned(16))) __bf16 sh1[4][16][KP2];
+    __shared__ __attribute__((aligned(16))) __bf16 sl1[4][16][KP2];
     __shared__ float sout[4][16];
+    // stencil offset of feature q in cells, for cells 2 or more cells from
+    // every face; features 125..127 take the stencil's last cell and do not use it
+    __shared__ __attribute__((aligned(16))) int soff[NF];
+    const long sy = g.nx, sz = (long)g.nx * g.ny;
+    if (threadIdx.x < NF) {
+        const int q = min((int)threadIdx.x, 124);
+        soff[threadIdx.x] = (q % 5 - 2) + (int)sy * ((q / 5) % 5 - 2) + (int)sz * (q / 25 - 2);
+    }
     stage_weights(L, P, nullptr);
     __syncthreads();
     const int t = threadIdx.x, wv = t >> 6, l = t & 63, lr = l & 15, lg = l >> 4;
@@ -583,22 +598,49 @@ __global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T 
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int c = tile * TILE + wv * 16 + lr;
         const bool valid = c < nlist;
-        int idx = 0, i = 0, j = 0, k = 0;
-        T pc = T(0);
-        if (valid) {
-            idx = list[c];
-            i = idx % g.nx; j = (idx / g.nx) % g.ny; k = idx / (g.nx * g.ny);
-            pc = phi[idx];
+        // lanes past the list's end take the tile's first cell and store nothing
+        const int idx = list[valid ? c : tile * TILE];
+        const int i = idx % g.nx, j = (idx / g.nx) % g.ny, k = idx / (g.nx * g.ny);
+        const T pc = phi[idx];
+        // the 32 stencil values and the 12 of |grad phi|, all loads in flight
+        // at once: from the offsets when no lane of the wave needs clamping,
+        // else with clamped indices (the zero-gradient ghost rule)
+        const bool inner = i >= 2 && i < g.nx - 2 && j >= 2 && j < g.ny - 2 && k >= 2 && k < g.nz - 2;
+        T sv[32];
+        float gm;
+        if (__all(inner)) {
+            const T *pcell = phi + idx;
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                for (int jj = 0; jj < 8; ++jj) sv[8 * ks + jj] = pcell[soff[32 * ks + 8 * lg + jj]];
+            auto d4 = [&](long st) {
+                return (float)((pcell[-2 * st] - pcell[2 * st]) + T(8) * (pcell[st] - pcell[-st])) * g.inv_12h;
+            };
+            const float gx = d4(1), gy = d4(sy), gz = d4(sz);
+            gm = sqrtf(gx * gx + gy * gy + gz * gz);
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                for (int jj = 0; jj < 8; ++jj) {
+                    const int q = min(32 * ks + 8 * lg + jj, 124);
+                    sv[8 * ks + jj] = load_clamped(phi, g, i + q % 5 - 2, j + (q / 5) % 5 - 2, k + q / 25 - 2);
+                }
+            gm = grad_norm4(phi, g, i, j, k);
         }
-        // |grad phi| once per lane (features 126 and 127 of the lanes lg = 3)
-        const float gm = valid && lg == 3 ? grad_norm4(phi, g, i, j, k) : 1.0f;
         bf16x8 xh[4], xl[4];
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
             for (int jj = 0; jj < 8; ++jj) {
+                const int q = 32 * ks + 8 * lg + jj;
+                float f = (float)(sv[8 * ks + jj] - pc) * g.inv_h;   // feature() for q < 125
+                if (q == 125) f = (float)pc * g.inv_h;
+                else if (q == 126) f = gm - 1.0f;
+                else if (q == 127) f = (float)pc * g.inv_h * (1.0f / fmaxf(gm, 0.1f) - 1.0f);
                 __bf16 hi, lo;
-                split_bf16(valid ? feature(phi, g, i, j, k, pc, 32 * ks + 8 * lg + jj, gm) : 0.0f, hi, lo);
+                split_bf16(valid ? f : 0.0f, hi, lo);
                 xh[ks][jj] = hi;
                 xl[ks][jj] = lo;
             }
