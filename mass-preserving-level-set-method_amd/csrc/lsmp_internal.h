@@ -47,6 +47,9 @@ constexpr int RKC_NB = 8;
 struct RedisArgs {
     int nx, ny, nz;
     double inv12dx, inv12dy, inv12dz, invdx, invdy, invdz, ls_wid, inv_ls_wid, rdt;
+    // 3.0 as an argument, so that it sits in a scalar register: a three-operand
+    // VALU instruction of gfx9 takes no literal operand
+    double three = 3.0;
 };
 
 struct CcdArgs {

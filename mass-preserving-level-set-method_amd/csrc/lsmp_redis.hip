@@ -69,8 +69,8 @@ template <typename T> LSMP_DEV bool in_band_x(T x)
 // quadratic forms 13 (a - b)^2 + 3 (a - 3b)^2 = 4 (4a^2 - 11ab + 10b^2) and
 // 13 (a - b)^2 + 3 (a + b)^2 = 4 (4a^2 - 5ab + 4b^2) over shared squares.
 // FP64 takes both reciprocals from one (the product of the two denominators
-// stays within [1e-48, 1e28]).
-template <typename T> LSMP_DEV void weno_hj_face(T sa, T sb, T sc, T sd, T &plus, T &minus)
+// stays within [1e-48, 1e28]).  three: 3.0.
+template <typename T> LSMP_DEV void weno_hj_face(T sa, T sb, T sc, T sd, T three, T &plus, T &minus)
 {
     const T eps = T(1.0e-6);
     const T bb = sb * sb, cc = sc * sc;
@@ -80,7 +80,7 @@ template <typename T> LSMP_DEV void weno_hj_face(T sa, T sb, T sc, T sd, T &plus
     const T qA = sq(fma(T(4), isA, eps)), qJ = sq(fma(T(4), isJ, eps)), qB = sq(fma(T(4), isB, eps));
     const T pJB = qJ * qB, pAJ = qA * qJ, pAB = qA * qB;
     // plus: q0 = qA, q1 = qJ, q2 = qB; minus: q0 = qB, q1 = qJ, q2 = qA
-    const T dp = pJB + T(6) * pAB + T(3) * pAJ, dm = pAJ + T(6) * pAB + T(3) * pJB;
+    const T dp = fma(three, fma(T(2), pAB, pAJ), pJB), dm = fma(three, fma(T(2), pAB, pJB), pAJ);
     T rp, rm;
     if constexpr (sizeof(T) == 8) {
         const T r = rcp_w(dp * dm);
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     }
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy, inv12dz = (T)ra.inv12dz;
     const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
-    const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
+    const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt, c3 = (T)ra.three;
     const int t = threadIdx.x;
     const int i0 = bx * R3X, j0 = by * R3Y, k0 = bz * KC;
     const int k1 = min(k0 + KC, nz);
@@ -188,10 +188,13 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
 
     // the faces i+1/2 and j+1/2 read the LDS plane at the thread's own tile
     // position: past the domain edge it holds the boundary cell, as the halo
-    // does.  Wave 0's lanes 0..R3Y-1 also take the x face i0-1/2 of row t,
-    // lanes R3Y..R3Y+R3X-1 the y face j0-1/2 of column t-R3Y.
-    const bool first_x = t < R3Y, first_y = t >= R3Y && t < R3Y + R3X;
-    const int f_off = first_x ? (t + 4) * P3X + 1 : (first_y ? P3X + 4 + (t - R3Y) : 0);
+    // does.  One wave also takes the tile's first faces -- its lanes
+    // 0..R3Y-1 the x face i0-1/2 of row u, lanes R3Y..R3Y+R3X-1 the y face
+    // j0-1/2 of column u-R3Y (u the lane) -- a different wave from tile to
+    // tile, so that the extra work spreads over the SIMDs of a CU
+    const int fu = t - 64 * (tile_id % (NT / 64));
+    const bool first_x = fu >= 0 && fu < R3Y, first_y = fu >= R3Y && fu < R3Y + R3X;
+    const int f_off = first_x ? (fu + 4) * P3X + 1 : (first_y ? P3X + 4 + (fu - R3Y) : 0);
     const int f_str = first_x ? 1 : P3X;
 
     T w[7];   // phi(p-3 .. p+3) of this cell's column
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     {
         T unused;
         const T zd = w[3] - T(2) * w[4] + w[5];   // s(k0 + 1)
-        weno_hj_face(za, zb, zc, zd, unused, zminus);
+        weno_hj_face(za, zb, zc, zd, c3, unused, zminus);
         za = zb; zb = zc; zc = zd;
     }
     T halo_cur = halo ? load_at(base(phi, k0), hcol) : T(0);
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
         // z face p + 1/2: its D+ term is this plane's, its D- term the next one's
         const T zs_new = w[4] - T(2) * w[5] + w[6];   // s(p + 2)
         T zplus, zminus_next;
-        weno_hj_face(za, zb, zc, zs_new, zplus, zminus_next);
+        weno_hj_face(za, zb, zc, zs_new, c3, zplus, zminus_next);
         const T az = inv12dz * (-(w[2] - w[1]) + T(7) * (w[3] - w[2]) + T(7) * (w[4] - w[3]) - (w[5] - w[4]));
         const T zs1 = az + invdz * zplus, zs2 = az - invdz * zminus;
         __syncthreads();
@@ -243,8 +246,8 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
             const T m2 = r[-2], m1 = r[-1], c = w[3], p1 = r[1], p2 = r[2], p3 = r[3];
             ax = inv12dx * (-(m1 - m2) + T(7) * (c - m1) + T(7) * (p1 - c) - (p2 - p1));
             T plus, minus;
-            weno_hj_face(m2 - T(2) * m1 + c, m1 - T(2) * c + p1, c - T(2) * p1 + p2, p1 - T(2) * p2 + p3, plus,
-                         minus);
+            weno_hj_face(m2 - T(2) * m1 + c, m1 - T(2) * c + p1, c - T(2) * p1 + p2, p1 - T(2) * p2 + p3, c3,
+                         plus, minus);
             xs1 = ax + invdx * plus;
             mxs[ty][tx + 1] = minus;
         }
@@ -254,8 +257,8 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
             const T m2 = r[-2 * P3X], m1 = r[-P3X], c = w[3], p1 = r[P3X], p2 = r[2 * P3X], p3 = r[3 * P3X];
             ay = inv12dy * (-(m1 - m2) + T(7) * (c - m1) + T(7) * (p1 - c) - (p2 - p1));
             T plus, minus;
-            weno_hj_face(m2 - T(2) * m1 + c, m1 - T(2) * c + p1, c - T(2) * p1 + p2, p1 - T(2) * p2 + p3, plus,
-                         minus);
+            weno_hj_face(m2 - T(2) * m1 + c, m1 - T(2) * c + p1, c - T(2) * p1 + p2, p1 - T(2) * p2 + p3, c3,
+                         plus, minus);
             ys1 = ay + invdy * plus;
             mys[ty + 1][tx] = minus;
         }
@@ -264,10 +267,10 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
             const T v0 = f[0], v1 = f[f_str], v2 = f[2 * f_str], v3 = f[3 * f_str], v4 = f[4 * f_str],
                     v5 = f[5 * f_str];
             T plus, minus;
-            weno_hj_face(v0 - T(2) * v1 + v2, v1 - T(2) * v2 + v3, v2 - T(2) * v3 + v4, v3 - T(2) * v4 + v5, plus,
-                         minus);
-            if (first_x) mxs[t][0] = minus;
-            else mys[0][t - R3Y] = minus;
+            weno_hj_face(v0 - T(2) * v1 + v2, v1 - T(2) * v2 + v3, v2 - T(2) * v3 + v4, v3 - T(2) * v4 + v5, c3,
+                         plus, minus);
+            if (first_x) mxs[fu][0] = minus;
+            else mys[0][fu - R3Y] = minus;
         }
         __syncthreads();
         const T xs2 = ax - invdx * mxs[ty][tx];
