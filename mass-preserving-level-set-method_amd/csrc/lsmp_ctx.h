@@ -64,8 +64,6 @@ struct lsmp_ctx {
     void *nrm[3] = {nullptr, nullptr, nullptr};
     double *ftab = nullptr;
     bool vof_set = false, vof_old_set = false, vof0_set = false, face_set = false;
-    bool wlic_stencil = false;   // LSMP_WLIC_STENCIL=1: the WLIC cell update that loads its stencil first
-    bool cut_lines = false;      // LSMP_CUT_LINES=1: the WLIC normals only on the lines holding a cut cell
     bool vel_set = false, phi0_set = false;
     lsmp_stats stats{};
     bool profiling = false;

@@ -645,10 +645,6 @@ int ensure_clsvof(lsmp_ctx *c)
         }
     if (!c->ftab) {
         LSMP_HIP(hipMalloc((void **)&c->ftab, sizeof(double) * 3 * (size_t)(c->nx + c->ny + c->nz)));
-        const char *e = std::getenv("LSMP_WLIC_STENCIL");
-        c->wlic_stencil = e && std::atoi(e) != 0;
-        e = std::getenv("LSMP_CUT_LINES");
-        c->cut_lines = e && std::atoi(e) != 0;
     }
     LSMP_HIP(hipStreamSynchronize(c->st));
     return LSMP_OK;
@@ -675,11 +671,12 @@ template <typename T> int vof_wlic(lsmp_ctx *c, double dt)
 {
     const CcdArgs ca = ccd_args(c);
     static const char *const nname[3] = {"norm_x", "norm_y", "norm_z"};
-    // the fluxes read the normals of cut cells only: with cut_lines the CCD
-    // solves run on the lines that hold one (the same solves, fewer lines)
+    // the fluxes read the normals of cut cells only: the CCD solves run on the
+    // lines that hold a cell a step can find cut (the same solves, fewer
+    // lines; at 256^3 0.29 ms against 0.46 ms for every line)
     const int *lst[3] = {nullptr, nullptr, nullptr};
     int nlst[3] = {0, 0, 0};
-    if (c->cut_lines) {
+    {
         KT(c, "cut_lines", (launch_cut_lines<T>(c->st, c->dim, (const T *)c->vof, c->nx, c->ny, c->nz, c->lflag,
                                                 c->lcount, c->llist[0], c->llist[1], c->llist[2])));
         unsigned *hc = reinterpret_cast<unsigned *>(c->hbuf);
@@ -701,7 +698,7 @@ template <typename T> int vof_wlic(lsmp_ctx *c, double dt)
     for (int d = 0; d < c->dim; ++d) {
         KT(c, "wlic", (launch_wlic_pass<T>(c->st, d, d == c->dim - 1, in, out, (const T *)c->vof_old, vel[d],
                                            (const T *)c->nrm[0], (const T *)c->nrm[1], n2, c->nx, c->ny, c->nz, dt,
-                                           dd[d], c->wlic_stencil)));
+                                           dd[d])));
         std::swap(in, out);
     }
     c->vof = in;

@@ -182,8 +182,7 @@ void launch_face_velocity(hipStream_t st, T *uf, T *vf, T *wf, const FaceVelArgs
 // last direction; stencil: the variant that loads each cell's stencil first)
 template <typename T>
 void launch_wlic_pass(hipStream_t st, int dir, bool clamp, const T *in, T *out, const T *old, const T *vel,
-                      const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd,
-                      bool stencil);
+                      const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd);
 // flags and ordered lists of the lines (x, y and, in 3D, z) through the cells
 // that a WLIC step can find cut (wlic_may_cut)
 template <typename T>

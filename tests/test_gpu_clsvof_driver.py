@@ -1,5 +1,4 @@
-"""The Fortran driver's method 3 (CLSVOF) on the GPU (not collected by pytest
-until it has passed on an MI355X: run it by naming the file).
+"""The Fortran driver's method 3 (CLSVOF) on the GPU.
 
 * The driver on a method-3 deck (Vortex 16^3, Vortex2D 32^2) equals the same
   calls issued from Python (1e-12 on the LS and VOF shape errors), writes
