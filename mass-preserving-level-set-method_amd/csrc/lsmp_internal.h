@@ -62,9 +62,6 @@ struct VelArgs {
     double xstart, ystart, zstart, dx, dy, dz, ct, omega;
 };
 
-// Narrow band: the RK stage of the last direction updates a cell only when its
-// x line and, in 3D, its y line cross the tube (fx, fy: the band line flags of
-// those directions); fx == nullptr updates every cell.
 // rows per lane at most in the partitioned line solves (8 up to 257 cells,
 // 16 up to 513)
 constexpr int PKX = 16;
@@ -72,6 +69,9 @@ constexpr int PKX = 16;
 // takes over from the paired, twisted and partitioned ones
 constexpr int TILE_MIN_LINES = 4096;
 
+// Narrow band: the RK stage of the last direction updates a cell only when its
+// x line and, in 3D, its y line cross the tube (fx, fy: the band line flags of
+// those directions); fx == nullptr updates every cell.
 struct TubeMask {
     const unsigned char *fx = nullptr, *fy = nullptr;
     int nx = 0, ny = 0;
@@ -179,7 +179,7 @@ template <typename T>
 void launch_face_velocity(hipStream_t st, T *uf, T *vf, T *wf, const FaceVelArgs &va, double xend, double yend,
                           double zend, double *tab);
 // one direction's WLIC-THINC pass, in -> out (n2 null in 2D; clamp on the
-// last direction; stencil: the variant that loads each cell's stencil first)
+// last direction)
 template <typename T>
 void launch_wlic_pass(hipStream_t st, int dir, bool clamp, const T *in, T *out, const T *old, const T *vel,
                       const T *n0, const T *n1, const T *n2, int nx, int ny, int nz, double dt, double dd);
