@@ -61,23 +61,28 @@ template <typename T> LSMP_DEV bool in_band_x(T x)
 }
 
 // Face m+1/2 of a line with second differences sa = s(m-1), sb = s(m),
-// sc = s(m+1), sd = s(m+2): plus = phyn(sd, sc, sb, sa) (the term of D+ at m)
-// and minus = phyn(sa, sb, sc, sd) (the term of D- at m+1), each as
-// phyn_fast evaluates it up to rounding.  IS0 of one is IS2 of the other,
+// sc = s(m+1), sd = s(m+2): plus = 12 phyn(sd, sc, sb, sa) (the term of D+
+// at m) and minus = 12 phyn(sa, sb, sc, sd) (the term of D- at m+1), each as
+// phyn_fast evaluates it up to rounding; the factor 12 joins the one of the
+// central part, so that both scale by 1 / (12 dx).  IS0 of one is IS2 of the other,
 // IS1 is common, and so are the products of their squares and the two
 // third differences the terms weigh.  The indicators are the expanded
 // quadratic forms 13 (a - b)^2 + 3 (a - 3b)^2 = 4 (4a^2 - 11ab + 10b^2) and
-// 13 (a - b)^2 + 3 (a + b)^2 = 4 (4a^2 - 5ab + 4b^2) over shared squares.
+// 13 (a - b)^2 + 3 (a + b)^2 = 4 (4a^2 - 5ab + 4b^2) over shared squares,
+// each formed as 4 IS + eps = 40b^2 + (a (16a - 44b) + eps) (and
+// 16a^2 + (b (16b - 20a) + eps)).  With the weights' denominators Dp and
+// Dm, plus = (4 pJB e1 + 6 pAJ e2) / Dp - e2 and minus the same with the
+// roles of the two products and third differences exchanged.
 // FP64 takes both reciprocals from one (the product of the two denominators
 // stays within [1e-48, 1e28]).  three: 3.0.
 template <typename T> LSMP_DEV void weno_hj_face(T sa, T sb, T sc, T sd, T three, T &plus, T &minus)
 {
     const T eps = T(1.0e-6);
     const T bb = sb * sb, cc = sc * sc;
-    const T isA = fma(sd, fma(T(4), sd, T(-11) * sc), T(10) * cc);   // IS0 of plus, IS2 of minus, over 4
-    const T isJ = fma(sc, fma(T(4), sc, T(-5) * sb), T(4) * bb);     // IS1 of both, over 4
-    const T isB = fma(sa, fma(T(4), sa, T(-11) * sb), T(10) * bb);   // IS2 of plus, IS0 of minus, over 4
-    const T qA = sq(fma(T(4), isA, eps)), qJ = sq(fma(T(4), isJ, eps)), qB = sq(fma(T(4), isB, eps));
+    const T fA = fma(T(40), cc, fma(sd, fma(T(16), sd, T(-44) * sc), eps));   // IS0 of plus, IS2 of minus
+    const T fJ = fma(T(16), bb, fma(sc, fma(T(16), sc, T(-20) * sb), eps));   // IS1 of both
+    const T fB = fma(T(40), bb, fma(sa, fma(T(16), sa, T(-44) * sb), eps));   // IS2 of plus, IS0 of minus
+    const T qA = sq(fA), qJ = sq(fJ), qB = sq(fB);
     const T pJB = qJ * qB, pAJ = qA * qJ, pAB = qA * qB;
     // plus: q0 = qA, q1 = qJ, q2 = qB; minus: q0 = qB, q1 = qJ, q2 = qA
     const T dp = fma(three, fma(T(2), pAB, pAJ), pJB), dm = fma(three, fma(T(2), pAB, pJB), pAJ);
@@ -91,12 +96,17 @@ template <typename T> LSMP_DEV void weno_hj_face(T sa, T sb, T sc, T sd, T three
         rm = rcp_w(dm);
     }
     const T e1 = sd - T(2) * sc + sb, e2 = sc - T(2) * sb + sa;
-    const T tp0 = pJB * (rp * T(1.0 / 3.0));
-    const T tp2 = fma(pAJ, rp * T(0.5), T(-1.0 / 12.0));
-    plus = fma(tp0, e1, tp2 * e2);
-    const T tm0 = pAJ * (rm * T(1.0 / 3.0));
-    const T tm2 = fma(pJB, rm * T(0.5), T(-1.0 / 12.0));
-    minus = fma(tm0, e2, tm2 * e1);
+    const T u1 = pJB * e1, u2 = pAJ * e2;
+    plus = fma(rp, fma(T(4), u1, T(6) * u2), -e2);
+    minus = fma(rm, fma(T(4), u2, T(6) * u1), -e1);
+}
+
+// The central part of the WENO-HJ derivatives times 12 dx,
+// -(m1 - m2) + 7 (c - m1) + 7 (p1 - c) - (p2 - p1), which is
+// (m2 - p2) + 8 (p1 - m1): c drops out.
+template <typename T> LSMP_DEV T central12(T m2, T m1, T p1, T p2)
+{
+    return fma(T(8), p1 - m1, m2 - p2);
 }
 
 // Godunov |grad phi| (level_set_rk3_redis.f90:341-360): with sigma the sign of
@@ -163,7 +173,6 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
         return;
     }
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy, inv12dz = (T)ra.inv12dz;
-    const T invdx = (T)ra.invdx, invdy = (T)ra.invdy, invdz = (T)ra.invdz;
     const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt, c3 = (T)ra.three;
     const int t = threadIdx.x;
     const int i0 = bx * R3X, j0 = by * R3Y, k0 = bz * KC;
@@ -236,30 +245,30 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
         const T zs_new = w[4] - T(2) * w[5] + w[6];   // s(p + 2)
         T zplus, zminus_next;
         weno_hj_face(za, zb, zc, zs_new, c3, zplus, zminus_next);
-        const T az = inv12dz * (-(w[2] - w[1]) + T(7) * (w[3] - w[2]) + T(7) * (w[4] - w[3]) - (w[5] - w[4]));
-        const T zs1 = az + invdz * zplus, zs2 = az - invdz * zminus;
+        const T az = inv12dz * central12(w[1], w[2], w[4], w[5]);
+        const T zs1 = az + inv12dz * zplus, zs2 = az - inv12dz * zminus;
         __syncthreads();
         // x face i+1/2: D+ term of this cell, D- term of cell i+1
         T xs1, ax, ys1, ay;
         {
             const T *r = &sphi[ty + 4][tx + 4];
             const T m2 = r[-2], m1 = r[-1], c = w[3], p1 = r[1], p2 = r[2], p3 = r[3];
-            ax = inv12dx * (-(m1 - m2) + T(7) * (c - m1) + T(7) * (p1 - c) - (p2 - p1));
+            ax = inv12dx * central12(m2, m1, p1, p2);
             T plus, minus;
             weno_hj_face(m2 - T(2) * m1 + c, m1 - T(2) * c + p1, c - T(2) * p1 + p2, p1 - T(2) * p2 + p3, c3,
                          plus, minus);
-            xs1 = ax + invdx * plus;
+            xs1 = ax + inv12dx * plus;
             mxs[ty][tx + 1] = minus;
         }
         // y face j+1/2
         {
             const T *r = &sphi[ty + 4][tx + 4];
             const T m2 = r[-2 * P3X], m1 = r[-P3X], c = w[3], p1 = r[P3X], p2 = r[2 * P3X], p3 = r[3 * P3X];
-            ay = inv12dy * (-(m1 - m2) + T(7) * (c - m1) + T(7) * (p1 - c) - (p2 - p1));
+            ay = inv12dy * central12(m2, m1, p1, p2);
             T plus, minus;
             weno_hj_face(m2 - T(2) * m1 + c, m1 - T(2) * c + p1, c - T(2) * p1 + p2, p1 - T(2) * p2 + p3, c3,
                          plus, minus);
-            ys1 = ay + invdy * plus;
+            ys1 = ay + inv12dy * plus;
             mys[ty + 1][tx] = minus;
         }
         if (first_x || first_y) {
@@ -273,8 +282,8 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
             else mys[0][fu - R3Y] = minus;
         }
         __syncthreads();
-        const T xs2 = ax - invdx * mxs[ty][tx];
-        const T ys2 = ay - invdy * mys[ty][tx];
+        const T xs2 = ax - inv12dx * mxs[ty][tx];
+        const T ys2 = ay - inv12dy * mys[ty][tx];
         const T grad = godunov_sig(xs1, xs2, ys1, ys2, zs1, zs2, s0_cur);
         // lambda = 0: final off the band, replaced by k_redis3d_band on it
         const T l = s0_cur * grad - s0_cur;
