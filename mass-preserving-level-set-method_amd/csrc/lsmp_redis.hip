@@ -371,7 +371,10 @@ template <typename T> LSMP_DEV T band_delta(T phi, T inv_w)
 // 3 loads the cell itself and finishes it.  Consecutive stages alternate
 // between two list lengths: this stage's (bcount) and the next one's (bnext),
 // which this pass clears -- the previous stage's pass has read it, and the
-// next stage's listing starts after this pass ends.
+// next stage's listing starts after this pass ends.  The stop flag, the list
+// length, the list entries and the box loads of the first pass are in flight
+// together: an entry at or past the length (stale, or never written) is
+// clamped to a cell of the domain and its result dropped.
 template <typename T, int STAGE>
 __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                      const T *__restrict__ phi0, const T *__restrict__ S0,
@@ -380,19 +383,25 @@ __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, 
                                                      const unsigned *bcount, unsigned *bnext)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *bnext = 0;
-    if (ctl[1]) return;   // stopped: the stage kernel listed nothing
+    const unsigned ncell = bcount[0];
+    // stage 3: the stage kernel's maximum change, complete now; a workgroup
+    // whose own maximum is not above it skips the atomic (the 1,024
+    // workgroups end together, and atomics on one address queue)
+    const unsigned long long stage_max = STAGE == 3 ? ctl[0] : 0ull;
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
     const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
-    const unsigned plane = (unsigned)nx * (unsigned)ny;
-    const unsigned ncell = bcount[0];
+    const unsigned plane = (unsigned)nx * (unsigned)ny, last = plane * (unsigned)nz - 1u;
     const int q = threadIdx.x & 3, g0 = (threadIdx.x & 63) & ~3;   // role, first lane of the four
     constexpr unsigned CPB = NT / 4;                                 // cells per workgroup and pass
     double lerr = 0.0;
     // the pass count is the same for every lane of the workgroup (shuffles)
-    for (unsigned e0 = blockIdx.x * CPB; e0 < ncell; e0 += gridDim.x * CPB) {
+    for (unsigned e0 = blockIdx.x * CPB;; e0 += gridDim.x * CPB) {
         const unsigned e = e0 + threadIdx.x / 4;
-        const bool live = e < ncell;
-        const unsigned c = blist[live ? e : e0];   // below 2^32: the list holds 32-bit indices
+        // the list has room for every cell of the domain
+        const unsigned c = min(blist[min(e, last)], last);
+        // the stop flag (the stage kernel listed nothing) as a vector load,
+        // behind the list entry, so that neither waits for the other
+        const bool stopped = *reinterpret_cast<const volatile unsigned long long *>(ctl + 1) != 0;
         const unsigned rem = c % plane;
         const int k = (int)(c / plane), cj = (int)(rem / (unsigned)nx), ci = (int)(rem % (unsigned)nx);
         T part2 = T(0), part3 = T(0);
@@ -414,6 +423,7 @@ __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, 
                     sv[jj * 3 + ii] = S0[at];
                     gv[jj * 3 + ii] = bgr[at];
                 }
+            if (stopped) return;
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
                 const bool inb = in_band_x(pv[r] * inv_w);
@@ -427,10 +437,11 @@ __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, 
             s0 = S0[c];
             grad = bgr[c];
             if (STAGE > 1) p0 = phi0[c];
+            if (stopped) return;
         }
         const T a0 = __shfl(part2, g0, 64), a1 = __shfl(part2, g0 + 1, 64), a2 = __shfl(part2, g0 + 2, 64);
         const T b0 = __shfl(part3, g0, 64), b1 = __shfl(part3, g0 + 1, 64), b2 = __shfl(part3, g0 + 2, 64);
-        if (q == 3 && live) {
+        if (q == 3 && e < ncell) {
             const T del = band_delta(phc, inv_w);
             const T c2 = del * s0 * (grad - T(1)), c3 = grad * del * del;
             const T la = T(51) * c2 + ((a0 + a1) + a2);
@@ -442,10 +453,12 @@ __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, 
             phi_out[c] = pn;
             if (STAGE == 3) lerr = fmax(lerr, (double)fabs(pn - p0));
         }
+        if (e0 + gridDim.x * CPB >= ncell) break;
     }
     if (STAGE == 3) {
         double m = block_max(lerr);
-        if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(ctl, m);
+        if (threadIdx.x == 0 && m > 0.0 && (unsigned long long)__double_as_longlong(m) > stage_max)
+            atomic_max_nonneg(ctl, m);
     }
 }
 
