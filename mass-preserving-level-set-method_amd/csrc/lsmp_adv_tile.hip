@@ -1,14 +1,17 @@
 // lsmp_adv_tile.hip -- the CRWENO5 line solve of an advection pass
 // (level_set_rk3_solver.f90:1-221: crweno_flux_split :347-414, the WENO-JS
 // boundary faces :288-345, the RK stage :16-88) for many lines of 34..513
-// cells, through LDS tiles.
+// cells, or lines of 514..1025 cells, through LDS tiles.
 //
 // A workgroup of TW waves takes TL lines with neighbouring indices (for y and
-// z lines, neighbouring in memory).  (1) Every thread loads cells of the TL
-// lines, coalesced -- along the line for x lines, across the lines for y and
-// z lines -- and stores their split fluxes g and f in two LDS tiles, one row
-// of slots per line, f in reversed cell order; then 32 threads form the
-// WENO-JS boundary faces of the TL lines' two systems.  Reversing a line
+// z lines, neighbouring in memory): 4 waves and 8 lines for lines of up to 513
+// cells, 8 waves and 4 lines (one system per wave, 16 rows per lane) for lines
+// of 514..1025 cells, which come in fewer lines per pass.
+// (1) Every thread loads cells of the TL lines, coalesced -- along the line
+// for x lines, across the lines for y and z lines -- and stores their split
+// fluxes g and f in two LDS tiles, one row of slots per line, f in reversed
+// cell order; then 4 TL threads form the WENO-JS boundary faces of the TL
+// lines' two systems.  Reversing a line
 // mirrors the stencil and exchanges A and C, which is what an f row does to a
 // g row, so the f system of a line is the g system of its reversed f fluxes,
 // with the reversed orientation's sign on the flux differences.  (2) Each
@@ -39,10 +42,6 @@ namespace lsmp {
 
 namespace {
 
-constexpr int TW = 4;   // waves per workgroup
-constexpr int TL = 8;   // lines per workgroup: 2 TL / TW systems per wave
-static_assert((2 * TL) % TW == 0 && (TL & (TL - 1)) == 0, "whole systems per wave, TL a power of two");
-
 // LDS slot of cell q (0-based) of a line: a pad slot after every 8 cells, so
 // that the lanes of a system, K cells apart, spread over the banks
 LSMP_DEV int tpos(int q) { return q + (q >> 3); }
@@ -57,12 +56,14 @@ static int adv_tile_row_slots(int n)
     return rs | 1;
 }
 
-template <typename T, int MODE, int STAGE, int KR, bool TUBE>
+// TW waves per workgroup, TL lines per workgroup: 2 TL / TW systems per wave
+template <typename T, int MODE, int STAGE, int KR, bool TUBE, int TL, int TW>
 __global__ __launch_bounds__(TW * 64) void k_adv_tile(T *phi, const T *__restrict__ vel, T *R, T *L1, T *L2,
                                                        int nlines, int n, int lw, long ostride, long istride,
                                                        long estride, T dd, T dt, const int *__restrict__ lines,
                                                        TubeMask tm, int rs)
 {
+    static_assert((2 * TL) % TW == 0 && (TL & (TL - 1)) == 0, "whole systems per wave, TL a power of two");
     extern __shared__ __align__(16) unsigned char adv_lds[];
     T *tg = reinterpret_cast<T *>(adv_lds);   // TL rows of rs slots: g fluxes, then g flux differences
     T *tf = tg + TL * rs;                     // the same for f, cells in reversed order
@@ -279,29 +280,33 @@ void launch_adv_tile(hipStream_t st, int mode, int stage, T *phi, const T *vel, 
                      const TubeMask &tm)
 {
     const int rs = adv_tile_row_slots(n);
-    const size_t lds = 2 * (size_t)TL * rs * sizeof(T);
-    const dim3 g((unsigned)((nlines + TL - 1) / TL)), b(TW * 64);
     const bool tube = tm.fx != nullptr;
     const bool k4 = n <= 64 * 4 + 1;   // four rows per lane suffice, else eight (up to 513 cells)
+    // lines of 514..1025 cells: sixteen rows per lane, four lines per workgroup
+    const bool k16 = n > 64 * 8 + 1;
+    const int tl = k16 ? 4 : 8;
+    const size_t lds = 2 * (size_t)tl * rs * sizeof(T);
+    const dim3 g((unsigned)((nlines + tl - 1) / tl)), b(k16 ? 8 * 64 : 4 * 64);
     // tiles of lines longer than 257 cells take more than 64 KiB of LDS (FP64)
-#define LSMP_TILEK(M, S, KR, TB)                                                                               \
+#define LSMP_TILEK(M, S, KR, TB, TLV, TWV)                                                                     \
     do {                                                                                                       \
         if (lds > 65536)                                                                                       \
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_adv_tile<T, M, S, KR, TB>),            \
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_adv_tile<T, M, S, KR, TB, TLV, TWV>),  \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                   \
-        hipLaunchKernelGGL((k_adv_tile<T, M, S, KR, TB>), g, b, lds, st, phi, vel, R, L1, L2, nlines, n, lw,  \
-                           ostride, istride, estride, dd, dt, lines, tm, rs);                                  \
+        hipLaunchKernelGGL((k_adv_tile<T, M, S, KR, TB, TLV, TWV>), g, b, lds, st, phi, vel, R, L1, L2, nlines, \
+                           n, lw, ostride, istride, estride, dd, dt, lines, tm, rs);                           \
     } while (0)
 #define LSMP_TILE(M, S)                                                                                        \
     do {                                                                                                       \
-        if constexpr (M == 2) {                                                                                \
-            if (k4 && tube) LSMP_TILEK(M, S, 4, true);                                                         \
-            else if (k4) LSMP_TILEK(M, S, 4, false);                                                           \
-            else if (tube) LSMP_TILEK(M, S, 8, true);                                                        \
-            else LSMP_TILEK(M, S, 8, false);                                                                 \
+        if (k16) LSMP_TILEK(M, S, 16, false, 4, 8);                                                            \
+        else if constexpr (M == 2) {                                                                           \
+            if (k4 && tube) LSMP_TILEK(M, S, 4, true, 8, 4);                                                   \
+            else if (k4) LSMP_TILEK(M, S, 4, false, 8, 4);                                                     \
+            else if (tube) LSMP_TILEK(M, S, 8, true, 8, 4);                                                    \
+            else LSMP_TILEK(M, S, 8, false, 8, 4);                                                             \
         } else {                                                                                               \
-            if (k4) LSMP_TILEK(M, S, 4, false);                                                                \
-            else LSMP_TILEK(M, S, 8, false);                                                                 \
+            if (k4) LSMP_TILEK(M, S, 4, false, 8, 4);                                                          \
+            else LSMP_TILEK(M, S, 8, false, 8, 4);                                                             \
         }                                                                                                      \
     } while (0)
     if (mode == 0) LSMP_TILE(0, 0);

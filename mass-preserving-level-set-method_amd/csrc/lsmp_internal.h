@@ -11,7 +11,7 @@ constexpr int CSEG = 16;   // rows per CCD checkpoint segment
 // every line of a direction goes through the partitioned (one wave or 32
 // lanes per line) solves up to these counts
 constexpr int PART_MAX_LINES_X = 32768, PART_MAX_LINES_STRIDED = 4096;
-constexpr int KCC = 17;   // CCD rows per lane at most (9 up to 284 cells, 17 up to 540)
+constexpr int KCC = 17;   // CCD rows per lane at most (9 up to 284 cells, 17 up to 540 with 32 lanes per line, 1084 with 64)
 
 // listed band lines solved with 32 lanes per line up to these counts (x lines
 // are contiguous; y and z lines are strided, and 32 lanes on one of them read

@@ -1071,19 +1071,20 @@ __global__ __launch_bounds__(256) void k_ccd_seg(const T *__restrict__ phi, cons
     }
 }
 
-// CCD line solve for few lines (the 2D grids): 32 lanes per line, two lines
-// per wave.  With the factorisation fixed per row, both sweeps are affine
+// CCD line solve for few lines (the 2D grids): LPL = 32 lanes per line, two
+// lines per wave, or for lines of 541..1084 cells LPL = 64, one line per wave.
+// With the factorisation fixed per row, both sweeps are affine
 // recurrences whose 2x2 matrices are the same for every line: forward
 // y_b = r_b - M_b y_{b-1}, M_b = [[A, B], [AA, BB]] of row b (the frow
 // update); backward x_b = Q_b (f_b - N_b x_{b+1}), N_b the row's third
 // block and Q_b its 2x2 solve (TWIN_BKS, matrix_solver.f90:47-78).  Lane s
 // owns the forward rows lo..hi (at most KCC of rows 1..nb-2).  Each lane
-// composes its rows into one affine map, an inclusive scan over the 32 lanes
-// (five shuffle rounds) gives every lane the state entering its rows, and
+// composes its rows into one affine map, an inclusive scan over the LPL lanes
+// (five or six shuffle rounds) gives every lane the state entering its rows, and
 // the lane re-runs its rows from that state with the reference's per-row
 // arithmetic; the backward sweep does the same from the closure row down.
 // Only the states entering each lane's rows carry the scan's rounding.
-template <typename T, int DIM, bool FINAL, bool TLDS, int KC>
+template <typename T, int DIM, bool FINAL, bool TLDS, int KC, int LPL>
 __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, const T *__restrict__ coef,
                                                    T *__restrict__ out, const T *__restrict__ dxv,
                                                    const T *__restrict__ dyv, T *__restrict__ gr,
@@ -1101,15 +1102,16 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
         tab = t;
     }
     const T *A = tab, *B = tab + 3 * nb, *AA = tab + 6 * nb, *BB = tab + 9 * nb, *IDEN = tab + 12 * nb;
-    const int lane = threadIdx.x & 63, s = lane & 31, half = lane & 32;
-    const int line = (int)((xcd_block() * blockDim.x + threadIdx.x) >> 6) * 2 + (lane >> 5);
+    static_assert(LPL == 32 || LPL == 64, "a half or a whole wave per line");
+    const int lane = threadIdx.x & 63, s = lane & (LPL - 1), half = lane & (64 - LPL);
+    const int line = (int)((xcd_block() * blockDim.x + threadIdx.x) >> 6) * (64 / LPL) + (LPL == 32 ? lane >> 5 : 0);
     const bool act = line < nlines;
     const int Lc = act ? line : nlines - 1;   // idle halves shadow the last line and emit nothing
     const int bl = lines ? lines[Lc] : Lc;     // with a list, half `line` solves listed line `line` of nlines
     const size_t base = (size_t)(bl / lw) * (size_t)ostride + (size_t)(bl % lw) * (size_t)istride;
     const T idx = T(1) / (T)ca.dx, idx2 = T(1) / ((T)ca.dx * (T)ca.dx);
     auto ldq = [&](int q) -> T { return phi[base + (size_t)clampi(q, 0, n - 1) * (size_t)estride]; };
-    const int NR = nb - 2, K = (NR + 31) / 32;
+    const int NR = nb - 2, K = (NR + LPL - 1) / LPL;
     const int lo = 1 + s * K, hi = min(lo + K - 1, NR);
     const int nrow = hi >= lo ? hi - lo + 1 : 0;
 
@@ -1146,7 +1148,7 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     }
     // inclusive scan: map of lanes 0..s composed
 #pragma unroll
-    for (int d = 1; d < 32; d <<= 1) {
+    for (int d = 1; d < LPL; d <<= 1) {
         const int src = s >= d ? lane - d : lane;
         const T a00 = __shfl(p00, src, 64), a01 = __shfl(p01, src, 64), a10 = __shfl(p10, src, 64);
         const T a11 = __shfl(p11, src, 64), w0 = __shfl(v0, src, 64), w1 = __shfl(v1, src, 64);
@@ -1160,7 +1162,7 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     }
     const T e0 = p00 * S0 + p01 * SS0 + v0, e1 = p10 * S0 + p11 * SS0 + v1;   // state after this lane's rows
     const T in0 = __shfl(e0, (lane - 1) & 63, 64), in1 = __shfl(e1, (lane - 1) & 63, 64);
-    const T last0 = __shfl(e0, half + 31, 64), last1 = __shfl(e1, half + 31, 64);   // after row nb-2
+    const T last0 = __shfl(e0, half + LPL - 1, 64), last1 = __shfl(e1, half + LPL - 1, 64);   // after row nb-2
     // this lane's forward states with the reference's per-row arithmetic
     T fs[KC], fss[KC];
     {
@@ -1213,13 +1215,13 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
             w0 = nw0; w1 = nw1;
         }
     }
-    // inclusive suffix scan: map of lanes s..31 composed (this ∘ later)
+    // inclusive suffix scan: map of lanes s..LPL-1 composed (this ∘ later)
 #pragma unroll
-    for (int d = 1; d < 32; d <<= 1) {
-        const int src = s + d < 32 ? lane + d : lane;
+    for (int d = 1; d < LPL; d <<= 1) {
+        const int src = s + d < LPL ? lane + d : lane;
         const T a00 = __shfl(g00, src, 64), a01 = __shfl(g01, src, 64), a10 = __shfl(g10, src, 64);
         const T a11 = __shfl(g11, src, 64), u0 = __shfl(w0, src, 64), u1 = __shfl(w1, src, 64);
-        if (s + d < 32) {
+        if (s + d < LPL) {
             const T nw0 = g00 * u0 + g01 * u1 + w0, nw1 = g10 * u0 + g11 * u1 + w1;
             const T q00 = g00 * a00 + g01 * a10, q01 = g00 * a01 + g01 * a11;
             const T q10 = g10 * a00 + g11 * a10, q11 = g10 * a01 + g11 * a11;
@@ -1231,7 +1233,7 @@ __global__ __launch_bounds__(256) void k_ccd_part(const T *__restrict__ phi, con
     const T nx0 = __shfl(lo0, (lane + 1) & 63, 64), nx1 = __shfl(lo1, (lane + 1) & 63, 64);
     // this lane's rows, top down, with the reference's arithmetic from x_{hi+1}
     double acc[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
-    T S = s == 31 ? XS : nx0, SS = s == 31 ? XSS : nx1;
+    T S = s == LPL - 1 ? XS : nx0, SS = s == LPL - 1 ? XSS : nx1;
 #pragma unroll
     for (int r = KC - 1; r >= 0; --r) {
         if (r < nrow) {
@@ -1768,8 +1770,12 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
     // as fast (FP64) or faster (FP32 182 against 220 us).  Listed lines (the
     // narrow band) take it up to PART_MAX_LISTED_ADV strided lines.
     int nlines = lines ? nlist : nall;
-    // many lines: the tiled solve reads and writes each field once, coalesced
-    if (n >= 34 && n <= 32 * PKX + 1 && nlines >= TILE_MIN_LINES) {
+    // many lines: the tiled solve reads and writes each field once, coalesced;
+    // lines of 514..1025 cells (2D 1024^2) take its long-line form whatever
+    // their count, since the paired and twisted kernels would give them one
+    // to four lanes each
+    if ((n >= 34 && n <= 32 * PKX + 1 && nlines >= TILE_MIN_LINES) ||
+        (n > 32 * PKX + 1 && n <= 64 * 16 + 1 && !tm.fx)) {
         launch_adv_tile<T>(st, mode, stage, phi, vel, R, L1, L2, nlines, n, lw, ostride, istride, estride, dd, dt,
                            lines, tm);
         return;
@@ -1920,22 +1926,26 @@ void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T 
     // few lines (the 2D grids, or the lines crossing the band): 32 lanes per
     // line instead of one
     const int part_max = !lines ? PART_MAX_LINES_STRIDED : dir == 0 ? PART_MAX_LISTED_X : PART_MAX_LISTED_STRIDED;
-    const bool part = nlines <= part_max && n + 4 <= 32 * KCC;
+    // lines of up to 540 cells: 32 lanes per line; up to 1084 cells: 64
+    const bool part32 = n + 4 <= 32 * KCC;
+    const bool part = nlines <= part_max && n + 4 <= 64 * KCC;
     const bool part9 = n + 4 <= 32 * 9;
-    const dim3 gp(nblk(nlines, 8));   // two lines per wave, four waves per block
-#define LSMP_CCDQ(D, F, KC)                                                                                    \
+    // two lines per wave (one with 64 lanes per line), four waves per block
+    const dim3 gp(nblk(nlines, part32 ? 8 : 4));
+#define LSMP_CCDQ(D, F, KC, LPL)                                                                               \
     do {                                                                                                       \
         if (lds)                                                                                               \
-            hipLaunchKernelGGL((k_ccd_part<T, D, F, true, KC>), gp, b, shm, st, phi, coef, out, dxv, dyv, gr,    \
-                               partial, nlines, n, lw, ostride, istride, estride, ca, lines);                  \
+            hipLaunchKernelGGL((k_ccd_part<T, D, F, true, KC, LPL>), gp, b, shm, st, phi, coef, out, dxv, dyv,   \
+                               gr, partial, nlines, n, lw, ostride, istride, estride, ca, lines);              \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_ccd_part<T, D, F, false, KC>), gp, b, 0, st, phi, coef, out, dxv, dyv, gr,     \
-                               partial, nlines, n, lw, ostride, istride, estride, ca, lines);                  \
+            hipLaunchKernelGGL((k_ccd_part<T, D, F, false, KC, LPL>), gp, b, 0, st, phi, coef, out, dxv, dyv,    \
+                               gr, partial, nlines, n, lw, ostride, istride, estride, ca, lines);              \
     } while (0)
 #define LSMP_CCDP(D, F)                                                                                        \
     do {                                                                                                       \
-        if (part9) LSMP_CCDQ(D, F, 9);                                                                         \
-        else LSMP_CCDQ(D, F, KCC);                                                                             \
+        if (part9) LSMP_CCDQ(D, F, 9, 32);                                                                     \
+        else if (part32) LSMP_CCDQ(D, F, KCC, 32);                                                             \
+        else LSMP_CCDQ(D, F, KCC, 64);                                                                         \
     } while (0)
     int nbl = 0;
     if (nlines > 0) {

@@ -83,6 +83,15 @@ CASES_MANY_LINES = [dict(nx=72, ny=66, nz=68, dim=3)]
 # of 40 cells whose last workgroup of 8 lines holds 4
 CASES_TILED = CASES_MANY_LINES + [dict(nx=300, ny=64, nz=64, dim=3), dict(nx=4100, ny=40, dim=2)]
 
+# lines of 514..1025 cells take the long-line form of the tiled solve (16 rows
+# per lane, 4 lines per workgroup) whatever their count, and lines of 541..1084
+# cells the CCD solve with 64 lanes per line: x lines of 1025 cells
+# (the longest), 3D x lines of 600 cells (the x lines of 640 cells in CASES as
+# well); advection alone on strided y lines of 520 cells (dt follows dx, so y
+# may not be much finer than x)
+CASES_LONG = [dict(nx=1025, ny=12, dim=2), dict(nx=600, ny=10, nz=8, dim=3)]
+CASES_LONG_ADVECT = [dict(nx=700, ny=520, dim=2)]
+
 
 @pytest.mark.parametrize("case", CASES, ids=ids)
 def test_upload_download_roundtrip(case):
@@ -94,7 +103,7 @@ def test_upload_download_roundtrip(case):
     s.close()
 
 
-@pytest.mark.parametrize("case", CASES + CASES_TILED, ids=ids)
+@pytest.mark.parametrize("case", CASES + CASES_TILED + CASES_LONG + CASES_LONG_ADVECT, ids=ids)
 def test_advect_parity(case):
     o, s = pair(**case)
     say("advect", case)
@@ -203,7 +212,7 @@ def test_reinit_btn0_parity(case):
     s.close()
 
 
-@pytest.mark.parametrize("case", CASES + CASES_MANY_LINES, ids=ids)
+@pytest.mark.parametrize("case", CASES + CASES_MANY_LINES + CASES_LONG, ids=ids)
 def test_mass_fix_parity(case):
     o, s = pair(**case)
     say("mass fix", case)
@@ -227,7 +236,7 @@ def test_mass_fix_parity(case):
     s.close()
 
 
-@pytest.mark.parametrize("case", CASES + CASES_MANY_LINES, ids=ids)
+@pytest.mark.parametrize("case", CASES + CASES_MANY_LINES + CASES_LONG, ids=ids)
 def test_mpls_steps_parity(case):
     """Full merged-MPLS steps (interface_solver.f90:123-129) with reinit every
     step, velocity at t^{n+1} (main.f90:23-29)."""
