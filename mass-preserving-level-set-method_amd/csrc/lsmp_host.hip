@@ -825,7 +825,7 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
     }
     if (alloc(&c->scr, sc * c->esz)) return fail(LSMP_E_HIP);
     if (alloc((void **)&c->vtab, (size_t)2 * (c->nx + c->ny + c->nz) * sizeof(double))) return fail(LSMP_E_HIP);
-    if (alloc((void **)&c->rctl, 4 * sizeof(unsigned long long))) return fail(LSMP_E_HIP);
+    if (alloc((void **)&c->rctl, CTL_WORDS * sizeof(unsigned long long))) return fail(LSMP_E_HIP);
     for (int d = 0; d < c->dim; ++d) {
         std::vector<double> tab;
         ccd_tables(ns[d] + 2 * GHC, params->dx, tab);   // all three solvers use dx (ccd_solvers.f90:406-408)

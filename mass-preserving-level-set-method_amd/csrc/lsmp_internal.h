@@ -44,6 +44,17 @@ constexpr int RKC = 32;
 // cell is skipped, and an active one's march is the kernel's critical path
 constexpr int RKC_NB = 8;
 
+// The pseudo-time loop's control words (k_redis_ctl): ctl[0..3], then the
+// running max |dphi| striped over CTL_SLOTS words by workgroup, so that the
+// workgroups of a stage, which end close together, do not queue their
+// atomics on one address.
+constexpr int CTL_SLOTS = 64;
+constexpr int CTL_WORDS = 4 + CTL_SLOTS;
+__device__ inline unsigned long long *ctl_slot(unsigned long long *ctl, unsigned b)
+{
+    return ctl + 4 + b % (unsigned)CTL_SLOTS;
+}
+
 struct RedisArgs {
     int nx, ny, nz;
     double inv12dx, inv12dy, inv12dz, invdx, invdy, invdz, ls_wid, inv_ls_wid, rdt;

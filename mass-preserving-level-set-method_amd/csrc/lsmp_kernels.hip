@@ -814,7 +814,7 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
     }
     if (STAGE == 3) {
         double m = block_max(lerr);
-        if (t == 0) atomic_max_nonneg(ctl, m);
+        if (t == 0 && m > 0.0) atomic_max_nonneg(ctl_slot(ctl, blockIdx.y * gridDim.x + blockIdx.x), m);
     }
 }
 
@@ -833,24 +833,38 @@ __global__ void k_sign_init(const T *__restrict__ phi, T *__restrict__ S0, size_
 }
 
 // Device-side control of the pseudo-time loop (level_set_rk3_redis.f90:17-23, 68):
-// ctl[0] max |dphi| of the running pseudo-step (bits of a non-negative double),
-// ctl[1] stop flag, ctl[2] pseudo-steps run, ctl[3] max |dphi| of the last one.
-// op 0 resets, op 2 closes a pseudo-step -- keeps its max |dphi|, clears the
-// running one for the next step and stops the loop once the step's max |dphi|
-// <= 1e-6 (a NaN never stops it, as in the reference); the time limit is the
-// host's count of launched steps.  Stage kernels return at once when the flag
-// is set, so steps can be queued ahead without host round trips and the step
-// count stays the reference's.
+// ctl[1] stop flag, ctl[2] pseudo-steps run, ctl[3] max |dphi| of the last
+// one; the running pseudo-step's max |dphi| (bits of a non-negative double)
+// is striped over ctl[4 .. 4 + CTL_SLOTS).  One wave: op 0 resets, op 2 closes
+// a pseudo-step -- reduces the stripes to its max |dphi|, clears them for the
+// next step and stops the loop once the step's max |dphi| <= 1e-6 (a NaN
+// never stops it, as in the reference); the time limit is the host's count of
+// launched steps.  Stage kernels return at once when the flag is set, so
+// steps can be queued ahead without host round trips and the step count stays
+// the reference's.
 __global__ void k_redis_ctl(unsigned long long *ctl, int op)
 {
-    if (threadIdx.x != 0) return;
+    const int t = threadIdx.x;   // stripe t
     if (op == 0) {
-        ctl[0] = 0; ctl[1] = 0; ctl[2] = 0; ctl[3] = 0;
-    } else if (!ctl[1]) {
+        ctl[4 + t] = 0;
+        if (t < 4) ctl[t] = 0;
+        return;
+    }
+    if (ctl[1]) return;
+    const double v = __longlong_as_double((long long)ctl[4 + t]);
+    ctl[4 + t] = 0;
+    // non-negative doubles (and a NaN, whose bits order above them) keep their
+    // order as integers: the max of the bit patterns
+    unsigned long long m = (unsigned long long)__double_as_longlong(v);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = __shfl_xor(m, d, 64);
+        m = o > m ? o : m;
+    }
+    if (t == 0) {
         ctl[2] += 1;
-        ctl[3] = ctl[0];
-        ctl[0] = 0;
-        const double e = __longlong_as_double((long long)ctl[3]);
+        ctl[3] = m;
+        const double e = __longlong_as_double((long long)m);
         if (e <= 1.0e-6) ctl[1] = 1;
     }
 }
@@ -2126,7 +2140,8 @@ void launch_convert(hipStream_t st, TO *out, const TI *in, size_t n)
 
 void launch_redis_ctl(hipStream_t st, unsigned long long *ctl, int op)
 {
-    hipLaunchKernelGGL(k_redis_ctl, dim3(1), dim3(64), 0, st, ctl, op);
+    static_assert(CTL_SLOTS == 64, "one wave, a stripe per lane");
+    hipLaunchKernelGGL(k_redis_ctl, dim3(1), dim3(CTL_SLOTS), 0, st, ctl, op);
 }
 
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n)

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     }
     if (STAGE == 3) {
         double m = block_max(lerr);
-        if (t == 0 && m > 0.0) atomic_max_nonneg(ctl, m);
+        if (t == 0 && m > 0.0) atomic_max_nonneg(ctl_slot(ctl, (unsigned)tile_id), m);
     }
 }
 
@@ -384,10 +384,11 @@ __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, 
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *bnext = 0;
     const unsigned ncell = bcount[0];
-    // stage 3: the stage kernel's maximum change, complete now; a workgroup
-    // whose own maximum is not above it skips the atomic (the 1,024
-    // workgroups end together, and atomics on one address queue)
-    const unsigned long long stage_max = STAGE == 3 ? ctl[0] : 0ull;
+    // stage 3: the maximum change in this workgroup's stripe, the stage
+    // kernel's part of it complete now; a workgroup whose own maximum is not
+    // above it skips the atomic (the 1,024 workgroups end together)
+    unsigned long long *slot = ctl_slot(ctl, blockIdx.x);
+    const unsigned long long stage_max = STAGE == 3 ? *slot : 0ull;
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
     const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
     const unsigned plane = (unsigned)nx * (unsigned)ny, last = plane * (unsigned)nz - 1u;
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(NT) void k_redis3d_band(const T *__restrict__ phi, 
     if (STAGE == 3) {
         double m = block_max(lerr);
         if (threadIdx.x == 0 && m > 0.0 && (unsigned long long)__double_as_longlong(m) > stage_max)
-            atomic_max_nonneg(ctl, m);
+            atomic_max_nonneg(slot, m);
     }
 }
 
