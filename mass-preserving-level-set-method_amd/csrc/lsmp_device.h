@@ -255,6 +255,55 @@ template <typename T> LSMP_DEV T phyn_fast(T a, T b, T c, T d, T sbc, T scd)
     return fma(t0, a - T(2) * b + c, t2 * (b - T(2) * c + d));
 }
 
+// Face m+1/2 of a line with second differences sa = s(m-1), sb = s(m),
+// sc = s(m+1), sd = s(m+2): plus = 12 phyn(sd, sc, sb, sa) (the term of D+
+// at m) and minus = 12 phyn(sa, sb, sc, sd) (the term of D- at m+1), each as
+// phyn_fast evaluates it up to rounding; the factor 12 joins the one of the
+// central part, so that both scale by 1 / (12 dx).  IS0 of one is IS2 of the other,
+// IS1 is common, and so are the products of their squares and the two
+// third differences the terms weigh.  The indicators are the expanded
+// quadratic forms 13 (a - b)^2 + 3 (a - 3b)^2 = 4 (4a^2 - 11ab + 10b^2) and
+// 13 (a - b)^2 + 3 (a + b)^2 = 4 (4a^2 - 5ab + 4b^2) over shared squares,
+// each formed as 4 IS + eps = 40b^2 + (a (16a - 44b) + eps) (and
+// 16a^2 + (b (16b - 20a) + eps)).  With the weights' denominators Dp and
+// Dm, plus = (4 pJB e1 + 6 pAJ e2) / Dp - e2 and minus the same with the
+// roles of the two products and third differences exchanged.
+// FP64 takes both reciprocals from one (the product of the two denominators
+// stays within [1e-48, 1e28]).  three: 3.0.
+template <typename T> LSMP_DEV void weno_hj_face(T sa, T sb, T sc, T sd, T three, T &plus, T &minus)
+{
+    const T eps = T(1.0e-6);
+    const T bb = sb * sb, cc = sc * sc;
+    const T fA = fma(T(40), cc, fma(sd, fma(T(16), sd, T(-44) * sc), eps));   // IS0 of plus, IS2 of minus
+    const T fJ = fma(T(16), bb, fma(sc, fma(T(16), sc, T(-20) * sb), eps));   // IS1 of both
+    const T fB = fma(T(40), bb, fma(sa, fma(T(16), sa, T(-44) * sb), eps));   // IS2 of plus, IS0 of minus
+    const T qA = sq(fA), qJ = sq(fJ), qB = sq(fB);
+    const T pJB = qJ * qB, pAJ = qA * qJ, pAB = qA * qB;
+    // plus: q0 = qA, q1 = qJ, q2 = qB; minus: q0 = qB, q1 = qJ, q2 = qA
+    const T dp = fma(three, fma(T(2), pAB, pAJ), pJB), dm = fma(three, fma(T(2), pAB, pJB), pAJ);
+    T rp, rm;
+    if constexpr (sizeof(T) == 8) {
+        const T r = rcp_w(dp * dm);
+        rp = dm * r;
+        rm = dp * r;
+    } else {
+        rp = rcp_w(dp);
+        rm = rcp_w(dm);
+    }
+    const T e1 = sd - T(2) * sc + sb, e2 = sc - T(2) * sb + sa;
+    const T u1 = pJB * e1, u2 = pAJ * e2;
+    plus = fma(rp, fma(T(4), u1, T(6) * u2), -e2);
+    minus = fma(rm, fma(T(4), u2, T(6) * u1), -e1);
+}
+
+// The central part of the WENO-HJ derivatives times 12 dx,
+// -(m1 - m2) + 7 (c - m1) + 7 (p1 - c) - (p2 - p1), which is
+// (m2 - p2) + 8 (p1 - m1): c drops out.
+template <typename T> LSMP_DEV T central12(T m2, T m1, T p1, T p2)
+{
+    return fma(T(8), p1 - m1, m2 - p2);
+}
+
 // D+ and D- from the five second differences s_m = phi(m+1) - 2 phi(m) + phi(m-1),
 // m = i-2..i+2, shared by both one-sided terms.
 template <typename T>

@@ -731,19 +731,30 @@ template <typename T> LSMP_DEV bool in_band(T x)
 
 // 2D stage: one tile, no marching (Disk/ and Vortex2D/ forms: |grad| without
 // z, lambda box 3x3 with centre weight 17, Disk/...redis.f90:357-361, 434-438).
+// The tile's RTX x RTY cells and the ring around them (the lambda box) need
+// D+ and D- in x and y.  Each WENO-HJ face m+1/2 gives the D+ term of cell m
+// and the D- term of cell m+1 (weno_hj_face), so the workgroup forms every
+// face once -- x faces i+1/2 for i = i0-2 .. i0+RTX on the REY ring rows, y
+// faces j+1/2 for j = j0-2 .. j0+RTY on the REX ring columns, two faces a
+// thread -- into LDS, and each cell of the tile and ring reads its four terms
+// there.
+constexpr int R2FX = (RTX + 3) * REY;   // x faces
+constexpr int R2F = R2FX + REX * (RTY + 3);   // and y faces
+static_assert(R2F <= 2 * RNT, "two faces a thread");
+
 template <typename T, int STAGE, int BTN>
 __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi, T *__restrict__ phi_out,
                                                        const T *__restrict__ phi0, const T *__restrict__ S0,
                                                        RedisArgs ra, unsigned long long *ctl)
 {
     __shared__ T sphi[RPY][RPX];
+    __shared__ T fplus[R2F], fminus[R2F];   // the faces' D+ and D- terms, times 12
     __shared__ T ss2[REY][REX];
     __shared__ T ss3[REY][REX];
     if (ctl[1]) return;   // the pseudo-time loop has stopped (k_redis_ctl)
     const int nx = ra.nx, ny = ra.ny;
     const T inv12dx = (T)ra.inv12dx, inv12dy = (T)ra.inv12dy;
-    const T invdx = (T)ra.invdx, invdy = (T)ra.invdy;
-    const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt;
+    const T inv_w = (T)ra.inv_ls_wid, rdt = (T)ra.rdt, c3 = (T)ra.three;
     const int t = threadIdx.x;
     const int i0 = blockIdx.x * RTX, j0 = blockIdx.y * RTY;
     const bool ext = t < REX * REY;
@@ -757,8 +768,31 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
         int gx = clampi(i0 - 4 + c, 0, nx - 1), gy = clampi(j0 - 4 + r, 0, ny - 1);
         sphi[r][c] = phi[(size_t)gy * nx + gx];
     }
+    const T s0_p = ext ? S0[(size_t)gj * nx + gi] : T(0);
     __syncthreads();
-    T grad_p = T(0), del_p = T(0), s0_p = T(0), phi_p = T(0);
+    // the faces: x face f (< R2FX) at ring row f / (RTX + 3), cell i0 - 2 + f % (RTX + 3);
+    // y face g = f - R2FX at cell j0 - 2 + g / REX, ring column g % REX.  Past
+    // the domain edge the LDS plane holds the boundary cell, as the ghosts do.
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int f = t + h * RNT;
+        if (f < R2F) {
+            const bool fx = f < R2FX;
+            const int g = fx ? f : f - R2FX;
+            const int r = fx ? g / (RTX + 3) + 3 : g / REX + 2;   // LDS row of the face's cell m
+            const int c = fx ? g % (RTX + 3) + 2 : g % REX + 3;   // and column
+            const int st = fx ? 1 : RPX;
+            const T *v = &sphi[r][c];
+            const T vm2 = v[-2 * st], vm1 = v[-st], v0 = v[0], vp1 = v[st], vp2 = v[2 * st], vp3 = v[3 * st];
+            T plus, minus;
+            weno_hj_face(vm2 - T(2) * vm1 + v0, vm1 - T(2) * v0 + vp1, v0 - T(2) * vp1 + vp2,
+                         vp1 - T(2) * vp2 + vp3, c3, plus, minus);
+            fplus[f] = plus;
+            fminus[f] = minus;
+        }
+    }
+    __syncthreads();
+    T grad_p = T(0), del_p = T(0), phi_p = T(0);
     // a ring cell's gradient only enters lambda through s2, s3 = 0 where delta = 0
     if (ext && !inner && !(BTN == 1 && in_band(sphi[pr][pc] * inv_w))) {
         if (BTN == 1) {
@@ -767,12 +801,15 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
         }
     } else if (ext) {
         const T c0 = sphi[pr][pc];
-        T xs1, xs2, ys1, ys2;
-        weno_hj_pm_fast(sphi[pr][pc - 3], sphi[pr][pc - 2], sphi[pr][pc - 1], c0, sphi[pr][pc + 1],
-                        sphi[pr][pc + 2], sphi[pr][pc + 3], inv12dx, invdx, xs1, xs2);
-        weno_hj_pm_fast(sphi[pr - 3][pc], sphi[pr - 2][pc], sphi[pr - 1][pc], c0, sphi[pr + 1][pc],
-                        sphi[pr + 2][pc], sphi[pr + 3][pc], inv12dy, invdy, ys1, ys2);
-        s0_p = S0[(size_t)gj * nx + gi];
+        // a cell past the domain edge takes the boundary cell's values (gi, gj):
+        // at ring position (cx, cy) its x faces i -/+ 1/2 are x faces
+        // cy (RTX + 3) + cx (+ 1), its y faces j -/+ 1/2 y faces cy REX + cx (+ REX)
+        const int cx = gi - i0 + 1, cy = gj - j0 + 1;
+        const int fxm = cy * (RTX + 3) + cx, fym = R2FX + cy * REX + cx;
+        const T ax = inv12dx * central12(sphi[pr][pc - 2], sphi[pr][pc - 1], sphi[pr][pc + 1], sphi[pr][pc + 2]);
+        const T ay = inv12dy * central12(sphi[pr - 2][pc], sphi[pr - 1][pc], sphi[pr + 1][pc], sphi[pr + 2][pc]);
+        const T xs1 = ax + inv12dx * fplus[fxm + 1], xs2 = ax - inv12dx * fminus[fxm];
+        const T ys1 = ay + inv12dy * fplus[fym + REX], ys2 = ay - inv12dy * fminus[fym];
         grad_p = godunov<T, 2>(xs1, xs2, ys1, ys2, T(0), T(0), s0_p);
         phi_p = c0;
         if (BTN == 1) {
@@ -782,7 +819,7 @@ __global__ __launch_bounds__(RNT) void k_redis_stage2d(const T *__restrict__ phi
             ss3[ey][ex] = grad_p * del_p * del_p;
         }
     }
-    __syncthreads();
+    if (BTN == 1) __syncthreads();
     double lerr = 0.0;
     if (inner) {
         T lam = T(0);
