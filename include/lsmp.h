@@ -164,6 +164,19 @@ int lsmp_set_narrow_band(lsmp_handle h, double width_cells);
  * (output.f90:169-204, 226, 233-248). */
 int lsmp_interface_l1(lsmp_handle h, double *l1);
 
+/* The per-step energy and momentum integrals of energy_and_momentum
+ * (output.f90:250-397, called every step by output(), :33) with rho_mu
+ * (ns_solver/others.f90:47-80): the nine first CCD derivatives of the
+ * velocity, then out[11] = A (the surface-area estimate, already / dx), v
+ * (sum marker dV), px, py, pz (sum marker u dV / v ...), xc, yc, zc (the
+ * centroid), Ke, Po, Q (kinetic and potential energy and the viscous
+ * dissipation rate over the cells with marker > 0.99) -- before the deck's
+ * Fr, We, Re and energy_unit scaling (:374-386), which the caller applies.
+ * marker = H(phi), or the VOF with vof_marker != 0 (method 3); mu_12 is the
+ * deck's viscosity ratio.  Needs the velocity (and the VOF); the first call
+ * allocates 9 fields for the derivatives. */
+int lsmp_energy_momentum(lsmp_handle h, double mu_12, int vof_marker, double *out);
+
 /* Device-time accounting per phase (the reference's ls_adv / ls_red timers,
  * level_set_rk3_solver.f90:91-92, level_set_rk3_redis.f90:76-77). */
 int lsmp_get_stats(lsmp_handle h, lsmp_stats *out);

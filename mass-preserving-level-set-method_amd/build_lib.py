@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 SOURCES = ["lsmp_kernels.hip", "lsmp_host.hip", "lsmp_learn.hip", "lsmp_metrics.hip", "lsmp_clsvof.hip",
-           "lsmp_noalloc.hip", "lsmp_redis.hip", "lsmp_adv_tile.hip"]
+           "lsmp_noalloc.hip", "lsmp_redis.hip", "lsmp_adv_tile.hip", "lsmp_energy.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -63,6 +63,10 @@ struct lsmp_ctx {
     void *uf = nullptr, *vf = nullptr, *wf = nullptr;
     void *nrm[3] = {nullptr, nullptr, nullptr};
     double *ftab = nullptr;
+    // energy_and_momentum, allocated on first use: the nine velocity derivatives
+    // and the block sums with the eleven results after them
+    void *em = nullptr;
+    double *em_part = nullptr;
     bool vof_set = false, vof_old_set = false, vof0_set = false, face_set = false;
     bool vel_set = false, phi0_set = false;
     lsmp_stats stats{};

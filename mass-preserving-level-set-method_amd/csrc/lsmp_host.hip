@@ -618,6 +618,43 @@ template <typename T> int interface_l1(lsmp_ctx *c, double *l1)
     return LSMP_OK;
 }
 
+// energy_and_momentum (output.f90:250-397): the nine first CCD derivatives of
+// the velocity along every line, then the eleven integrals (lsmp_energy.hip)
+template <typename T> int energy_momentum(lsmp_ctx *c, double mu_12, bool vof_marker, double *out)
+{
+    if (!c->vel_set || (vof_marker && !c->vof_set)) return LSMP_E_STATE;
+    constexpr int NBMAX = 1024;
+    if (!c->em) {
+        if (alloc(&c->em, (size_t)9 * c->ncell * c->esz) ||
+            alloc((void **)&c->em_part, ((size_t)ENERGY_SUMS * NBMAX + ENERGY_SUMS) * sizeof(double)))
+            return LSMP_E_HIP;
+    }
+    const CcdArgs ca = ccd_args(c);
+    const T *vel[3] = {(const T *)c->u, (const T *)c->v, (const T *)c->w};
+    T *ten = (T *)c->em;
+    static const char *const nm[3] = {"em_ccd_x", "em_ccd_y", "em_ccd_z"};
+    for (int f = 0; f < c->dim; ++f)
+        for (int d = 0; d < c->dim; ++d)
+            KT(c, nm[d], (launch_ccd_pass<T>(c->st, d, c->dim, false, vel[f], (const T *)c->coef[d], (T *)c->scr,
+                                             ten + (size_t)(3 * f + d) * c->ncell, nullptr, nullptr, nullptr, nullptr,
+                                             c->nx, c->ny, c->nz, ca, nullptr)));
+    EnergyArgs ea;
+    ea.nx = c->nx; ea.ny = c->ny; ea.nz = c->nz; ea.inverse = c->p.inverse;
+    ea.dx = c->p.dx; ea.dv = c->p.dx * c->p.dy * c->p.dz; ea.ls_wid = c->p.ls_wid;
+    ea.rho_12 = c->p.rho_12; ea.mu_12 = mu_12;
+    ea.xstart = c->p.xstart; ea.ystart = c->p.ystart; ea.zstart = c->p.zstart;
+    ea.dxc = c->p.dx; ea.dyc = c->p.dy; ea.dzc = c->p.dz;
+    const int nb = (int)std::min<size_t>(NBMAX, (c->ncell + 255) / 256);
+    KT(c, "energy", (launch_energy_sums<T>(c->st, c->dim, vof_marker, (const T *)c->phi, (const T *)c->vof, vel[0],
+                                           vel[1], vel[2], ten, ea, c->em_part, nb)));
+    double *res = c->em_part + (size_t)ENERGY_SUMS * NBMAX;
+    KT(c, "energy_final", (launch_energy_finalize(c->st, c->em_part, nb, c->p.dx, res)));
+    LSMP_HIP(hipMemcpyAsync(c->hbuf, res, ENERGY_SUMS * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    LSMP_HIP(hipStreamSynchronize(c->st));
+    for (int q = 0; q < ENERGY_SUMS; ++q) out[q] = c->hbuf[q];
+    return LSMP_OK;
+}
+
 template <typename T> int velocity_analytic(lsmp_ctx *c, int field, double t, double t2s)
 {
     const double pi = std::acos(-1.0);
@@ -746,7 +783,7 @@ void free_all(lsmp_ctx *c)
                     c->dxv, c->dyv, c->gr, c->scr, c->coef[0], c->coef[1], c->coef[2], c->partial, c->scal, c->u64, c->band_list, c->band_count,
                     c->mk[0], c->mk[1], c->mflag, c->lflag, c->lcount, c->llist[0], c->llist[1], c->llist[2], c->tiles,
                     c->rb_gr, c->rb_list, c->rb_count,
-                    c->vof, c->vofB, c->vof_old, c->vkeep, c->vof0, c->uf, c->vf, c->wf, c->nrm[0], c->nrm[1], c->nrm[2], c->ftab};
+                    c->vof, c->vofB, c->vof_old, c->vkeep, c->vof0, c->uf, c->vf, c->wf, c->nrm[0], c->nrm[1], c->nrm[2], c->ftab, c->em, c->em_part};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
@@ -847,7 +884,7 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
     if (alloc((void **)&c->partial, (size_t)3 * c->npartial * sizeof(double))) return fail(LSMP_E_HIP);
     if (alloc((void **)&c->scal, 8 * sizeof(double))) return fail(LSMP_E_HIP);
     if (alloc((void **)&c->u64, 4 * sizeof(unsigned long long))) return fail(LSMP_E_HIP);
-    if (hipHostMalloc((void **)&c->hbuf, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&c->hbuf, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return fail(LSMP_E_HIP);
     // band lines of the mass correction: flags of every x, y, z line, lists of the x and y lines
     const size_t nfx = (size_t)c->ny * c->nz, nfy = (size_t)c->nx * c->nz, nfz = (size_t)c->nx * c->ny;
@@ -1022,6 +1059,13 @@ int lsmp_mass_fix(lsmp_handle h, double imass, double *mass, int *n_iter)
     h->stats.n_massfix++;
     resolve(h, false);
     return rc;
+}
+
+int lsmp_energy_momentum(lsmp_handle h, double mu_12, int vof_marker, double *out)
+{
+    if (!h || !out) return LSMP_E_INVALID;
+    return h->fp32 ? energy_momentum<float>(h, mu_12, vof_marker != 0, out)
+                   : energy_momentum<double>(h, mu_12, vof_marker != 0, out);
 }
 
 int lsmp_interface_l1(lsmp_handle h, double *l1)

@@ -207,6 +207,19 @@ template <typename T>
 void launch_vof_sums(hipStream_t st, const T *vof, const T *vof0, size_t n, double dv, double r12, int inverse,
                      int l1, double *partial, int nblocks);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
+
+// energy_and_momentum (lsmp_energy.hip): eleven block sums per workgroup
+// over the interior cells and their fixed-order reduction into out[11]
+constexpr int ENERGY_SUMS = 11;
+struct EnergyArgs {
+    int nx, ny, nz, inverse;
+    double dx, dv, ls_wid, rho_12, mu_12;
+    double xstart, ystart, zstart, dxc, dyc, dzc;   // cell-centre coordinates
+};
+template <typename T>
+void launch_energy_sums(hipStream_t st, int dim, bool vof_marker, const T *phi, const T *vof, const T *u, const T *v,
+                        const T *w, const T *ten, const EnergyArgs &ea, double *partial, int nblocks);
+void launch_energy_finalize(hipStream_t st, const double *partial, int nblk, double dx, double *out);
 void launch_redis_ctl(hipStream_t st, unsigned long long *ctl, int op);
 
 }  // namespace lsmp

@@ -313,6 +313,14 @@ interface
         type(c_ptr), value :: h
         real(c_double), intent(out) :: l1
     end function
+    ! energy_and_momentum's integrals (output.f90:250-397), out(11)
+    integer(c_int) function c_lsmp_energy_momentum(h, mu_12, vof_marker, out) bind(C, name="lsmp_energy_momentum")
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: h
+        real(c_double), value :: mu_12
+        integer(c_int), value :: vof_marker
+        real(c_double), intent(out) :: out(*)
+    end function
     integer(c_int) function c_lsmp_set_face_velocity(h, u, v, w) bind(C, name="lsmp_set_face_velocity")
         import :: c_int, c_ptr, c_double
         type(c_ptr), value :: h
@@ -353,7 +361,7 @@ public :: c_lsmp_net_clear_samples, c_lsmp_net_apply, c_lsmp_net_train, c_lsmp_n
 public :: c_lsmp_reinit_learned, c_lsmp_reinit_residual
 public :: c_lsmp_upload_vof, c_lsmp_download_vof, c_lsmp_store_vof_old, c_lsmp_set_face_velocity
 public :: c_lsmp_set_face_velocity_analytic, c_lsmp_vof_wlic, c_lsmp_clsvof_recon
-public :: c_lsmp_store_vof0, c_lsmp_vof_mass_vol, c_lsmp_vof_interface_l1
+public :: c_lsmp_store_vof0, c_lsmp_vof_mass_vol, c_lsmp_vof_interface_l1, c_lsmp_energy_momentum
 public :: lsmp_check, lsmp_error_message
 
 contains

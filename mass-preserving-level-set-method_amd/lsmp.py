@@ -40,9 +40,11 @@ EXPORTS = [
     "lsmp_net_clear_samples", "lsmp_net_apply",
     "lsmp_upload_vof", "lsmp_download_vof", "lsmp_store_vof_old", "lsmp_set_face_velocity",
     "lsmp_set_face_velocity_analytic", "lsmp_vof_wlic", "lsmp_clsvof_recon", "lsmp_store_vof0",
-    "lsmp_vof_mass_vol", "lsmp_vof_interface_l1", "lsmp_mass_vol_noalloc",
+    "lsmp_vof_mass_vol", "lsmp_vof_interface_l1", "lsmp_mass_vol_noalloc", "lsmp_energy_momentum",
 ]
 NO_STATUS = ("lsmp_finalize", "lsmp_error_string", "lsmp_net_destroy")
+# order of lsmp_energy_momentum's eleven results
+ENERGY_KEYS = ("A", "v", "px", "py", "pz", "xc", "yc", "zc", "Ke", "Po", "Q")
 
 
 class Params(C.Structure):
@@ -98,6 +100,7 @@ def load_library(path=LIB_PATH):
     L.lsmp_mass_vol.argtypes = [H, _DP, _DP]
     L.lsmp_mass_fix.argtypes = [H, C.c_double, _DP, C.POINTER(C.c_int)]
     L.lsmp_interface_l1.argtypes = [H, _DP]
+    L.lsmp_energy_momentum.argtypes = [H, C.c_double, C.c_int, _DP]
     L.lsmp_get_stats.argtypes = [H, C.POINTER(Stats)]
     L.lsmp_set_profiling.argtypes = [H, C.c_int]
     L.lsmp_kernel_time.argtypes = [H, C.c_char_p, C.POINTER(C.c_longlong), _DP]
@@ -291,6 +294,14 @@ class Solver:
         v = C.c_double()
         _check(self.lib.lsmp_interface_l1(self.h, C.byref(v)))
         return v.value
+
+    def energy_momentum(self, mu_12=1.0, vof_marker=False):
+        """energy_and_momentum's integrals (output.f90:250-397) before the deck's
+        Fr, We, Re and energy_unit scaling: dict of A (surface area), v, px, py,
+        pz, xc, yc, zc, Ke, Po, Q.  marker = H(phi), or the VOF (method 3)."""
+        out = (C.c_double * len(ENERGY_KEYS))()
+        _check(self.lib.lsmp_energy_momentum(self.h, float(mu_12), int(bool(vof_marker)), out))
+        return dict(zip(ENERGY_KEYS, list(out)))
 
     # -- accounting
     def stats(self):

@@ -1134,6 +1134,109 @@ double orc_interface_l1(orc_state *s)
 }
 
 /* ------------------------------------------------------------------------
+ * energy_and_momentum (Vortex/src/others/output.f90:250-397; the Disk/ and
+ * Vortex2D/ copies are the same text): rho_mu (ns_solver/others.f90:47-80)
+ * on the marker, the nine first CCD derivatives of the velocity along the
+ * interior lines (:262-300; all three solvers use dx, ccd_solvers.f90:406-408),
+ * then one pass over the interior cells (:313-360).  out[0] A (already / dx,
+ * :364), out[1] v, out[2..4] px / v, py / v, pz / v, out[5..7] xc / v, yc / v,
+ * zc / v, out[8] Ke, out[9] Po, out[10] Q -- the integrals before the deck's
+ * Fr, We, Re and energy_unit scaling (:374-386).  The reference adds to px,
+ * py, pz without clearing them (an uninitialised local); they start at 0
+ * here.  marker = heavy (ls_funs of phi) or, with vof_marker, the VOF
+ * (method 3).  Cell coordinates are coord()'s.
+ * ---------------------------------------------------------------------- */
+static int orc_cmp_int(int a, int b) { return a > b ? a : b; }
+static double coord(double start, double end, double d, int n, int i);
+
+void orc_energy_momentum(orc_state *s, double mu_12, int vof_marker, double *out)
+{
+    const int nx = s->p.nx, ny = s->p.ny, ks = s->ks, ke = s->ke;
+    const double dx = s->p.dx;
+    const double dv = s->p.dx * s->p.dy * s->p.dz;
+    double *P = s->f[ORC_PHI], *H = s->f[ORC_HEAVY], *VF = s->f[ORC_VOF];
+    double *vel[3] = {s->f[ORC_U], s->f[ORC_V], s->f[ORC_W]};
+    orc_ls_funs(s);   /* rho_mu calls ls_funs first (others.f90:53) */
+    /* ten[3 f + d]: d/dx_d of velocity component f on the box */
+    double *ten[9];
+    for (int q = 0; q < 9; ++q) ten[q] = (double *)calloc((size_t)s->nbox, sizeof(double));
+    ccd_mat m[3];
+    ccd_build(&m[0], s->nxb, dx);
+    ccd_build(&m[1], s->nyb, dx);
+    ccd_build(&m[2], s->nzb, dx);
+    const int len = orc_cmp_int(orc_cmp_int(s->nxb, s->nyb), s->nzb);
+    double *f = (double *)malloc(sizeof(double) * 3 * len), *fx = f + len, *fxx = fx + len;
+    for (int c = 0; c < 3; ++c) {
+        double *U = vel[c];
+        for (int k = ks; k <= ke; ++k)
+            for (int j = 1; j <= ny; ++j) {
+                for (int i = 1 - GHC; i <= nx + GHC; ++i) f[i - 1 + GHC] = AT(U, i, j, k);
+                ccd_solve(&m[0], f, fx, fxx);
+                for (int i = 1 - GHC; i <= nx + GHC; ++i) AT(ten[3 * c + 0], i, j, k) = fx[i - 1 + GHC];
+            }
+        for (int k = ks; k <= ke; ++k)
+            for (int i = 1; i <= nx; ++i) {
+                for (int j = 1 - GHC; j <= ny + GHC; ++j) f[j - 1 + GHC] = AT(U, i, j, k);
+                ccd_solve(&m[1], f, fx, fxx);
+                for (int j = 1 - GHC; j <= ny + GHC; ++j) AT(ten[3 * c + 1], i, j, k) = fx[j - 1 + GHC];
+            }
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                for (int k = ks - GHC; k <= ke + GHC; ++k) f[k - ks + GHC] = AT(U, i, j, k);
+                ccd_solve(&m[2], f, fx, fxx);
+                for (int k = ks - GHC; k <= ke + GHC; ++k) AT(ten[3 * c + 2], i, j, k) = fx[k - ks + GHC];
+            }
+    }
+    free(f);
+    ccd_free(&m[0]); ccd_free(&m[1]); ccd_free(&m[2]);
+    double A = 0.0, Ke = 0.0, Po = 0.0, Q = 0.0, v = 0.0, xc = 0.0, yc = 0.0, zc = 0.0;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    const double r12 = s->p.rho_12;
+    for (int k = ks; k <= ke; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const double phi = AT(P, i, j, k);
+                if (phi < dx && phi > 0.0) A = A + dv;
+                const double marker = vof_marker ? AT(VF, i, j, k) : AT(H, i, j, k);
+                const double u = AT(vel[0], i, j, k), vv = AT(vel[1], i, j, k), w = AT(vel[2], i, j, k);
+                const double x = coord(s->p.xstart, s->p.xend, s->p.dx, nx, i);
+                const double y = coord(s->p.ystart, s->p.yend, s->p.dy, ny, j);
+                const double z = coord(s->p.zstart, s->p.zend, s->p.dz, s->p.nz, k);
+                px = px + marker * u * dv;
+                py = py + marker * vv * dv;
+                pz = pz + marker * w * dv;
+                v = v + marker * dv;
+                xc = xc + marker * x * dv;
+                yc = yc + marker * y * dv;
+                zc = zc + marker * z * dv;
+                if (marker > 0.99) {
+                    double rho, mu;
+                    if (!s->p.inverse) {
+                        rho = marker + r12 * (1.0 - marker);
+                        mu = marker + mu_12 * (1.0 - marker);
+                    } else {
+                        rho = marker * r12 + (1.0 - marker);
+                        mu = marker * mu_12 + (1.0 - marker);
+                    }
+                    const double lke = u * u + vv * vv + w * w;
+                    Ke = Ke + lke * rho * 0.5 * dv;
+                    Po = Po + z * rho * dv;
+                    const double ux = AT(ten[0], i, j, k), uy = AT(ten[1], i, j, k), uz = AT(ten[2], i, j, k);
+                    const double vx = AT(ten[3], i, j, k), vy = AT(ten[4], i, j, k), vz = AT(ten[5], i, j, k);
+                    const double wx = AT(ten[6], i, j, k), wy = AT(ten[7], i, j, k), wz = AT(ten[8], i, j, k);
+                    const double lq = 0.5 * (sq(uy + vx) + sq(uz + wx) + sq(wy + vz)) + sq(ux) + sq(vy) + sq(wz);
+                    Q = Q + lq * 2.0 * mu * dv;
+                }
+            }
+    for (int q = 0; q < 9; ++q) free(ten[q]);
+    out[0] = A / dx;
+    out[1] = v;
+    out[2] = px / v; out[3] = py / v; out[4] = pz / v;
+    out[5] = xc / v; out[6] = yc / v; out[7] = zc / v;
+    out[8] = Ke; out[9] = Po; out[10] = Q;
+}
+
+/* ------------------------------------------------------------------------
  * Inputs.
  * ---------------------------------------------------------------------- */
 /* cell-centre coordinate x(i) = xstart + (i - 0.5)*dx with (i - 0.5) in single

@@ -93,6 +93,9 @@ int orc_redis(orc_state *s, int btn, double timestop);
 void orc_surface_norms(orc_state *s);                     /* derivatives.f90:1-50, 96-116 */
 int orc_mass_fix(orc_state *s);                           /* mass_preserving_level_set.f90:1-74 */
 double orc_interface_l1(orc_state *s);                    /* output.f90:169-204, 233-248 */
+/* output.f90:250-397 with rho_mu (ns_solver/others.f90:47-80): out[11] = A, v, px/v, py/v,
+   pz/v, xc/v, yc/v, zc/v, Ke, Po, Q (unscaled); vof_marker: method 3's marker */
+void orc_energy_momentum(orc_state *s, double mu_12, int vof_marker, double *out);
 
 /* ---- inputs (what the reference's setup routines produce) ---- */
 /* src/setup/velocity.f90:1-60 (3D Vortex form and 2D Vortex2D form),

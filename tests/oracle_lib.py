@@ -43,6 +43,10 @@ class OrcParams(C.Structure):
     ]
 
 
+# order of the energy_and_momentum integrals (oracle and library alike)
+ENERGY_KEYS = ("A", "v", "px", "py", "pz", "xc", "yc", "zc", "Ke", "Po", "Q")
+
+
 def _dp(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(DP)
@@ -94,6 +98,7 @@ def lib():
         L.orc_vof_wlic.argtypes = [C.c_void_p]
         L.orc_clsvof_recon.argtypes = [C.c_void_p]
         L.orc_clsvof_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.orc_energy_momentum.argtypes = [C.c_void_p, C.c_double, C.c_int, DP]
         _lib = L
     return _lib
 
@@ -178,3 +183,10 @@ class Oracle:
 
     def bc_field(self, name):
         lib().orc_bc(self.h, lib().orc_field(self.h, F[name]))
+
+    def energy_momentum(self, mu_12=1.0, vof_marker=False):
+        """energy_and_momentum's integrals (output.f90:250-397) before the deck's
+        scaling: dict of A, v, px, py, pz, xc, yc, zc, Ke, Po, Q."""
+        out = np.zeros(11)
+        lib().orc_energy_momentum(self.h, float(mu_12), int(bool(vof_marker)), _dp(out))
+        return dict(zip(ENERGY_KEYS, out.tolist()))
