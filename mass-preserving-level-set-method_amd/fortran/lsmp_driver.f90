@@ -10,9 +10,12 @@
 ! metric -- runs on the GPU through lsmp_mod's ISO_C_BINDING interface.
 !
 ! usage:  lsmp_driver <disk|vortex2d|vortex> [deck=./input/default.txt] [max_steps]
-!                     [fp64|fp32] [crweno|uccd]
-! (max_steps 0 = no limit; the last two choose the bulk precision and the
-! level_set_rk3_source variant, level_set_rk3_solver.f90:96-102)
+!                     [fp64|fp32] [crweno|uccd] [energy]
+! (max_steps 0 = no limit; the next two choose the bulk precision and the
+! level_set_rk3_source variant, level_set_rk3_solver.f90:96-102; "energy"
+! adds energy_and_momentum and its Energy, Momentum and Position rows every
+! step, output.f90:33-36, 250-397 -- the reference's Navier-Stokes
+! diagnostics, nine CCD solves of the velocity per step, off by default)
 !
 ! The deck has the reference's format (odd lines are comments).  Like the
 ! reference's shipped cases it needs threads grid 1 1 1 and ghc = 3; methods 1
@@ -32,6 +35,8 @@ integer(8) :: clk0, clk1, clkrate
 real(8) :: xstart, xend, ystart, yend, zstart, zend, t2s, t2p, dt, rdt, ls_wid
 real(8) :: p_tol, p_w1, p_w2, p_b, t_tol, t_w, rho_1, mu_1, rho_2, mu_2, sigma, g, ggx, ggy, ggz
 real(8) :: cl, cu, ct, re, fr, we, rho_12, mu_12
+real(8) :: energy_unit, ek, ep, es, ev, ek0, ep0, es0, em(11)
+logical :: energy
 integer :: btn_sf, btn_g
 integer :: nx, ny, nz, kb0, kb1
 real(8) :: dx, dy, dz, time, mass, vol, imass, ivol, red_error, l1
@@ -64,6 +69,8 @@ if (trim(arg) == "fp32") precision = LSMP_FP32
 scheme = LSMP_SCHEME_CRWENO
 call get_command_argument(5, arg)
 if (trim(arg) == "uccd") scheme = LSMP_SCHEME_UCCD
+call get_command_argument(6, arg)
+energy = trim(arg) == "energy"
 
 select case (trim(casename))
 case ("disk")
@@ -130,6 +137,30 @@ if (how_to_paras < 1 .or. how_to_paras > 5) then
     stop 1
 endif
 if (how_to_paras > 1) rho_12 = rho_2 / rho_1                 ! tree.f90:310-312
+! reference scales and the dimensionless groups (tree.f90:289-318): cl, cu, ct
+! are the deck's L, U, T
+select case (how_to_paras)
+case (1)
+    cl = ((re * mu_1 / rho_1)**2.0d0 * g / fr)**(1.0d0/3.0d0)
+    cu = dsqrt(fr * g * cl)
+    ct = cl / cu
+case (2)
+    cu = dsqrt(cl * g)
+    ct = cl / cu
+case (3)
+    ct = cl / cu
+case (4)
+    cu = cl / ct
+case (5)
+    cl = cu * ct
+end select
+if (how_to_paras > 1) then
+    mu_12 = mu_2 / mu_1
+    re = rho_1 * cu * cl / mu_1
+    we = rho_1 * cu**2.0d0 * cl / sigma
+    fr = cu**2.0d0 / (g * cl)
+endif
+energy_unit = rho_1 * g * cl**4.0
 
 ! ------------------------------------------------------------ grid (tree.f90:230-280)
 nx = int(ug * (xend - xstart))
@@ -219,6 +250,8 @@ loss_mass_avg = 0.0d0; loss_mass_max = 0.0d0
 loss_vol_avg = 0.0d0; loss_vol_max = 0.0d0
 loss_mass_avgv = 0.0d0; loss_mass_maxv = 0.0d0
 loss_vol_avgv = 0.0d0; loss_vol_maxv = 0.0d0
+ev = -1.0d0                                                  ! problem_init.f90:138
+ek0 = 0.0d0; ep0 = 0.0d0; es0 = 0.0d0
 call lsmp_check(c_lsmp_synchronize(h), "lsmp_synchronize")
 call system_clock(clk0, clkrate)
 do
@@ -269,6 +302,24 @@ do
     loss_vol_avg = loss_vol_avg + abs(ivol-vol)/ivol
     loss_mass_max = max(loss_mass_max, abs(imass-mass)/imass)
     loss_vol_max = max(loss_vol_max, abs(ivol-vol)/ivol)
+
+    if (energy) then
+        ! energy_and_momentum (output.f90:250-397) and its rows (:33-36)
+        call lsmp_check(c_lsmp_energy_momentum(h, mu_12, merge(1_c_int, 0_c_int, method == 3), em), &
+                        "lsmp_energy_momentum")
+        es = em(1) * fr / we * energy_unit * btn_sf
+        ek = em(9) * fr * energy_unit
+        ep = em(10) * energy_unit
+        if (ev < 0.0d0) then
+            es0 = es; ek0 = ek; ep0 = ep
+            ev = 0.0d0
+        else
+            ev = ev + em(11) * dt * fr / re * energy_unit
+        endif
+        write(fen, '(7ES15.4)') time, ek, ep, es, ev, ek+es+ep+ev, ek0+ep0+es0
+        write(fmom, '(7ES15.4)') time, em(3), em(4), em(5)
+        write(fpos, '(7ES15.4)') time, em(6), em(7), em(8)
+    endif
 
     if (mod(iter, 5) == 0) then
         write(*,*) "========================================="

@@ -78,3 +78,10 @@ int lsmp_clsvof_recon(lsmp_handle h, int *a, int *b)
     return LSMP_OK;
 }
 int lsmp_vof_interface_l1(lsmp_handle h, double *l1) { (void)h; rec("lsmp_vof_interface_l1"); *l1 = 0.0; return LSMP_OK; }
+int lsmp_energy_momentum(lsmp_handle h, double mu_12, int vof_marker, double *out)
+{
+    (void)h; (void)mu_12; (void)vof_marker;
+    rec("lsmp_energy_momentum");
+    for (int q = 0; q < 11; ++q) out[q] = 1.0;
+    return LSMP_OK;
+}

@@ -98,3 +98,46 @@ def test_driver_classical_ls_loses_mass(tmp_path):
     """Method 1 (classical_ls, interface_solver.f90:15-23) has no mass fix."""
     steps, l1, dmax = run_driver(tmp_path, "vortex", "vortex_ls_16.txt", 4)
     assert dmax > 1e-8
+
+
+@pytest.mark.gpu
+def test_driver_energy_rows(tmp_path):
+    """The driver's `energy` option: Energy, Momentum and Position rows every
+    step (output.f90:33-36) from lsmp_energy_momentum with the deck's scales
+    (tree.f90:289-318, selector 1), against the same calls from Python; the
+    rows are printed with four significant digits."""
+    steps = 3
+    r = subprocess.run([DRIVER, "vortex", os.path.join(DECKS, "vortex_mpls_16.txt"), str(steps), "fp64", "crweno",
+                        "energy"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    en = np.loadtxt(tmp_path / "out" / "Vortex16_Energy.plt", skiprows=1, ndmin=2)
+    mom = np.loadtxt(tmp_path / "out" / "Vortex16_Momentum.plt", skiprows=1, ndmin=2)
+    pos = np.loadtxt(tmp_path / "out" / "Vortex16_Position.plt", skiprows=1, ndmin=2)
+    assert en.shape == (steps, 7) and mom.shape == (steps, 4) and pos.shape == (steps, 4)
+    # deck vortex_mpls_16.txt: rho_1, mu_1, sigma, g, Re, Fr, We, mu_12, btn_sf
+    rho_1, mu_1, g, re, fr, we, mu_12 = 995.0, 0.0007972, 9.81, 67.27, 1.0, 16.0, 0.01
+    L = ((re * mu_1 / rho_1) ** 2 * g / fr) ** (1.0 / 3.0)
+    unit = rho_1 * g * L ** 4
+    lsmp = load_lsmp()
+    o = O.Oracle(O.make_params(16, dim=3))
+    o.initial_phi(2, 0)
+    s = lsmp.Solver(lsmp.make_params(16, dim=3))
+    s.upload_phi(o.field("PHI"))
+    s.store_phi0()
+    imass, _ = s.mass_vol()
+    t, ev = 0.0, -1.0
+    for it in range(1, steps + 1):
+        t = t + s.params.dt
+        s.mpls_step(t, 3.0, 2, imass, iteration=it)
+        e = s.energy_momentum(mu_12)
+        es, ek, ep = e["A"] * fr / we * unit, e["Ke"] * fr * unit, e["Po"] * unit
+        if ev < 0:
+            e0, ev = ek + ep + es, 0.0
+        else:
+            ev = ev + e["Q"] * s.params.dt * fr / re * unit
+        row = en[it - 1]
+        assert row[1:5] == pytest.approx([ek, ep, es, ev], rel=1e-3, abs=1e-300)
+        assert row[6] == pytest.approx(e0, rel=1e-3)
+        assert mom[it - 1, 1:] == pytest.approx([e["px"], e["py"], e["pz"]], rel=1e-3, abs=1e-8)
+        assert pos[it - 1, 1:] == pytest.approx([e["xc"], e["yc"], e["zc"]], rel=1e-3)
+    s.close()

@@ -28,10 +28,10 @@ def driver():
     return DRIVER
 
 
-def run(driver, tmp_path, case, deck, steps):
+def run(driver, tmp_path, case, deck, steps, extra=()):
     log = tmp_path / "calls.log"
     env = dict(os.environ, LSMP_MOCK_LOG=str(log))
-    r = subprocess.run([driver, case, os.path.join(DECKS, deck), str(steps)], cwd=tmp_path, env=env,
+    r = subprocess.run([driver, case, os.path.join(DECKS, deck), str(steps)] + list(extra), cwd=tmp_path, env=env,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     return log.read_text().split(), r.stdout
@@ -44,6 +44,21 @@ def test_mpls_order(driver, tmp_path):
         + ["lsmp_synchronize", "lsmp_interface_l1", "lsmp_finalize"]
     assert calls == want
     assert "Method: MPLS" in (tmp_path / "Vortex_Results.txt").read_text()
+
+
+def test_energy_option_adds_the_diagnostic_each_step(driver, tmp_path):
+    """With `energy` the driver calls energy_and_momentum after the step's
+    mass sums (output.f90:33) and writes one row per step to the Energy,
+    Momentum and Position files."""
+    calls, _ = run(driver, tmp_path, "vortex", "vortex_mpls_16.txt", 3, ("fp64", "crweno", "energy"))
+    step = ["lsmp_set_velocity_analytic", "lsmp_rk3_advect", "lsmp_reinit(1)", "lsmp_mass_fix", "lsmp_mass_vol",
+            "lsmp_energy_momentum"]
+    want = INIT + ["lsmp_store_phi0", "lsmp_mass_vol", "lsmp_synchronize"] + step * 3 \
+        + ["lsmp_synchronize", "lsmp_interface_l1", "lsmp_finalize"]
+    assert calls == want
+    for f, cols in (("Energy", 7), ("Momentum", 4), ("Position", 4)):
+        lines = (tmp_path / "out" / f"Vortex16_{f}.plt").read_text().splitlines()
+        assert len(lines) == 4 and all(len(l.split()) == cols for l in lines[1:])
 
 
 def test_classical_ls_has_no_mass_fix(driver, tmp_path):
