@@ -7,8 +7,8 @@
 // the band |phi| < ls_wid.  For the cells inside it, it also stores |grad| in
 // a side field and lists the cell: first in the workgroup's list in LDS,
 // which the workgroup appends to the device list with one atomic at its end.
-// k_redis3d_band then takes the listed cells, one per thread over the whole
-// device, forms s2 = delta S0 (|grad| - 1) and s3 = |grad| delta^2 of each
+// k_redis3d_band then takes the listed cells, four lanes per cell over the
+// whole device, forms s2 = delta S0 (|grad| - 1) and s3 = |grad| delta^2 of each
 // band cell of the 3x3x3 box around it, lambda from their sums (centre weight
 // 51) and the cell's final value with the same expression.  Off the band s2
 // and s3 are zero, so the box sums take band cells only.
@@ -23,8 +23,9 @@
 // direction evaluates one face m+1/2 per cell.  z: each plane evaluates face
 // p+1/2 and carries its D- term and the second differences to the next
 // plane.  x and y: each thread evaluates the faces i+1/2 and j+1/2 of its
-// cell and hands the D- terms to its neighbours through LDS; wave 0 also
-// evaluates the tile's first faces i0-1/2 and j0-1/2.  Two barriers per
+// cell and hands the D- terms to its neighbours through LDS; one wave (a
+// different one from tile to tile) also evaluates the tile's first faces
+// i0-1/2 and j0-1/2.  Two barriers per
 // plane: the plane is in LDS, the D- terms are.
 //
 // The RK stage is written in Shu-Osher form, algebraically identical to the
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(NT) void k_redis3d(const T *__restrict__ phi, T *__
     // D- terms of the x and y faces: column tx (row ty) holds the one of cell tx (ty)
     __shared__ T mxs[R3Y][R3X + 1];
     __shared__ T mys[R3Y + 1][R3X];
-    static_assert(R3X + R3Y <= 64, "the first faces in wave 0");
+    static_assert(R3X + R3Y <= 64, "the first faces in one wave");
     // the workgroup's band cells, (plane - k0) * NT + thread, and their count
     __shared__ unsigned short lcell[BTN == 1 ? KC * NT : 1];
     __shared__ unsigned lcount, lbase;
