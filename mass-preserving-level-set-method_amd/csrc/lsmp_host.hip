@@ -880,7 +880,7 @@ int lsmp_init(const lsmp_params *params, lsmp_handle *out)
     // 8 lines per block, then the unlisted lines' sums) or reduction kernel
     size_t maxlines = 0;
     for (int d = 1; d < c->dim; ++d) maxlines = std::max(maxlines, c->ncell / (size_t)ns[d]);
-    c->npartial = (int)std::max<size_t>(1024, (maxlines + 7) / 8 + (maxlines + 255) / 256);
+    c->npartial = (int)std::max<size_t>(1024, (maxlines + 7) / 8 + LINE_SEGMENTS * ((maxlines + 255) / 256));
     if (alloc((void **)&c->partial, (size_t)3 * c->npartial * sizeof(double))) return fail(LSMP_E_HIP);
     if (alloc((void **)&c->scal, 8 * sizeof(double))) return fail(LSMP_E_HIP);
     if (alloc((void **)&c->u64, 4 * sizeof(unsigned long long))) return fail(LSMP_E_HIP);

@@ -140,7 +140,10 @@ template <typename T>
 void launch_ccd_pass(hipStream_t st, int dir, int dim, bool final_pass, const T *phi, const T *coef, T *scr, T *out,
                      const T *dxv, const T *dyv, T *gr, double *partial, int nx, int ny, int nz, const CcdArgs &ca,
                      int *nblocks_out, const int *lines = nullptr, int nlist = 0);
-// mass and volume partial sums of the lines of direction dir with lflag 0
+// mass and volume partial sums of the lines of direction dir with lflag 0,
+// each line in up to LINE_SEGMENTS segments (one block of partial sums per 256
+// lines and segment)
+constexpr int LINE_SEGMENTS = 8;
 template <typename T>
 void launch_line_hsums(hipStream_t st, int dir, const T *phi, const unsigned char *lflag, int nx, int ny, int nz,
                        const CcdArgs &ca, double *partial, int *nblocks_out);

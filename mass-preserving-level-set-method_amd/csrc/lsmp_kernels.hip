@@ -1652,25 +1652,28 @@ __global__ __launch_bounds__(1024) void k_compact_lines(const unsigned char *__r
 }
 
 // Mass and volume sums (the ls_mv terms of the final CCD pass) of the lines
-// without a band cell (lflag 0), one lane per line: delta = 0 on them, so
-// they add nothing to the correction's denominator and need no solve.
+// without a band cell (lflag 0): delta = 0 on them, so they add nothing to the
+// correction's denominator and need no solve.  One lane per line and segment
+// of seg cells (blockIdx.y the segment): a lane walking a whole line keeps one
+// wave per SIMD busy with a chain of dependent load batches.
 template <typename T>
 __global__ __launch_bounds__(256) void k_line_hsums(const T *__restrict__ phi, const unsigned char *__restrict__ lflag,
                                                     int nlines, int n, int lw, long ostride, long istride,
-                                                    long estride, CcdArgs ca, double *__restrict__ partial)
+                                                    long estride, int seg, CcdArgs ca, double *__restrict__ partial)
 {
     const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    const int qb = (int)blockIdx.y * seg, qe = min(qb + seg, n);
     double acc[3] = {0.0, 0.0, 0.0}, cmp[3] = {0.0, 0.0, 0.0};
     if (L < nlines && !lflag[L]) {
         const size_t base = (size_t)(L / lw) * (size_t)ostride + (size_t)(L % lw) * (size_t)istride;
         const double r12 = ca.rho_12;
-        for (int q0 = 0; q0 < n; q0 += 16) {
+        for (int q0 = qb; q0 < qe; q0 += 16) {
             T v[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) v[r] = phi[base + (size_t)min(q0 + r, n - 1) * (size_t)estride];
+            for (int r = 0; r < 16; ++r) v[r] = phi[base + (size_t)min(q0 + r, qe - 1) * (size_t)estride];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                if (q0 + r < n) {
+                if (q0 + r < qe) {
                     double h, dd;
                     ls_funs<double>((double)v[r], ca.ls_wid, h, dd);
                     csum_add(acc[0], cmp[0], h * ca.dv);
@@ -1682,9 +1685,10 @@ __global__ __launch_bounds__(256) void k_line_hsums(const T *__restrict__ phi, c
     }
     block_sum_comp<3>(acc, cmp);
     if (threadIdx.x == 0) {
-        partial[3 * blockIdx.x + 0] = acc[0];
-        partial[3 * blockIdx.x + 1] = acc[1];
-        partial[3 * blockIdx.x + 2] = acc[2];
+        const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        partial[3 * b + 0] = acc[0];
+        partial[3 * b + 1] = acc[1];
+        partial[3 * b + 2] = acc[2];
     }
 }
 
@@ -2025,10 +2029,13 @@ void launch_line_hsums(hipStream_t st, int dir, const T *phi, const unsigned cha
     const long ostride = dir == 0 ? 0 : dir == 1 ? (long)nx * ny : (long)nx;
     const long istride = dir == 0 ? (long)nx : 1;
     const long estride = dir == 0 ? 1 : dir == 1 ? (long)nx : (long)nx * ny;
-    const dim3 g(nblk(nall, 256));
+    // up to LINE_SEGMENTS segments of at least 32 cells per line
+    const int nseg = std::max(1, std::min(LINE_SEGMENTS, n / 32));
+    const int seg = (n + nseg - 1) / nseg;
+    const dim3 g(nblk(nall, 256), (unsigned)nseg);
     hipLaunchKernelGGL((k_line_hsums<T>), g, dim3(256), 0, st, phi, lflag, nall, n, lw, ostride, istride, estride,
-                       ca, partial);
-    *nblocks_out = (int)g.x;
+                       seg, ca, partial);
+    *nblocks_out = (int)(g.x * g.y);
 }
 
 template <typename T>
