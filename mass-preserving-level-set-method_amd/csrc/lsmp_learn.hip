@@ -120,51 +120,37 @@ LSMP_DEV float feature(const T *__restrict__ phi, const Geom &g, int i, int j, i
 // band list, pair collection, residual
 // ---------------------------------------------------------------------------
 // Ordered narrow-band compaction: cells with |phi| < thr in ascending index
-// order.  Each workgroup of 256 threads owns BAND_CHUNK consecutive cells (4 per
-// thread); pass 1 counts, a one-workgroup scan makes offsets, pass 2 writes.
-constexpr int BAND_CHUNK = 1024;
+// order.  Each workgroup of 256 threads owns BAND_CHUNK consecutive cells as
+// BAND_ROWS rows of 256 (thread t reads cell t of every row, so every load is
+// one coalesced 1 KB (FP32) or 2 KB (FP64) row); pass 1 counts, a
+// one-workgroup scan makes the workgroups' offsets, pass 2 places each cell by
+// the ballots of its row: the cells of earlier rows, of earlier waves in its
+// row, and of lower lanes in its wave come first.
+constexpr int BAND_ROWS = 16;
+constexpr int BAND_CHUNK = 256 * BAND_ROWS;
+static_assert(4 * BAND_ROWS == 64, "k_band_write scans the 4 x BAND_ROWS wave counts with one wave");
 
 template <typename T>
-LSMP_DEV int band_flags(const T *__restrict__ phi, size_t n, T thr, size_t q0, bool (&f)[4])
+LSMP_DEV bool band_flag(const T *__restrict__ phi, size_t n, T thr, size_t q)
 {
-    int c = 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const size_t q = q0 + r;
-        f[r] = q < n && fabs(phi[q]) < thr;
-        c += f[r];
-    }
-    return c;
-}
-
-// Exclusive prefix of v over the block (256 threads) and the block total.
-LSMP_DEV int block_scan256(int v, int &total)
-{
-    __shared__ int wsum[4];
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
-        if (l >= o) x += y;
-    }
-    if (l == 63) wsum[w] = x;
-    __syncthreads();
-    int base = 0;
-    for (int q = 0; q < w; ++q) base += wsum[q];
-    total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    __syncthreads();
-    return base + x - v;
+    return q < n && fabs(phi[q]) < thr;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_band_count(const T *__restrict__ phi, size_t n, T thr,
                                                     unsigned *__restrict__ cnt)
 {
-    bool f[4];
-    const int c = band_flags(phi, n, thr, (size_t)blockIdx.x * BAND_CHUNK + 4 * threadIdx.x, f);
-    int total;
-    (void)block_scan256(c, total);
-    if (threadIdx.x == 0) cnt[blockIdx.x] = (unsigned)total;
+    __shared__ unsigned wsum[4];
+    const size_t q0 = (size_t)blockIdx.x * BAND_CHUNK + threadIdx.x;
+    bool f[BAND_ROWS];
+#pragma unroll
+    for (int r = 0; r < BAND_ROWS; ++r) f[r] = band_flag(phi, n, thr, q0 + (size_t)r * 256);
+    unsigned c = 0;
+#pragma unroll
+    for (int r = 0; r < BAND_ROWS; ++r) c += (unsigned)__popcll(__ballot(f[r]));
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
 // Exclusive scan of cnt[0..nblk) into off[], total into off[nblk]; one
@@ -201,14 +187,38 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_band_write(const T *__restrict__ phi, size_t n, T thr,
                                                     const unsigned *__restrict__ off, int *__restrict__ list)
 {
-    bool f[4];
-    const size_t q0 = (size_t)blockIdx.x * BAND_CHUNK + 4 * threadIdx.x;
-    const int c = band_flags(phi, n, thr, q0, f);
-    int total;
-    unsigned pos = off[blockIdx.x] + (unsigned)block_scan256(c, total);
+    // pre[4 r + w]: cells of row r's wave w, then their exclusive prefix in
+    // (row, wave) order
+    __shared__ unsigned pre[4 * BAND_ROWS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const size_t q0 = (size_t)blockIdx.x * BAND_CHUNK + threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    bool f[BAND_ROWS];
+    unsigned rank[BAND_ROWS];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-        if (f[r]) list[pos++] = (int)(q0 + r);
+    for (int r = 0; r < BAND_ROWS; ++r) f[r] = band_flag(phi, n, thr, q0 + (size_t)r * 256);
+#pragma unroll
+    for (int r = 0; r < BAND_ROWS; ++r) {
+        const unsigned long long m = __ballot(f[r]);
+        rank[r] = (unsigned)__popcll(m & below);
+        if (lane == 0) pre[4 * r + w] = (unsigned)__popcll(m);
+    }
+    __syncthreads();
+    if (w == 0) {
+        const unsigned v = pre[lane];   // 4 * BAND_ROWS == 64 entries
+        unsigned x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        pre[lane] = x - v;
+    }
+    __syncthreads();
+    const unsigned base = off[blockIdx.x];
+#pragma unroll
+    for (int r = 0; r < BAND_ROWS; ++r)
+        if (f[r]) list[base + pre[4 * r + w] + rank[r]] = (int)(q0 + (size_t)r * 256);
 }
 
 // Pair c takes band entry first + c * stride.
