@@ -1745,36 +1745,40 @@ __global__ void k_l1(const T *__restrict__ phi, const T *__restrict__ phi0, size
 // (Vortex/lib/data/tree.f90:254-256).
 
 template <typename T>
-__global__ void k_velocity(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, VelArgs va,
-                           const double *__restrict__ tab)
+__global__ __launch_bounds__(256) void k_velocity(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
+                                                  VelArgs va, const double *__restrict__ tab)
 {
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t n = (size_t)va.nx * va.ny * va.nz;
-    if (q >= n) return;
-    const int i = (int)(q % va.nx), j = (int)((q / va.nx) % va.ny), k = (int)(q / ((size_t)va.nx * va.ny));
-    double uu, vv, ww = 0.0;
-    if (va.field == 0) {
-        const double x = va.xstart + (double)((float)(i + 1) - 0.5f) * va.dx;
-        const double y = va.ystart + (double)((float)(j + 1) - 0.5f) * va.dy;
-        uu = -va.omega * (y - 0.5);
-        vv = va.omega * (x - 0.5);
-    } else {
-        // sin(pi c) and sin(2 pi c) of each axis from the tables of k_vel_tables
-        const double *tx = tab, *ty = tab + 2 * va.nx, *tz = ty + 2 * va.ny;
-        const double sx = tx[2 * i], s2x = tx[2 * i + 1], sy = ty[2 * j], s2y = ty[2 * j + 1];
-        if (va.field == 1) {
-            uu = -sq(sx) * s2y * va.ct;
-            vv = sq(sy) * s2x * va.ct;
-        } else {
-            const double sz = tz[2 * k], s2z = tz[2 * k + 1];
-            uu = 2.0 * sq(sx) * s2y * s2z * va.ct;
-            vv = -s2x * sq(sy) * s2z * va.ct;
-            ww = -s2x * s2y * sq(sz) * va.ct;
+    // the (x, y) threads of a workgroup cover a block of rows of planes
+    // k = blockIdx.z, blockIdx.z + gridDim.z, ...
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= va.nx) return;
+    const double *tx = tab, *ty = tab + 2 * va.nx, *tz = ty + 2 * va.ny;
+    for (int k = blockIdx.z; k < va.nz; k += gridDim.z)
+        for (int j = blockIdx.y * blockDim.y + threadIdx.y; j < va.ny; j += gridDim.y * blockDim.y) {
+            const size_t q = ((size_t)k * va.ny + j) * va.nx + i;
+            double uu, vv, ww = 0.0;
+            if (va.field == 0) {
+                const double x = va.xstart + (double)((float)(i + 1) - 0.5f) * va.dx;
+                const double y = va.ystart + (double)((float)(j + 1) - 0.5f) * va.dy;
+                uu = -va.omega * (y - 0.5);
+                vv = va.omega * (x - 0.5);
+            } else {
+                // sin(pi c) and sin(2 pi c) of each axis from the tables of k_vel_tables
+                const double sx = tx[2 * i], s2x = tx[2 * i + 1], sy = ty[2 * j], s2y = ty[2 * j + 1];
+                if (va.field == 1) {
+                    uu = -sq(sx) * s2y * va.ct;
+                    vv = sq(sy) * s2x * va.ct;
+                } else {
+                    const double sz = tz[2 * k], s2z = tz[2 * k + 1];
+                    uu = 2.0 * sq(sx) * s2y * s2z * va.ct;
+                    vv = -s2x * sq(sy) * s2z * va.ct;
+                    ww = -s2x * s2y * sq(sz) * va.ct;
+                }
+            }
+            u[q] = (T)uu;
+            v[q] = (T)vv;
+            w[q] = (T)ww;
         }
-    }
-    u[q] = (T)uu;
-    v[q] = (T)vv;
-    w[q] = (T)ww;
 }
 
 // sin(pi c) and sin(2 pi c) at the cell centres c of each axis
@@ -2143,7 +2147,11 @@ void launch_velocity(hipStream_t st, T *u, T *v, T *w, const VelArgs &va, double
     const size_t n = (size_t)va.nx * va.ny * va.nz;
     if (va.field != 0)
         hipLaunchKernelGGL(k_vel_tables, dim3(nblk((size_t)va.nx + va.ny + va.nz, 256)), dim3(256), 0, st, va, tab);
-    hipLaunchKernelGGL((k_velocity<T>), dim3(nblk(n, 256)), dim3(256), 0, st, u, v, w, va, tab);
+    // rows of up to 256 cells, several rows per workgroup when they are short
+    const int bx = (int)std::min<size_t>(256, (va.nx + 63) / 64 * 64), by = 256 / bx;
+    const dim3 g((unsigned)((va.nx + bx - 1) / bx), (unsigned)std::min((va.ny + by - 1) / by, 65535),
+                 (unsigned)std::min(va.nz, 65535));
+    if (n) hipLaunchKernelGGL((k_velocity<T>), g, dim3(bx, by), 0, st, u, v, w, va, tab);
 }
 
 // Merge of the two markers of the MPLS two-marker branch
