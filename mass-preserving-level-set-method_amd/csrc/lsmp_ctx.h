@@ -52,6 +52,7 @@ struct lsmp_ctx {
     int rb_par = 0;
     int *band_list = nullptr;        // narrow-band cell indices (learned reinitialisation)
     unsigned *band_count = nullptr;
+    void *band_val = nullptr;        // the network's value per band entry
     void *mk[2] = {nullptr, nullptr};   // the two markers' fields (two-marker MPLS branch)
     unsigned *mflag = nullptr;          // "inside both markers" flag of the merge
     bool mk_set[2] = {false, false};

@@ -579,16 +579,22 @@ __global__ void k_net_update(float *__restrict__ P, float *__restrict__ m, float
     P[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + 1.0e-8f);
 }
 
-// Inference on the band cells: phi_out = phi_c + h * net(features) * gate,
-// gate = clamp((|phi_c| - w) / w, 0, 1) with w = ls_wid: the network leaves
-// the interface region |phi| <= ls_wid to the plain sweeps and ramps in over
-// [ls_wid, 2 ls_wid].  Persistent workgroups: each stages the weights into LDS
-// once and then takes tiles of TILE band cells blockIdx.x, + gridDim.x, ...
+// Inference on the band cells: val[e] = phi_c + h * net(features) * gate for
+// band entry e (cell list[e]), gate = clamp((|phi_c| - w) / w, 0, 1) with
+// w = ls_wid: the network leaves the interface region |phi| <= ls_wid to the
+// plain sweeps and ramps in over [ls_wid, 2 ls_wid].  The band's length is
+// read from device memory (*nlist_p).  Persistent workgroups: each stages the
+// weights into LDS once and then takes tiles of TILE band cells blockIdx.x,
+// + gridDim.x, ...; workgroups past the last tile return at once.
 template <typename T>
-__global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T *__restrict__ phi_out,
-                                                   const int *__restrict__ list, int nlist, Geom g,
+__global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T *__restrict__ val,
+                                                   const int *__restrict__ list,
+                                                   const unsigned *__restrict__ nlist_p, Geom g,
                                                    const float *__restrict__ P, T h, T w)
 {
+    const int nlist = (int)*nlist_p;
+    const int ntiles = (nlist + TILE - 1) / TILE;
+    if ((int)blockIdx.x >= ntiles) return;   // the whole workgroup, before any barrier
     __shared__ NetLds L;
     __shared__ __attribute__((aligned(16))) __bf16 sh1[4][16][KP2];
     __shared__ __attribute__((aligned(16))) __bf16 sl1[4][16][KP2];
@@ -604,7 +610,6 @@ __global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T 
     stage_weights(L, P, nullptr);
     __syncthreads();
     const int t = threadIdx.x, wv = t >> 6, l = t & 63, lr = l & 15, lg = l >> 4;
-    const int ntiles = (nlist + TILE - 1) / TILE;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int c = tile * TILE + wv * 16 + lr;
         const bool valid = c < nlist;
@@ -662,10 +667,22 @@ __global__ __launch_bounds__(256) void k_net_infer(const T *__restrict__ phi, T 
         __syncthreads();
         if (valid && lg == 0) {
             const T gate = fmin(fmax((fabs(pc) - w) / w, T(0)), T(1));
-            phi_out[idx] = pc + h * (T)sout[wv][lr] * gate;
+            val[c] = pc + h * (T)sout[wv][lr] * gate;
         }
         __syncthreads();   // sout and the wave's H1 tiles are reused by the next tile
     }
+}
+
+// phi[list[e]] = val[e] for the *nlist_p band entries (the network's values,
+// formed from the whole field before any is stored)
+template <typename T>
+__global__ __launch_bounds__(256) void k_band_scatter(T *__restrict__ phi, const T *__restrict__ val,
+                                                      const int *__restrict__ list,
+                                                      const unsigned *__restrict__ nlist_p)
+{
+    const size_t n = *nlist_p;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        phi[list[e]] = val[e];
 }
 
 template <typename TO, typename TI>
@@ -742,11 +759,13 @@ int ensure_band_list(lsmp_ctx *c)
     const size_t nblk = (c->ncell + BAND_CHUNK - 1) / BAND_CHUNK;
     NET_HIP(hipMalloc(&c->band_list, c->ncell * sizeof(int)));
     NET_HIP(hipMalloc(&c->band_count, (2 * nblk + 1) * sizeof(unsigned)));
+    NET_HIP(hipMalloc(&c->band_val, c->ncell * (c->fp32 ? sizeof(float) : sizeof(double))));
     return LSMP_OK;
 }
 
-// Ordered list of the cells with |phi| < thr; returns its length.
-template <typename T> int build_band(lsmp_ctx *c, const T *phi, double thr, unsigned *n_out)
+// Queues the ordered list of the cells with |phi| < thr; *count_dev is where
+// its length lands in device memory.
+template <typename T> int queue_band(lsmp_ctx *c, const T *phi, double thr, const unsigned **count_dev)
 {
     int rc = ensure_band_list(c);
     if (rc) return rc;
@@ -757,7 +776,17 @@ template <typename T> int build_band(lsmp_ctx *c, const T *phi, double thr, unsi
     hipLaunchKernelGGL((k_band_write<T>), dim3(nblk), dim3(256), 0, c->st, phi, c->ncell, (T)thr, off,
                        c->band_list);
     NET_HIP(hipGetLastError());
-    NET_HIP(hipMemcpyAsync(c->hbuf, off + nblk, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+    *count_dev = off + nblk;
+    return LSMP_OK;
+}
+
+// The same list; returns its length.
+template <typename T> int build_band(lsmp_ctx *c, const T *phi, double thr, unsigned *n_out)
+{
+    const unsigned *count_dev = nullptr;
+    int rc = queue_band<T>(c, phi, thr, &count_dev);
+    if (rc) return rc;
+    NET_HIP(hipMemcpyAsync(c->hbuf, count_dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
     NET_HIP(hipStreamSynchronize(c->st));
     unsigned n = 0;
     std::memcpy(&n, c->hbuf, sizeof n);
@@ -824,28 +853,32 @@ template <typename T> int residual(lsmp_ctx *c, double band_cells, double *res)
     return LSMP_OK;
 }
 
-// phi <- network(phi) on |phi| < band_cells dx (phi unchanged elsewhere).
+// phi <- network(phi) on |phi| < band_cells dx (phi unchanged elsewhere), in
+// place: the network's values go to band_val, then onto phi.  No host round
+// trip unless the band's length is asked for.
 template <typename T> int apply(lsmp_ctx *c, lsmp_net *net, double band_cells, int *n_band)
 {
-    const size_t n = c->ncell;
-    T *phi = (T *)c->phi, *out = (T *)c->phiB;
-    NET_HIP(hipMemcpyAsync(out, phi, n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
-    unsigned nb = 0;
-    int rc = build_band<T>(c, phi, band_cells * c->p.dx, &nb);
+    T *phi = (T *)c->phi;
+    const unsigned *nb_dev = nullptr;
+    int rc = queue_band<T>(c, phi, band_cells * c->p.dx, &nb_dev);
     if (rc) return rc;
-    if (nb) {
-        // persistent workgroups, two per CU (the weights fill a third of the LDS)
-        int dev = 0, ncu = 256;
-        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        const unsigned nt = cdiv(nb, TILE), grid = std::min<unsigned>(nt, 2u * (unsigned)std::max(ncu, 1));
-        hipLaunchKernelGGL((k_net_infer<T>), dim3(grid), dim3(256), 0, c->st, phi, out, c->band_list, (int)nb,
-                           geom(c), net->P, (T)c->p.dx, (T)c->p.ls_wid);
-        NET_HIP(hipGetLastError());
+    T *val = (T *)c->band_val;   // allocated with the list on the first call
+    // persistent workgroups, two per CU (the weights fill a third of the LDS)
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const unsigned grid = std::min<unsigned>(cdiv(c->ncell, TILE), 2u * (unsigned)std::max(ncu, 1));
+    hipLaunchKernelGGL((k_net_infer<T>), dim3(grid), dim3(256), 0, c->st, phi, val, c->band_list, nb_dev, geom(c),
+                       net->P, (T)c->p.dx, (T)c->p.ls_wid);
+    hipLaunchKernelGGL((k_band_scatter<T>), dim3(std::min<unsigned>(cdiv(c->ncell, 256), 1024u)), dim3(256), 0,
+                       c->st, phi, val, c->band_list, nb_dev);
+    NET_HIP(hipGetLastError());
+    if (n_band) {
+        NET_HIP(hipMemcpyAsync(c->hbuf, nb_dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+        NET_HIP(hipStreamSynchronize(c->st));
+        unsigned nb = 0;
+        std::memcpy(&nb, c->hbuf, sizeof nb);
+        *n_band = (int)nb;
     }
-    c->phiB = c->phi;
-    c->phi = out;
-    NET_HIP(hipStreamSynchronize(c->st));
-    if (n_band) *n_band = (int)nb;
     return LSMP_OK;
 }
 
