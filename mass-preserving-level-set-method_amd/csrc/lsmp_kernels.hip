@@ -1899,26 +1899,34 @@ void launch_adv_pass(hipStream_t st, int dir, int mode, int stage, T *phi, const
 }
 
 // Narrow band: flag the reinitialisation workgroups (R3X x R3Y x RKC_NB cells)
-// holding a cell with |phi| < wband; the flags must be zero on entry.
+// holding a cell with |phi| < wband.  One workgroup per flag: thread (x, y)
+// reads its column's RKC_NB cells, and one thread stores the flag, 1 or 0.
+static_assert(R3X * R3Y == 256, "k_tube_tiles maps one thread to each column of a workgroup's cells");
 template <typename T>
-__global__ void k_tube_tiles(const T *__restrict__ phi, int nx, int ny, int nz, double wband,
-                             unsigned char *__restrict__ tiles)
+__global__ __launch_bounds__(256) void k_tube_tiles(const T *__restrict__ phi, int nx, int ny, int nz, double wband,
+                                                    unsigned char *__restrict__ tiles)
 {
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (size_t)nx * ny * nz) return;
-    if (!(fabs((double)phi[q]) < wband)) return;
-    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
-    const int gx = (nx + R3X - 1) / R3X, gy = (ny + R3Y - 1) / R3Y;
-    tiles[((size_t)(k / RKC_NB) * gy + j / R3Y) * gx + i / R3X] = 1;
+    const int i = blockIdx.x * R3X + (threadIdx.x % R3X), j = blockIdx.y * R3Y + (threadIdx.x / R3X);
+    const int k0 = blockIdx.z * RKC_NB;
+    bool hit = false;
+    if (i < nx && j < ny) {
+        T v[RKC_NB];
+#pragma unroll
+        for (int kk = 0; kk < RKC_NB; ++kk)
+            v[kk] = k0 + kk < nz ? phi[((size_t)(k0 + kk) * ny + j) * nx + i] : T(wband);
+#pragma unroll
+        for (int kk = 0; kk < RKC_NB; ++kk) hit = hit || fabs((double)v[kk]) < wband;
+    }
+    hit = __syncthreads_or(hit);
+    if (threadIdx.x == 0)
+        tiles[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = hit ? 1 : 0;
 }
 
 template <typename T>
 void launch_tube_tiles(hipStream_t st, const T *phi, int nx, int ny, int nz, double wband, unsigned char *tiles)
 {
-    const size_t nt = (size_t)nblk(nx, R3X) * nblk(ny, R3Y) * nblk(nz, RKC_NB);
-    hipLaunchKernelGGL(k_zero_bytes, dim3(nblk(nt, 256)), dim3(256), 0, st, tiles, nt);
-    const size_t n = (size_t)nx * ny * nz;
-    hipLaunchKernelGGL((k_tube_tiles<T>), dim3(nblk(n, 256)), dim3(256), 0, st, phi, nx, ny, nz, wband, tiles);
+    const dim3 g(nblk(nx, R3X), nblk(ny, R3Y), nblk(nz, RKC_NB));
+    hipLaunchKernelGGL((k_tube_tiles<T>), g, dim3(256), 0, st, phi, nx, ny, nz, wband, tiles);
 }
 
 template <typename T>
