@@ -835,8 +835,9 @@ template <typename T> int collect(lsmp_net *net, lsmp_ctx *c, double band_cells,
 template <typename T> int residual(lsmp_ctx *c, double band_cells, double *res)
 {
     const RedisArgs ra = lsmp_redis_args(c);
+    // as many workgroups as there are partial slots (10,240 at 256^3): the
+    // grid-stride loop is load-latency bound and needs the waves
     size_t b = (c->ncell + 255) / 256;
-    if (b > 1024) b = 1024;
     if ((int)b > c->npartial) b = c->npartial;
     if (c->dim == 3)
         hipLaunchKernelGGL((k_residual<T, 3>), dim3((unsigned)b), dim3(256), 0, c->st, (const T *)c->phi,
