@@ -481,12 +481,24 @@ template <int NV> LSMP_DEV void block_sum_comp(double (&s)[NV], double (&c)[NV])
 // land in acc of thread 0.  Fixed order: the result does not depend on timing.
 LSMP_DEV void sum_partials3(const double *__restrict__ partial, int nblk, double (&acc)[3])
 {
+    // four slots' loads in flight, then their additions in slot order
+    constexpr int AHEAD = 4;
     double cmp[3] = {0.0, 0.0, 0.0};
     acc[0] = acc[1] = acc[2] = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        csum_add(acc[0], cmp[0], partial[3 * b + 0]);
-        csum_add(acc[1], cmp[1], partial[3 * b + 1]);
-        csum_add(acc[2], cmp[2], partial[3 * b + 2]);
+    const int bd = blockDim.x;
+    for (int b0 = threadIdx.x; b0 < nblk; b0 += AHEAD * bd) {
+        double v[AHEAD][3];
+#pragma unroll
+        for (int u = 0; u < AHEAD; ++u) {
+            const int b = b0 + u * bd;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v[u][q] = b < nblk ? partial[3 * b + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < AHEAD; ++u)
+            if (b0 + u * bd < nblk)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) csum_add(acc[q], cmp[q], v[u][q]);
     }
     block_sum_comp<3>(acc, cmp);
 }
