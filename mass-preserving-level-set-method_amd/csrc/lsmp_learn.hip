@@ -136,15 +136,31 @@ LSMP_DEV bool band_flag(const T *__restrict__ phi, size_t n, T thr, size_t q)
     return q < n && fabs(phi[q]) < thr;
 }
 
+// With S0 given, the pass also stores the sign field S0 = 2 H(phi) - 1 of
+// every cell (k_sign_init's values), saving that kernel's read of phi.
 template <typename T>
 __global__ __launch_bounds__(256) void k_band_count(const T *__restrict__ phi, size_t n, T thr,
-                                                    unsigned *__restrict__ cnt)
+                                                    unsigned *__restrict__ cnt, T *__restrict__ S0, T ls_wid)
 {
     __shared__ unsigned wsum[4];
     const size_t q0 = (size_t)blockIdx.x * BAND_CHUNK + threadIdx.x;
     bool f[BAND_ROWS];
+    if (S0) {
 #pragma unroll
-    for (int r = 0; r < BAND_ROWS; ++r) f[r] = band_flag(phi, n, thr, q0 + (size_t)r * 256);
+        for (int r = 0; r < BAND_ROWS; ++r) {
+            const size_t q = q0 + (size_t)r * 256;
+            const T p = q < n ? phi[q] : T(0);
+            f[r] = q < n && fabs(p) < thr;
+            if (q < n) {
+                T H, D;
+                ls_funs(p, ls_wid, H, D);
+                S0[q] = T(2) * H - T(1);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < BAND_ROWS; ++r) f[r] = band_flag(phi, n, thr, q0 + (size_t)r * 256);
+    }
     unsigned c = 0;
 #pragma unroll
     for (int r = 0; r < BAND_ROWS; ++r) c += (unsigned)__popcll(__ballot(f[r]));
@@ -764,14 +780,16 @@ int ensure_band_list(lsmp_ctx *c)
 }
 
 // Queues the ordered list of the cells with |phi| < thr; *count_dev is where
-// its length lands in device memory.
-template <typename T> int queue_band(lsmp_ctx *c, const T *phi, double thr, const unsigned **count_dev)
+// its length lands in device memory.  S0, when given, receives 2 H(phi) - 1.
+template <typename T>
+int queue_band(lsmp_ctx *c, const T *phi, double thr, const unsigned **count_dev, T *S0 = nullptr)
 {
     int rc = ensure_band_list(c);
     if (rc) return rc;
     const int nblk = (int)((c->ncell + BAND_CHUNK - 1) / BAND_CHUNK);
     unsigned *cnt = c->band_count, *off = c->band_count + nblk;
-    hipLaunchKernelGGL((k_band_count<T>), dim3(nblk), dim3(256), 0, c->st, phi, c->ncell, (T)thr, cnt);
+    hipLaunchKernelGGL((k_band_count<T>), dim3(nblk), dim3(256), 0, c->st, phi, c->ncell, (T)thr, cnt, S0,
+                       (T)c->p.ls_wid);
     hipLaunchKernelGGL(k_band_scan, dim3(1), dim3(1024), 0, c->st, cnt, nblk, off);
     hipLaunchKernelGGL((k_band_write<T>), dim3(nblk), dim3(256), 0, c->st, phi, c->ncell, (T)thr, off,
                        c->band_list);
@@ -856,12 +874,13 @@ template <typename T> int residual(lsmp_ctx *c, double band_cells, double *res)
 
 // phi <- network(phi) on |phi| < band_cells dx (phi unchanged elsewhere), in
 // place: the network's values go to band_val, then onto phi.  No host round
-// trip unless the band's length is asked for.
-template <typename T> int apply(lsmp_ctx *c, lsmp_net *net, double band_cells, int *n_band)
+// trip unless the band's length is asked for.  S0, when given, receives the
+// sign field 2 H(phi) - 1 of the input (from the band list's counting pass).
+template <typename T> int apply(lsmp_ctx *c, lsmp_net *net, double band_cells, int *n_band, T *S0 = nullptr)
 {
     T *phi = (T *)c->phi;
     const unsigned *nb_dev = nullptr;
-    int rc = queue_band<T>(c, phi, band_cells * c->p.dx, &nb_dev);
+    int rc = queue_band<T>(c, phi, band_cells * c->p.dx, &nb_dev, S0);
     if (rc) return rc;
     T *val = (T *)c->band_val;   // allocated with the list on the first call
     // persistent workgroups, two per CU (the weights fill a third of the LDS)
@@ -902,14 +921,18 @@ int reinit_learned(lsmp_ctx *c, lsmp_net *net, double band_cells, int sweeps, do
                    double *res_out, int *fallback, int *n_pseudo)
 {
     const size_t n = c->ncell;
-    T *phi = (T *)c->phi, *backup = (T *)c->R;
-    // the sign of the field before the correction, as level_set_redis_init sets it
-    launch_sign_init<T>(c->st, phi, (T *)c->S0, n, (T)c->p.ls_wid, 0);
-    NET_HIP(hipGetLastError());
+    T *phi = (T *)c->phi, *backup = (T *)c->R, *S0 = (T *)c->S0;
+    // the sign of the field before the correction, as level_set_redis_init
+    // sets it: stored by the network pass's band counting (which reads phi
+    // before any change), or by its own kernel in the control without the
+    // network (net == nullptr)
+    if (!net) {
+        launch_sign_init<T>(c->st, phi, S0, n, (T)c->p.ls_wid, 0);
+        NET_HIP(hipGetLastError());
+    }
     if (mode == LSMP_FALLBACK_RESTART)
         NET_HIP(hipMemcpyAsync(backup, phi, n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
-    // net == nullptr: the control without the network pass
-    int rc = net ? apply<T>(c, net, band_cells, nullptr) : LSMP_OK;
+    int rc = net ? apply<T>(c, net, band_cells, nullptr, S0) : LSMP_OK;
     if (rc) return rc;
     double err = 0.0;
     int np = 0;
