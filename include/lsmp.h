@@ -309,7 +309,9 @@ int lsmp_net_clear_samples(lsmp_net_handle net);
 /* phi <- phi + dx * net(stencil) * clamp((|phi| - ls_wid) / ls_wid, 0, 1) on
  * |phi| < band_cells dx, nothing else (the network pass of
  * lsmp_reinit_learned on its own).  The network leaves the interface region
- * |phi| <= ls_wid to the plain sweeps and the mass correction. */
+ * |phi| <= ls_wid to the plain sweeps and the mass correction.  *n_band
+ * receives the number of band cells; n_band may be NULL, and the call then
+ * only queues its kernels (no host round trip). */
 int lsmp_net_apply(lsmp_net_handle net, lsmp_handle h, double band_cells, int *n_band);
 /* steps Adam steps (lr, beta 0.9/0.999, eps 1e-8) on minibatches of `batch`
  * pairs drawn with replacement, loss = mean of w (o - y)^2 with
