@@ -79,4 +79,12 @@ struct lsmp_ctx {
 
 // Plain reinitialisation (lsmp_host.hip), also used by the learned path.
 int lsmp_reinit_run(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, double *err, int *n);
+// The same run queued without its closing read of the control block, when its
+// pseudo-steps fit one queued chunk (*launched > 0; else it ran as
+// lsmp_reinit_run does and *launched = 0).  While it is pending, the result is
+// in c->phiB if (launched - rctl[2]) is odd, else in c->phi;
+// lsmp_reinit_finish takes a host copy of rctl[0..3] read after the queued
+// work and does what lsmp_reinit_run does after its last read.
+int lsmp_reinit_queue(lsmp_ctx *c, int btn, int max_iter, bool init_s0, int *launched, double *err, int *n);
+int lsmp_reinit_finish(lsmp_ctx *c, int launched, const unsigned long long *ctl4, double *err, int *n);
 lsmp::RedisArgs lsmp_redis_args(const lsmp_ctx *c);

@@ -359,8 +359,26 @@ RedisArgs redis_args(const lsmp_ctx *c)
     return ra;
 }
 
+int reinit_finish(lsmp_ctx *c, int launched, const unsigned long long *hc, double *red_error, int *n_pseudo)
+{
+    // each queued step that the stop flag skipped still exchanged phi and phiB
+    const int iter = (int)hc[2];
+    if ((launched - iter) % 2) std::swap(c->phi, c->phiB);
+    double error;
+    std::memcpy(&error, &hc[3], sizeof error);
+    if (std::isnan(error)) return LSMP_E_NAN;
+    c->stats.last_redis_iters = iter;
+    c->stats.last_red_error = error;
+    if (red_error) *red_error = error;
+    if (n_pseudo) *n_pseudo = iter;
+    return LSMP_OK;
+}
+
+// queued (non-null): queue the steps without the closing read when they fit
+// one chunk, *queued = the steps launched (0 when the run went the usual way)
 template <typename T>
-int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, double *red_error, int *n_pseudo)
+int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, double *red_error, int *n_pseudo,
+           int *queued = nullptr)
 {
     const RedisArgs ra = redis_args(c);
     const size_t n = c->ncell;
@@ -421,6 +439,8 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
     launch_redis_ctl(c->st, ctl, 0);
     int launched = 0;
     unsigned long long hc[4] = {0, 0, 0, 0};
+    const bool defer = queued && budget <= CHUNK;
+    if (queued) *queued = 0;
     while (launched < budget) {
         const int k = std::min(CHUNK, budget - launched);
         for (int q = 0; q < k; ++q) {
@@ -453,22 +473,16 @@ int reinit(lsmp_ctx *c, int btn, double timestop, int max_iter, bool init_s0, do
             c->phi = b;
         }
         launched += k;
+        if (defer) {
+            *queued = launched;
+            return LSMP_OK;
+        }
         LSMP_HIP(hipMemcpyAsync(c->hbuf, ctl, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
         LSMP_HIP(hipStreamSynchronize(c->st));
         std::memcpy(hc, c->hbuf, sizeof hc);
         if (hc[1]) break;
     }
-    // each queued step that the stop flag skipped still exchanged phi and phiB
-    const int iter = (int)hc[2];
-    if ((launched - iter) % 2) std::swap(c->phi, c->phiB);
-    double error;
-    std::memcpy(&error, &hc[3], sizeof error);
-    if (std::isnan(error)) return LSMP_E_NAN;
-    c->stats.last_redis_iters = iter;
-    c->stats.last_red_error = error;
-    if (red_error) *red_error = error;
-    if (n_pseudo) *n_pseudo = iter;
-    return LSMP_OK;
+    return reinit_finish(c, launched, hc, red_error, n_pseudo);
 }
 
 CcdArgs ccd_args(const lsmp_ctx *c)
@@ -800,6 +814,17 @@ int lsmp_reinit_run(lsmp_ctx *c, int btn, double timestop, int max_iter, bool in
 {
     return c->fp32 ? reinit<float>(c, btn, timestop, max_iter, init_s0, err, n)
                    : reinit<double>(c, btn, timestop, max_iter, init_s0, err, n);
+}
+
+int lsmp_reinit_queue(lsmp_ctx *c, int btn, int max_iter, bool init_s0, int *launched, double *err, int *n)
+{
+    return c->fp32 ? reinit<float>(c, btn, -1.0, max_iter, init_s0, err, n, launched)
+                   : reinit<double>(c, btn, -1.0, max_iter, init_s0, err, n, launched);
+}
+
+int lsmp_reinit_finish(lsmp_ctx *c, int launched, const unsigned long long *ctl4, double *err, int *n)
+{
+    return reinit_finish(c, launched, ctl4, err, n);
 }
 
 RedisArgs lsmp_redis_args(const lsmp_ctx *c) { return redis_args(c); }

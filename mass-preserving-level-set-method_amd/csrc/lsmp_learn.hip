@@ -254,11 +254,16 @@ __global__ void k_collect(const T *__restrict__ pre, const T *__restrict__ post,
     Y[s] = (float)(post[idx] - pc) * g.inv_h;
 }
 
-// band-averaged | |grad phi| - 1 | with the Godunov gradient signed by S0
+// band-averaged | |grad phi| - 1 | with the Godunov gradient signed by S0.
+// With ctl given, the field is the result of `launched` queued pseudo-steps
+// whose stop flag may have skipped some: phi_alt when launched - ctl[2] (the
+// steps run) is odd, phi_in otherwise.
 template <typename T, int DIM>
-__global__ void k_residual(const T *__restrict__ phi, const T *__restrict__ S0, RedisArgs ra, double thr,
-                           double *__restrict__ partial)
+__global__ void k_residual(const T *__restrict__ phi_in, const T *__restrict__ S0, RedisArgs ra, double thr,
+                           double *__restrict__ partial, const T *__restrict__ phi_alt,
+                           const unsigned long long *__restrict__ ctl, int launched)
 {
+    const T *__restrict__ phi = ctl && ((launched - (int)ctl[2]) & 1) ? phi_alt : phi_in;
     const int nx = ra.nx, ny = ra.ny, nz = ra.nz;
     double acc[3] = {0.0, 0.0, 0.0};
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < (size_t)nx * ny * nz;
@@ -850,25 +855,40 @@ template <typename T> int collect(lsmp_net *net, lsmp_ctx *c, double band_cells,
     return LSMP_OK;
 }
 
-template <typename T> int residual(lsmp_ctx *c, double band_cells, double *res)
+// pending > 0: the pseudo-steps of lsmp_reinit_queue are still open; the
+// residual reads the buffer that holds their result, and its read-back also
+// brings the control block, which closes them (err, np as lsmp_reinit_run).
+template <typename T>
+int residual(lsmp_ctx *c, double band_cells, double *res, int pending = 0, double *err = nullptr, int *np = nullptr)
 {
     const RedisArgs ra = lsmp_redis_args(c);
     // as many workgroups as there are partial slots (10,240 at 256^3): the
     // grid-stride loop is load-latency bound and needs the waves
     size_t b = (c->ncell + 255) / 256;
     if ((int)b > c->npartial) b = c->npartial;
+    const unsigned long long *ctl = pending > 0 ? c->rctl : nullptr;
     if (c->dim == 3)
         hipLaunchKernelGGL((k_residual<T, 3>), dim3((unsigned)b), dim3(256), 0, c->st, (const T *)c->phi,
-                           (const T *)c->S0, ra, band_cells * c->p.dx, c->partial);
+                           (const T *)c->S0, ra, band_cells * c->p.dx, c->partial, (const T *)c->phiB, ctl,
+                           pending);
     else
         hipLaunchKernelGGL((k_residual<T, 2>), dim3((unsigned)b), dim3(256), 0, c->st, (const T *)c->phi,
-                           (const T *)c->S0, ra, band_cells * c->p.dx, c->partial);
+                           (const T *)c->S0, ra, band_cells * c->p.dx, c->partial, (const T *)c->phiB, ctl,
+                           pending);
     NET_HIP(hipGetLastError());
     launch_finalize3(c->st, c->partial, (int)b, 0.0, c->scal);
     NET_HIP(hipGetLastError());
     NET_HIP(hipMemcpyAsync(c->hbuf, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    if (ctl)
+        NET_HIP(hipMemcpyAsync(c->hbuf + 2, ctl, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
     NET_HIP(hipStreamSynchronize(c->st));
     *res = c->hbuf[1] > 0 ? c->hbuf[0] / c->hbuf[1] : 0.0;
+    if (ctl) {
+        unsigned long long hc[4];
+        std::memcpy(hc, c->hbuf + 2, sizeof hc);
+        const int rc = lsmp_reinit_finish(c, pending, hc, err, np);
+        if (rc) return rc;
+    }
     return LSMP_OK;
 }
 
@@ -936,10 +956,12 @@ int reinit_learned(lsmp_ctx *c, lsmp_net *net, double band_cells, int sweeps, do
     if (rc) return rc;
     double err = 0.0;
     int np = 0;
-    rc = lsmp_reinit_run(c, 1, -1.0, sweeps, false, &err, &np);
+    int queued = 0;
+    rc = lsmp_reinit_queue(c, 1, sweeps, false, &queued, &err, &np);
     if (rc) return rc;
     double res = 0.0;
-    rc = residual<T>(c, res_band, &res);
+    // the residual's read-back also closes the queued sweeps
+    rc = residual<T>(c, res_band, &res, queued, &err, &np);
     if (rc) return rc;
     *fallback = 0;
     if (!(res <= tol)) {
