@@ -984,14 +984,14 @@ int reinit_learned(lsmp_ctx *c, lsmp_net *net, double band_cells, int sweeps, do
             int budget = plain_budget(c) - np;
             bool progress = true;
             while (budget > 0 && !(res <= tol) && err > 1.0e-6 && progress) {
-                int np2 = 0;
-                rc = lsmp_reinit_run(c, 1, -1.0, sweeps < budget ? sweeps : budget, false, &err, &np2);
+                int np2 = 0, queued2 = 0;
+                rc = lsmp_reinit_queue(c, 1, sweeps < budget ? sweeps : budget, false, &queued2, &err, &np2);
+                if (rc) return rc;
+                const double prev = res;
+                rc = residual<T>(c, res_band, &res, queued2, &err, &np2);
                 if (rc) return rc;
                 np += np2;
                 budget -= np2;
-                const double prev = res;
-                rc = residual<T>(c, res_band, &res);
-                if (rc) return rc;
                 if (mode == LSMP_FALLBACK_CONTINUE_RELATIVE) progress = res < 0.9 * prev;
             }
         }
