@@ -164,6 +164,34 @@ def test_network_pass_matches_oracle(precision):
     assert np.array_equal(out[~inband], pre[~inband])
 
 
+def test_network_pass_without_band_count():
+    """lsmp_net_apply with n_band = NULL only queues the pass (the band's
+    length stays on the device): the same field bit for bit as the call that
+    reads the length back, on fresh solvers (the pass's buffers are allocated
+    on the first call)."""
+    lsmp = load_lsmp()
+    n = 32
+    P = random_params(np.random.default_rng(5))
+    P[M.OFF["w3"]:] *= 0.1
+    box = sphere_box(n, r=0.25, warp=0.1)
+    outs = []
+    with lsmp.Net() as net:
+        net.set_params(P)
+        for read_count in (True, False):
+            with lsmp.Solver(lsmp.make_params(n, dim=3, precision=1)) as s:
+                s.upload_phi(box)
+                pre = s.download_phi()
+                if read_count:
+                    nb = net.apply(s, 6.0)
+                else:
+                    assert net.lib.lsmp_net_apply(net.h, s.h, 6.0, None) == 0
+                outs.append(s.download_phi())
+    say(f"net apply without the count: band {nb}")
+    assert nb > 0
+    assert np.array_equal(outs[0], outs[1])
+    assert not np.array_equal(outs[0], pre)   # the pass moved some band cells
+
+
 def test_collected_pairs_match_plain_reinit():
     """lsmp_net_collect stores the features of the pre-reinit band and the
     plain method's correction: the loss of an lr = 0 step over them equals the
